@@ -1,0 +1,101 @@
+// arbo_device.h — HBM layout of a batch of simulations, as the kernels see it.
+//
+// Every simulation of the batch is flattened into global arrays:
+//   human words   u32  h_word[]  (4-element aligned per community)
+//   human next    u16  h_next[]
+//   human birth   i32  h_birth[]   (cold: written at rebirth only)
+//   mosquito word u16  m_word[]  (8-element aligned per community)
+//   mosquito next u16  m_next[]
+// A community is the unit of work of both day kernels; its descriptor gives
+// the offsets of its members and of its CSR row and in-edges.
+#pragma once
+#include <stdint.h>
+
+namespace arbo {
+
+constexpr uint32_t kBlock = 256;          // threads per workgroup (4 wavefronts)
+constexpr uint32_t kCntPerComm = 72;      // event counts: [dir 2][serotype 4][row slot 9]
+constexpr uint32_t kHumanAlign = 4;       // humans per 16-byte vector
+constexpr uint32_t kMosqAlign = 8;        // mosquito slots per 16-byte vector
+
+struct CommDesc {
+    uint64_t h_off;       // global offset of the first human
+    uint64_t m_off;       // global offset of the first mosquito slot
+    uint32_t sim;         // simulation index
+    uint32_t c;           // community index within the simulation
+    uint32_t n;           // humans
+    uint32_t cap;         // mosquito slots
+    uint32_t h_base;      // simulation-local index of the first human (Philox entity)
+    uint32_t m_base;      // simulation-local index of the first slot (Philox entity)
+    uint32_t row_off;     // global offset of the community's CSR row
+    uint32_t row_len;
+    uint32_t in_off;      // global offset of its in-edges
+    uint32_t in_len;
+    float base;           // m_ratio * n (f32)
+    uint32_t pad;
+};
+
+struct SimDesc {
+    uint32_t k0, k1;              // Philox key = 64-bit seed
+    uint32_t C, S;                // communities, serotypes
+    uint32_t comm_off;            // global index of community 0
+    float beta_hm, beta_mh, omega, recovery, eip, intro;
+    uint32_t iip_min, iip_span;   // incubation uniform on [iip_min, iip_min + iip_span)
+    uint32_t death_h_off;         // offsets into the shared table pools
+    uint32_t death_m_off;
+    uint32_t init_h_off;
+    uint32_t init_m_off;
+    uint32_t dens_off;
+    uint32_t immune_off;          // [C*S] thresholds
+    uint32_t seed_h, seed_m;
+    uint32_t death_m_max;         // largest death age the mosquito table can return
+    uint32_t pad;
+    uint64_t series_off;          // this simulation's block in the series array
+};
+
+// Series block of one simulation with room for T days (T+1 rows):
+//   n_count[T+1] m_count[T+1] n_inf[(T+1)S] n_inc[(T+1)S] n_one[(T+1)S]
+struct SeriesIdx {
+    uint64_t n_count, m_count, n_inf, n_inc, n_one;
+};
+
+__host__ __device__ inline SeriesIdx series_index(uint64_t off, uint32_t T, uint32_t S) {
+    SeriesIdx x;
+    const uint64_t R = (uint64_t)T + 1;
+    x.n_count = off;
+    x.m_count = off + R;
+    x.n_inf = off + 2 * R;
+    x.n_inc = off + 2 * R + R * S;
+    x.n_one = off + 2 * R + 2 * R * S;
+    return x;
+}
+
+__host__ __device__ inline uint64_t series_words(uint32_t T, uint32_t S) {
+    return ((uint64_t)T + 1) * (2 + 3 * (uint64_t)S);
+}
+
+struct EngineView {
+    const CommDesc* comm;
+    const SimDesc* sims;
+    uint32_t* h_word;
+    uint16_t* h_next;
+    int32_t* h_birth;
+    uint16_t* m_word;
+    uint16_t* m_next;
+    uint32_t* cnt;                 // [G][72]
+    unsigned long long* lam;       // [2 parity][n_sims][8]  long-distance rate, 2^-24 fixed point
+    uint32_t* series;
+    const uint32_t* csr_to;        // simulation-local destination
+    const uint32_t* csr_thr;       // bits 0..24 threshold, 27..31 alias slot
+    const uint32_t* in_edge;       // global source community
+    const uint32_t* in_slot;       // slot of the destination in the source's row
+    const uint32_t* tables;        // death / initial-age / immunity pools
+    const float* dens;
+    const uint32_t* day;           // device day counter (graph replay)
+    uint32_t n_sims;
+    uint32_t n_comm;
+    uint32_t max_days;
+    uint32_t pad;
+};
+
+}  // namespace arbo

@@ -1,0 +1,226 @@
+// arbo_tables.cpp — host tables of one simulation (see arbo_tables.h).
+//
+// Lifetimes: the reference draws PLE ~ U(0,1] at birth and kills on the first
+// day with F(age) >= PLE (ndemographic.cu:62, mdemographic.cu:72).  Here F is
+// tabulated once per parameter set as 24-bit thresholds K[a] = floor(F(a) 2^24)
+// and the kernels search K for a 24-bit PLE: an exact integer version of the
+// same law.  Initial ages follow the cumulative survival sums of
+// demographicintial.cu:48-83.  The network is the reference's non-wrapping
+// lattice (network.cpp:31-41) with rows in ascending column order
+// (network.cpp:95-113), generalised to Moore neighbourhoods, a self entry and
+// exp(-d/scale) weights; destinations are drawn from Vose alias tables.
+// All floating point is the deterministic double arithmetic of arbo_rng.h.
+#include "arbo_tables.h"
+
+#include <algorithm>
+#include <cmath>
+
+#include "arbo_rng.h"
+
+namespace arbo {
+
+uint32_t fixed24(double f) {
+    const double v = floor(f * 16777216.0);
+    if (!(v > 0.0)) return 0u;
+    if (v >= 16777216.0) return kOne24;
+    return (uint32_t)v;
+}
+
+static double powd(double x, double y) { return x > 0.0 ? dexp(y * dlog(x)) : 0.0; }
+
+// ndemographic.cu:13-29: F(a) = 1 - exp(-[(a b)^alpha + 1{a >= L} ((a-L)/d)^c]).
+static double human_cdf(const ArboParams& p, double a) {
+    double haz = powd(a * (double)p.h_infant_scale, (double)p.h_infant_shape);
+    if (a >= (double)p.h_loc) haz = haz + powd((a - (double)p.h_loc) / (double)p.h_scale, (double)p.h_shape);
+    return 1.0 - dexp(-haz);
+}
+
+// mdemographic.cu:13-17: F(a) = 1 - exp(-(a/scale)^shape).
+static double mosq_cdf(const ArboParams& p, double a) {
+    return 1.0 - dexp(-powd(a / (double)p.m_scale, (double)p.m_shape));
+}
+
+template <class Cdf>
+static void threshold_table(uint32_t n, Cdf cdf, std::vector<uint32_t>& K) {
+    K.resize(n);
+    uint32_t prev = 0;
+    for (uint32_t a = 0; a < n; ++a) {
+        uint32_t k = fixed24(cdf((double)a));
+        if (k < prev) k = prev;
+        K[a] = k;
+        prev = k;
+    }
+}
+
+template <class Cdf>
+static void initial_age_table(uint32_t n, uint32_t step, Cdf cdf, std::vector<uint32_t>& T) {
+    std::vector<double> cum(n);
+    double acc = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        acc = acc + (1.0 - cdf((double)(i * step)));
+        cum[i] = acc;
+    }
+    T.resize(n);
+    for (uint32_t i = 0; i < n; ++i) T[i] = fixed24(cum[i] / cum[n - 1]);
+    T[n - 1] = kOne24;
+    for (uint32_t i = 1; i < n; ++i) T[i] = std::max(T[i], T[i - 1]);
+}
+
+std::string validate(const ArboShape& sh, const ArboParams& p) {
+    if (p.n_serotypes < 1 || p.n_serotypes > ARBO_MAX_SEROTYPES) return "n_serotypes must be 1..4";
+    if (sh.grid_w < 1 || sh.grid_h < 1 || !sh.humans) return "empty lattice";
+    if ((uint64_t)sh.grid_w * sh.grid_h > (1u << 24)) return "too many communities";
+    if (p.neighbourhood != 4 && p.neighbourhood != 8) return "neighbourhood must be 4 or 8";
+    if (!(p.beta_hm >= 0.0f) || !(p.beta_mh >= 0.0f)) return "beta must be >= 0";
+    if (!(p.long_distance >= 0.0f && p.long_distance <= 1.0f)) return "long_distance must be in [0,1]";
+    if (!(p.season_amp >= 0.0f && p.season_amp <= 1.0f)) return "season_amp must be in [0,1]";
+    if (!(p.recovery >= 0.0f && p.recovery < 1000.0f) || !(p.eip >= 0.0f && p.eip < 1000.0f)) return "recovery/eip out of range";
+    if (p.iip_max < p.iip_min || p.iip_max > 1000) return "bad incubation range";
+    if (!(p.m_scale > 0.0f) || !(p.m_shape > 0.0f) || !(p.h_scale > 0.0f) || !(p.h_shape > 0.0f)) return "bad lifespan parameters";
+    if (!(p.m_ratio >= 0.0f)) return "m_ratio must be >= 0";
+    if (!(p.external_infection >= 0.0f)) return "external_infection must be >= 0";
+    if (!(p.seed_frac_h >= 0.0f && p.seed_frac_h <= 1.0f) || !(p.seed_frac_m >= 0.0f && p.seed_frac_m <= 1.0f)) return "seed fractions must be in [0,1]";
+    if (p.immunity_mode > 1 || !(p.sero_prev >= 0.0f && p.sero_prev <= 1.0f)) return "bad immunity settings";
+    if (!(p.dispersal_scale >= 0.0f)) return "dispersal_scale must be >= 0";
+    const uint64_t C = (uint64_t)sh.grid_w * sh.grid_h;
+    uint64_t nh = 0, nm = 0;
+    for (uint64_t c = 0; c < C; ++c) {
+        const uint32_t n = sh.humans[c];
+        if (n < 1 || n > ARBO_MAX_COMMUNITY_HUMANS) return "humans per community out of range";
+        const double cap = sh.mosq_slots ? (double)sh.mosq_slots[c]
+                                         : ceil((double)p.m_ratio * (1.0 + (double)p.season_amp) * (double)n);
+        if (cap > (double)ARBO_MAX_COMMUNITY_SLOTS) return "mosquito slots per community out of range";
+        nh += n;
+        nm += (uint64_t)cap;
+    }
+    if (nh >= (1ull << 32) || nm >= (1ull << 32)) return "simulation too large";
+    return std::string();
+}
+
+static void build_network(const ArboParams& p, SimTables& t) {
+    t.row_ptr.assign(1, 0u);
+    t.to.clear();
+    t.thr.clear();
+    t.alias.clear();
+    for (uint32_t y = 0; y < t.H; ++y) {
+        for (uint32_t x = 0; x < t.W; ++x) {
+            uint32_t col[kMaxRow];
+            double w[kMaxRow];
+            uint32_t n = 0;
+            for (int dy = -1; dy <= 1; ++dy) {
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int xx = (int)x + dx, yy = (int)y + dy;
+                    if (xx < 0 || yy < 0 || xx >= (int)t.W || yy >= (int)t.H) continue;
+                    const int m = (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy);
+                    if (m == 0 && !p.include_self) continue;
+                    if (m == 2 && p.neighbourhood == 4) continue;
+                    const double dist = (m == 0) ? 0.0 : ((m == 1) ? 1.0 : 1.4142135623730951);
+                    col[n] = (uint32_t)yy * t.W + (uint32_t)xx;
+                    w[n] = (p.dispersal_scale > 0.0f) ? dexp(-dist / (double)p.dispersal_scale) : 1.0;
+                    ++n;
+                }
+            }
+            if (n == 0) {  // isolated community: local events stay at home
+                col[0] = y * t.W + x;
+                w[0] = 1.0;
+                n = 1;
+            }
+            // Vose's alias method: scaled weights q = n w / sum(w); a stack of
+            // under-full and of over-full columns, paired until one is empty.
+            double sum = 0.0;
+            for (uint32_t i = 0; i < n; ++i) sum = sum + w[i];
+            double q[kMaxRow], prob[kMaxRow];
+            uint32_t al[kMaxRow], small[kMaxRow], large[kMaxRow];
+            uint32_t ns = 0, nl = 0;
+            for (uint32_t i = 0; i < n; ++i) {
+                q[i] = w[i] / sum * (double)n;
+                prob[i] = 1.0;
+                al[i] = i;
+                if (q[i] < 1.0) small[ns++] = i; else large[nl++] = i;
+            }
+            while (ns > 0 && nl > 0) {
+                const uint32_t l = small[--ns];
+                const uint32_t g = large[--nl];
+                prob[l] = q[l];
+                al[l] = g;
+                q[g] = (q[g] + q[l]) - 1.0;
+                if (q[g] < 1.0) small[ns++] = g; else large[nl++] = g;
+            }
+            for (uint32_t i = 0; i < n; ++i) {
+                t.to.push_back(col[i]);
+                t.thr.push_back(fixed24(prob[i]));
+                t.alias.push_back(al[i]);
+            }
+            t.row_ptr.push_back((uint32_t)t.to.size());
+        }
+    }
+    // transpose: in-edges per destination, sources ascending (a source visits
+    // each destination at most once, and sources are enumerated in order)
+    std::vector<uint32_t> deg(t.C + 1, 0u);
+    for (uint32_t e = 0; e < t.to.size(); ++e) deg[t.to[e] + 1]++;
+    for (uint32_t c = 0; c < t.C; ++c) deg[c + 1] += deg[c];
+    t.in_ptr = deg;
+    t.in_src.assign(t.to.size(), 0u);
+    t.in_slot.assign(t.to.size(), 0u);
+    std::vector<uint32_t> fill(t.in_ptr.begin(), t.in_ptr.end() - 1);
+    for (uint32_t c = 0; c < t.C; ++c) {
+        for (uint32_t e = t.row_ptr[c]; e < t.row_ptr[c + 1]; ++e) {
+            const uint32_t d = t.to[e];
+            t.in_src[fill[d]] = c;
+            t.in_slot[fill[d]] = e - t.row_ptr[c];
+            fill[d]++;
+        }
+    }
+}
+
+void build_tables(const ArboShape& sh, const ArboParams& p, uint64_t seed, uint32_t max_days,
+                  SimTables& t) {
+    t.W = sh.grid_w;
+    t.H = sh.grid_h;
+    t.C = t.W * t.H;
+    t.S = p.n_serotypes;
+    t.humans.assign(sh.humans, sh.humans + t.C);
+    t.slots.resize(t.C);
+    t.base.resize(t.C);
+    t.n_humans = 0;
+    t.n_slots = 0;
+    for (uint32_t c = 0; c < t.C; ++c) {
+        t.slots[c] = sh.mosq_slots ? sh.mosq_slots[c]
+                                   : (uint32_t)ceil((double)p.m_ratio * (1.0 + (double)p.season_amp) * (double)t.humans[c]);
+        t.base[c] = p.m_ratio * (float)t.humans[c];
+        t.n_humans += t.humans[c];
+        t.n_slots += t.slots[c];
+    }
+    threshold_table(kHumanAgeTable, [&](double a) { return human_cdf(p, a); }, t.death_h);
+    threshold_table(kMosqAgeTable, [&](double a) { return mosq_cdf(p, a); }, t.death_m);
+    initial_age_table(kHumanYears, 365, [&](double a) { return human_cdf(p, a); }, t.init_h);
+    initial_age_table(kMosqInitDays, 1, [&](double a) { return mosq_cdf(p, a); }, t.init_m);
+    t.dens.resize((size_t)max_days + 1);
+    for (uint32_t d = 0; d <= max_days; ++d) {
+        const double arg = 6.283185307179586 * (double)d / 365.0 + (double)p.season_phase;
+        t.dens[d] = (float)(1.0 + (double)p.season_amp * dsin(arg));
+    }
+    const Key key{(uint32_t)seed, (uint32_t)(seed >> 32)};
+    t.immune.assign((size_t)t.C * t.S, 0u);
+    for (uint32_t c = 0; c < t.C; ++c) {
+        for (uint32_t s = 0; s < t.S; ++s) {
+            uint32_t thr;
+            if (p.immunity_mode == 0) {
+                thr = fixed24((double)p.sero_prev);
+            } else {
+                // Beta(2,5) as the 2nd order statistic of 6 uniforms
+                const U4 a = draw(key, c, INIT_DAY, ctr_tag(P_IMMUNE, 0, s, 0), 0);
+                const U4 b = draw(key, c, INIT_DAY, ctr_tag(P_IMMUNE, 0, s, 0), 1);
+                uint32_t u[6] = {a.x >> 8, a.y >> 8, a.z >> 8, a.w >> 8, b.x >> 8, b.y >> 8};
+                std::sort(u, u + 6);
+                thr = u[1];
+            }
+            t.immune[(size_t)c * t.S + s] = thr;
+        }
+    }
+    t.seed_h = fixed24((double)p.seed_frac_h);
+    t.seed_m = fixed24((double)p.seed_frac_m);
+    build_network(p, t);
+}
+
+}  // namespace arbo

@@ -1,0 +1,190 @@
+"""Host-side mirror of the reference's simulation driver over the C ABI.
+
+`Engine` wraps one ArboEngine handle: create -> batch_init -> step -> read.
+`simulation()` keeps the shape of the reference's simulation()
+(simulation.cpp:18-191): initialise, run the day loop, return the incidence
+series that MCMC::storeIncidence (MCMC.cpp:190-213) and writeIncidence
+(writeoutput.cpp:13-98) consume.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+
+
+class ArboError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"arbo error {code}: {msg}")
+        self.code = code
+
+
+@dataclass
+class SimSpec:
+    """One simulation: lattice, humans per community, parameters and seed."""
+    grid_w: int
+    grid_h: int
+    humans: np.ndarray            # uint32 [grid_w*grid_h]
+    params: abi.Params
+    seed: int
+    mosq_slots: np.ndarray | None = None
+
+    @property
+    def communities(self) -> int:
+        return self.grid_w * self.grid_h
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Engine:
+    """An engine over a library exporting include/arbo_c.h (the HIP engine by
+    default)."""
+
+    def __init__(self, lib: C.CDLL | None = None, device: int = 0, flags: int = 0, threads: int = 0):
+        self.lib = lib if lib is not None else abi.load_hip_library()
+        self._h = abi.EnginePtr()
+        desc = abi.EngineDesc(abi.ABI_VERSION, device, flags, threads)
+        rc = self.lib.arbo_create(C.byref(desc), C.byref(self._h))
+        if rc != 0:
+            raise ArboError(rc, "arbo_create failed")
+        self.specs: list[SimSpec] = []
+        self.max_days = 0
+
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self.lib.arbo_last_error(self._h)
+            raise ArboError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if self._h:
+            self.lib.arbo_destroy(self._h)
+            self._h = abi.EnginePtr()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init(self, specs: list[SimSpec], max_days: int):
+        n = len(specs)
+        shapes = (abi.Shape * n)()
+        params = (abi.Params * n)()
+        seeds = (C.c_uint64 * n)()
+        keep = []
+        for i, s in enumerate(specs):
+            hum = np.ascontiguousarray(s.humans, dtype=np.uint32)
+            assert hum.size == s.communities
+            keep.append(hum)
+            shapes[i].grid_w = s.grid_w
+            shapes[i].grid_h = s.grid_h
+            shapes[i].humans = _ptr(hum, C.c_uint32)
+            if s.mosq_slots is not None:
+                ms = np.ascontiguousarray(s.mosq_slots, dtype=np.uint32)
+                keep.append(ms)
+                shapes[i].mosq_slots = _ptr(ms, C.c_uint32)
+            params[i] = s.params
+            seeds[i] = s.seed & 0xFFFFFFFFFFFFFFFF
+        self._check(self.lib.arbo_batch_init(self._h, n, shapes, params, seeds, max_days))
+        self.specs = list(specs)
+        self.max_days = max_days
+
+    def step(self, n_days: int = 1):
+        self._check(self.lib.arbo_batch_step(self._h, n_days))
+
+    def sync(self):
+        self._check(self.lib.arbo_sync(self._h))
+
+    def set_day(self, day: int):
+        self._check(self.lib.arbo_set_day(self._h, day))
+
+    @property
+    def day(self) -> int:
+        d = C.c_uint32()
+        self._check(self.lib.arbo_days_done(self._h, C.byref(d)))
+        return d.value
+
+    def sizes(self, sim: int):
+        c, s = C.c_uint32(), C.c_uint32()
+        h, m = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.arbo_sim_sizes(self._h, sim, C.byref(c), C.byref(h), C.byref(m), C.byref(s)))
+        return c.value, h.value, m.value, s.value
+
+    def series(self, sim: int) -> dict:
+        _, _, _, S = self.sizes(sim)
+        rows = self.day + 1
+        out = {
+            "n_count": np.zeros(rows, np.uint32),
+            "m_count": np.zeros(rows, np.uint32),
+            "n_infected": np.zeros((rows, S), np.uint32),
+            "n_incidence": np.zeros((rows, S), np.uint32),
+            "n_one_subpop": np.zeros((rows, S), np.uint32),
+        }
+        ser = abi.Series(*(_ptr(out[k], C.c_uint32) for k in
+                           ("n_count", "m_count", "n_infected", "n_incidence", "n_one_subpop")))
+        self._check(self.lib.arbo_read_series(self._h, sim, C.byref(ser)))
+        return out
+
+    def state(self, sim: int) -> dict:
+        _, nh, nm, _ = self.sizes(sim)
+        out = {
+            "h_word": np.zeros(nh, np.uint32),
+            "h_next": np.zeros(nh, np.uint16),
+            "h_birth": np.zeros(nh, np.int32),
+            "m_word": np.zeros(nm, np.uint16),
+            "m_next": np.zeros(nm, np.uint16),
+        }
+        st = abi.State(_ptr(out["h_word"], C.c_uint32), _ptr(out["h_next"], C.c_uint16),
+                       _ptr(out["h_birth"], C.c_int32), _ptr(out["m_word"], C.c_uint16),
+                       _ptr(out["m_next"], C.c_uint16))
+        self._check(self.lib.arbo_read_state(self._h, sim, C.byref(st)))
+        return out
+
+    def write_state(self, sim: int, state: dict):
+        arrs = {k: np.ascontiguousarray(state[k]) for k in ("h_word", "h_next", "h_birth", "m_word", "m_next")}
+        st = abi.State(_ptr(arrs["h_word"], C.c_uint32), _ptr(arrs["h_next"], C.c_uint16),
+                       _ptr(arrs["h_birth"], C.c_int32), _ptr(arrs["m_word"], C.c_uint16),
+                       _ptr(arrs["m_next"], C.c_uint16))
+        self._check(self.lib.arbo_write_state(self._h, sim, C.byref(st)))
+
+    def kernel_time(self, kind: int):
+        ms, n = C.c_double(), C.c_uint64()
+        self._check(self.lib.arbo_kernel_time(self._h, kind, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def reset_timing(self):
+        self._check(self.lib.arbo_reset_timing(self._h))
+
+
+def simulation(spec: SimSpec, max_time: int, lib: C.CDLL | None = None) -> dict:
+    """One stochastic run (simulation.cpp:18-191): t=0 census, max_time days,
+    and the series rows 0..max_time."""
+    with Engine(lib) as eng:
+        eng.init([spec], max_time)
+        eng.step(max_time)
+        return eng.series(0)
+
+
+def weekly_incidence(n_incidence: np.ndarray, max_week: int) -> np.ndarray:
+    """Weekly binning of MCMC::storeIncidence (MCMC.cpp:199-209), summed over
+    serotypes (the reference indexes [7w+t] and is correct only for one
+    serotype)."""
+    daily = n_incidence.sum(axis=1) if n_incidence.ndim == 2 else n_incidence
+    max_time = daily.shape[0] - 1
+    out = np.zeros(max_week, np.int64)
+    for w in range(max_week):
+        for t in range(7):
+            if 7 * w + t <= max_time:
+                out[w] += int(daily[7 * w + t])
+    return out

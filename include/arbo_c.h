@@ -1,0 +1,187 @@
+/* arbo_c.h — C ABI of the MI355X dengue individual-based-model engine.
+ *
+ * One engine owns the device memory for a batch of independent simulations
+ * (each its own lattice of communities, parameters and Philox seed) and
+ * advances all of them together, one day per step.  The same ABI is exported
+ * by the GPU engine (libarbo_hip.so) and, under the same names, by the
+ * serial/OpenMP CPU oracle (test infrastructure only).
+ *
+ * The reference crosses the host/device boundary through the host launchers
+ * called by simulation() once per day (simulation.cpp:115-155):
+ *   demographicInitialization  demographicinitial.h:13-44
+ *   epidemicInitial            epidemicinitial.h:14-34
+ *   nDemographic               demographic.h:17-31
+ *   mDemographic               demographic.h:36-51
+ *   epidemic                   epidemic.h:17-53
+ *   dataCollect                datacollect.h:14-29
+ *   setupCudaRNG               setuprng.h:11-12
+ * and the caller reads d_nIncidence back (MCMC.cpp:190-213,
+ * writeoutput.cpp:13-98).  Each entry point below names the launchers it
+ * replaces.
+ *
+ * Conventions: every function returns 0 on success and a negative ARBO_E_*
+ * code on failure; arbo_last_error() then holds a message.  An engine is
+ * single-threaded (one HIP stream); distinct engines may run concurrently.
+ * Plain pointers and sizes only; all host buffers are owned by the caller.
+ */
+#ifndef ARBO_C_H
+#define ARBO_C_H
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define ARBO_API __attribute__((visibility("default")))
+#else
+#define ARBO_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARBO_ABI_VERSION 1u
+#define ARBO_MAX_SEROTYPES 4u
+#define ARBO_MAX_COMMUNITY_HUMANS 8192u
+#define ARBO_MAX_COMMUNITY_SLOTS 8192u
+
+enum {
+    ARBO_OK = 0,
+    ARBO_E_INVALID = -1,   /* bad argument or parameter */
+    ARBO_E_DEVICE = -2,    /* HIP runtime failure */
+    ARBO_E_ALLOC = -3,     /* device or host allocation failure */
+    ARBO_E_STATE = -4,     /* call out of order (e.g. step before init) */
+    ARBO_E_CAPACITY = -5   /* more days requested than the series can hold */
+};
+
+/* Engine creation options. */
+typedef struct ArboEngineDesc {
+    uint32_t abi_version;  /* must be ARBO_ABI_VERSION */
+    int32_t device;        /* HIP device ordinal (ignored by the oracle) */
+    uint32_t flags;        /* ARBO_F_* */
+    uint32_t threads;      /* oracle only: OpenMP threads (0 = runtime default) */
+} ArboEngineDesc;
+
+#define ARBO_F_GRAPH 1u        /* capture day loops in hipGraphs */
+#define ARBO_F_TIMING 2u       /* time each day kernel with HIP events */
+
+/* Model parameters of one simulation: the fields of SimulationParam that reach
+ * the kernels (SimulationParam.h:16-54) plus the seasonal, incubation and
+ * seeding extensions of the configured workloads. */
+typedef struct ArboParams {
+    uint32_t n_serotypes;       /* 1..4 (C_STRAINS, constant.h:17) */
+    /* transmission (nepidemic.cu:66-70, mepidemic.cu:65-66) */
+    float beta_hm;              /* events per infectious human per day per mosquito/human ratio:
+                                   bitingRate * nmBitingSuccess */
+    float beta_mh;              /* events per infectious mosquito per day: bitingRate * mnBitingSuccess */
+    float long_distance;        /* fraction of events sent to a uniform random community */
+    float external_infection;   /* introductions per day per serotype per simulation */
+    /* dispersal network (network.cpp:31-41) */
+    uint32_t neighbourhood;     /* 4 (von Neumann) or 8 (Moore) */
+    uint32_t include_self;      /* 1: the community itself is a CSR entry */
+    float dispersal_scale;      /* weights exp(-d/scale); 0 = unit weights */
+    /* natural history */
+    uint32_t iip_min, iip_max;  /* human intrinsic incubation, uniform integer days */
+    float recovery;             /* human infectious period in days (stochastic rounding) */
+    float eip;                  /* mosquito extrinsic incubation in days (stochastic rounding) */
+    /* human bi-Weibull lifespan (ndemographic.cu:13-29): F(a) = 1-exp(-[(a*infant_scale)^infant_shape
+       + 1{a>=loc}((a-loc)/scale)^shape]), a in days */
+    float h_infant_shape, h_infant_scale, h_loc, h_scale, h_shape;
+    /* mosquito Weibull lifespan (mdemographic.cu:13-17): F(a) = 1-exp(-(a/scale)^shape) */
+    float m_scale, m_shape;
+    /* mosquito emergence target: m_ratio * N_c * (1 + season_amp*sin(2 pi t/365 + season_phase)) */
+    float m_ratio, season_amp, season_phase;
+    /* t=0 seeding: independent Bernoulli per individual (alive mosquitoes only) */
+    float seed_frac_h, seed_frac_m;
+    /* t=0 immunity per serotype: 0 = probability sero_prev everywhere,
+       1 = fraction drawn Beta(2,5) per community and serotype */
+    uint32_t immunity_mode;
+    float sero_prev;
+} ArboParams;
+
+/* Shape of one simulation: a grid_w x grid_h non-wrapping lattice of
+ * communities (community id = y*grid_w + x) and the humans of each. */
+typedef struct ArboShape {
+    uint32_t grid_w, grid_h;
+    const uint32_t* humans;      /* [grid_w*grid_h], each 1..ARBO_MAX_COMMUNITY_HUMANS */
+    const uint32_t* mosq_slots;  /* [grid_w*grid_h] slot capacity, or NULL for
+                                    ceil(m_ratio*(1+season_amp)*humans[c]) */
+} ArboShape;
+
+/* Time series of one simulation, layout of the reference's d_n* arrays
+ * (datacollect.cu:10-74): row t is the state at the start of day t,
+ * rows 0..days_done; per-serotype arrays are [t*S + s]. */
+typedef struct ArboSeries {
+    uint32_t* n_count;        /* [T+1]  humans alive              (d_nCount) */
+    uint32_t* m_count;        /* [T+1]  mosquitoes alive          (d_mCount) */
+    uint32_t* n_infected;     /* [(T+1)*S] infectious humans      (d_nInfectedCount) */
+    uint32_t* n_incidence;    /* [(T+1)*S] E->I onsets of day t-1 (d_nIncidence) */
+    uint32_t* n_one_subpop;   /* [(T+1)*S] infectious humans in community 0 (d_nOneSubPopInfectedCount) */
+} ArboSeries;
+
+/* Packed census of one simulation, community-contiguous (no padding).
+ * Human word: bits 0-1 status (0 S, 1 E, 2 I), 2-3 strain, 4-7 immune mask,
+ *             8-31 death day (dies during day t iff t >= death day).
+ * Mosquito word: bits 0-1 status (0 S, 1 E, 2 I, 3 dead slot), 2-3 strain,
+ *             4-15 death day mod 4096.
+ * next_day: E->I day (or I->S day for humans), mod 2^16; the transition fires
+ *           on the first day t with t > next_day. */
+typedef struct ArboState {
+    uint32_t* h_word;     /* [N] */
+    uint16_t* h_next;     /* [N] */
+    int32_t* h_birth;     /* [N] day of birth (negative: before day 0) */
+    uint16_t* m_word;     /* [M slots] */
+    uint16_t* m_next;     /* [M slots] */
+} ArboState;
+
+typedef struct ArboEngine ArboEngine;
+
+/* Create an engine on one device.  Replaces the per-MCMC-step cudaMalloc of
+ * main.cpp:198-249 (buffers are sized by arbo_batch_init and reused). */
+ARBO_API int arbo_create(const ArboEngineDesc* desc, ArboEngine** out);
+
+/* Build the dispersal CSR and alias tables, the lifespan tables, the seasonal
+ * targets and the t=0 census of n_sims simulations, room for max_days days.
+ * Replaces setupCudaRNG, demographicInitialization, epidemicInitial and the
+ * first dataCollect (simulation.cpp:88-113). */
+ARBO_API int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes,
+                    const ArboParams* params, const uint64_t* seeds, uint32_t max_days);
+
+/* Advance every simulation by n_days days.  One day = nDemographic,
+ * mDemographic, epidemic and dataCollect of the reference's day loop
+ * (simulation.cpp:115-155). */
+ARBO_API int arbo_batch_step(ArboEngine* e, uint32_t n_days);
+
+/* Wait for queued work. */
+ARBO_API int arbo_sync(ArboEngine* e);
+
+/* Days simulated so far. */
+ARBO_API int arbo_days_done(const ArboEngine* e, uint32_t* out);
+
+/* Sizes of simulation `sim`: communities, humans, mosquito slots, serotypes. */
+ARBO_API int arbo_sim_sizes(const ArboEngine* e, uint32_t sim, uint32_t* communities,
+                   uint64_t* humans, uint64_t* slots, uint32_t* serotypes);
+
+/* Copy the time series of one simulation (rows 0..days_done) to the host.
+ * Replaces the D2H copies of MCMC::storeIncidence and writeIncidence. */
+ARBO_API int arbo_read_series(ArboEngine* e, uint32_t sim, ArboSeries* out);
+
+/* Copy the census of one simulation to the host / restore it (checkpoint). */
+ARBO_API int arbo_read_state(ArboEngine* e, uint32_t sim, ArboState* out);
+ARBO_API int arbo_write_state(ArboEngine* e, uint32_t sim, const ArboState* in);
+
+/* Set the day counter of the whole batch (resume from a checkpoint written by
+ * arbo_read_state at that day); series rows after `day` are cleared. */
+ARBO_API int arbo_set_day(ArboEngine* e, uint32_t day);
+
+/* Kernel timing (ARBO_F_TIMING): total milliseconds and launches of the
+ * demography kernel (kind 0) and event-application kernel (kind 1) since the
+ * last reset. */
+ARBO_API int arbo_kernel_time(ArboEngine* e, uint32_t kind, double* ms, uint64_t* launches);
+ARBO_API int arbo_reset_timing(ArboEngine* e);
+
+ARBO_API void arbo_destroy(ArboEngine* e);
+ARBO_API const char* arbo_last_error(const ArboEngine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARBO_C_H */

@@ -1,0 +1,791 @@
+// arbo_oracle.cpp — TEST INFRASTRUCTURE ONLY: the serial / OpenMP CPU oracle
+// of the dengue IBM day loop.  It exports the engine's C ABI (include/arbo_c.h)
+// so tests can drive it exactly like the GPU engine, and it is what the GPU
+// engine must match bit-for-bit on every census word and series value.  No
+// product code links or loads it.
+//
+// It restates the reference's per-day algorithm (simulation.cpp:115-155):
+//   human ageing, death/rebirth, E->I and I->S   ndemographic.cu:32-105
+//   mosquito ageing, death, E->I, emergence       mdemographic.cu:22-150
+//   event counts and dispersal                    nepidemic.cu:28-142, mepidemic.cu:28-138
+//   event application onto mosquitoes             mepidemic.cu:143-246
+//   event application onto humans + introductions nepidemic.cu:146-319
+//   time-series counters                          datacollect.cu:10-189
+//   t=0 census                                    demographicintial.cu:86-297, epidemicinitial.cu:27-99
+// with these deliberate changes, the same in the engine:
+//   - stateless Philox keyed on (seed; entity, day, purpose, index) instead of
+//     curand states picked by SM/warp/lane id;
+//   - lifetimes as integer death days: the reference's daily test
+//     PLE > F(age) becomes age >= death_age with death_age = min{a>=1: F(a) >= PLE},
+//     F tabulated once in double, PLE a 24-bit uniform;
+//   - Poisson event counts (local ~ Poisson(lambda(1-w)), long-distance
+//     ~ Poisson(lambda w)) instead of decimalResolve splits; local
+//     destinations from per-row alias tables, long-distance arrivals drawn at
+//     the destination from the summed long-distance rate (Poisson thinning);
+//   - emergence per community into its lowest-index dead slots;
+//   - mosquito targets uniform over alive slots (the reference retries dead
+//     picks until an alive one is found: the same law);
+//   - fixed event order per destination: serotype rotation from a keyed
+//     start, local events by source community then event index, then the
+//     long-distance arrivals.
+// Serial semantics throughout: each event checks the target's current state.
+#include "arbo_c.h"
+#include "oracle_rng.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#else
+static int omp_get_max_threads() { return 1; }
+#endif
+
+using namespace orc;
+
+namespace {
+
+constexpr uint32_t HUMAN_AGE_TABLE = 65536;  // death ages 1..65535
+constexpr uint32_t MOSQ_AGE_TABLE = 2048;    // death ages 1..2047
+constexpr uint32_t ONE24 = 1u << 24;
+
+struct Sim {
+    uint32_t W = 0, H = 0, C = 0, S = 0;
+    Seed key{0, 0};
+    ArboParams p{};
+    std::vector<uint32_t> N, cap;
+    std::vector<uint64_t> h_off, m_off;
+    uint64_t NH = 0, NM = 0;
+    std::vector<float> base;
+    std::vector<uint32_t> row_ptr, to, thr, alias;
+    std::vector<uint32_t> in_ptr, in_src, in_slot;
+    std::vector<uint32_t> Kh, Km, Thy, Tmd;
+    std::vector<float> dens;
+    std::vector<uint32_t> imm_thr;
+    uint32_t seed_thr_h = 0, seed_thr_m = 0;
+    // census
+    std::vector<uint32_t> hw;
+    std::vector<uint16_t> hn;
+    std::vector<int32_t> hb;
+    std::vector<uint16_t> mw, mn;
+    // per-day
+    std::vector<uint32_t> cnt;      // [C][2][4][9]
+    std::vector<uint32_t> Ih, Im, inc, Mc;
+    std::vector<uint64_t> ldfx;     // [C][2][4] long-distance rate, 2^-24 fixed point
+    std::vector<uint64_t> lam;      // [2][4] summed long-distance rate
+    // series
+    std::vector<uint32_t> n_count, m_count, n_inf, n_inc, n_one;
+};
+
+}  // namespace
+
+struct ArboEngine {
+    std::vector<Sim> sims;
+    uint32_t day = 0, max_days = 0;
+    int threads = 0;
+    bool ready = false;
+    std::string err;
+};
+
+namespace {
+
+int fail(ArboEngine* e, int code, const char* msg) {
+    if (e) e->err = msg;
+    return code;
+}
+
+double pw(double x, double y) { return x > 0.0 ? exp_d(y * log_d(x)) : 0.0; }
+
+// Human bi-Weibull CDF of ndemographic.cu:13-29 (b multiplies the age).
+double cdf_h(const ArboParams& p, double a) {
+    double h = pw(a * (double)p.h_infant_scale, (double)p.h_infant_shape);
+    if (a >= (double)p.h_loc) h = h + pw((a - (double)p.h_loc) / (double)p.h_scale, (double)p.h_shape);
+    return 1.0 - exp_d(-h);
+}
+
+// Mosquito Weibull CDF of mdemographic.cu:13-17.
+double cdf_m(const ArboParams& p, double a) {
+    return 1.0 - exp_d(-pw(a / (double)p.m_scale, (double)p.m_shape));
+}
+
+uint32_t to24(double f) {
+    double v = std::floor(f * 16777216.0);
+    if (!(v > 0.0)) return 0;
+    if (v >= 16777216.0) return ONE24;
+    return (uint32_t)v;
+}
+
+// Lifetime threshold table: K[a] = floor(F(a) 2^24), non-decreasing.
+template <class F>
+std::vector<uint32_t> death_table(uint32_t n, F cdf) {
+    std::vector<uint32_t> K(n);
+    for (uint32_t a = 0; a < n; ++a) {
+        K[a] = to24(cdf((double)a));
+        if (a > 0 && K[a] < K[a - 1]) K[a] = K[a - 1];
+    }
+    return K;
+}
+
+// Initial-age table from the cumulative survival sums of
+// demographicintial.cu:48-83: T[i] = floor(S[i]/S[last] 2^24).
+template <class F>
+std::vector<uint32_t> age_table(uint32_t n, uint32_t step, F cdf) {
+    std::vector<double> S(n);
+    double acc = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        acc = acc + (1.0 - cdf((double)(i * step)));
+        S[i] = acc;
+    }
+    std::vector<uint32_t> T(n);
+    for (uint32_t i = 0; i < n; ++i) T[i] = to24(S[i] / S[n - 1]);
+    T[n - 1] = ONE24;
+    for (uint32_t i = 1; i < n; ++i)
+        if (T[i] < T[i - 1]) T[i] = T[i - 1];
+    return T;
+}
+
+// min{a in [1, n-1] : K[a] >= k}, else n-1 (K is non-decreasing).
+uint32_t death_age(const std::vector<uint32_t>& K, uint32_t k) {
+    auto it = std::lower_bound(K.begin() + 1, K.end(), k);
+    if (it == K.end()) return (uint32_t)K.size() - 1;
+    return (uint32_t)(it - K.begin());
+}
+
+// min{i : k <= T[i]} (T is non-decreasing and ends at 2^24).
+uint32_t pick_index(const std::vector<uint32_t>& T, uint32_t k) {
+    auto it = std::lower_bound(T.begin(), T.end(), k);
+    if (it == T.end()) return (uint32_t)T.size() - 1;
+    return (uint32_t)(it - T.begin());
+}
+
+// Death age conditional on having reached age a0: PLE uniform on (F(a0-1), 1]
+// (demographicintial.cu:143-152, 266-275).
+uint32_t death_age_given(const std::vector<uint32_t>& K, uint32_t a0, uint32_t r) {
+    uint32_t prev = K[a0 >= 1 ? a0 - 1 : 0];
+    if (prev >= ONE24) return a0;
+    uint32_t k = prev + 1 + range(r, ONE24 - prev);
+    return death_age(K, k);
+}
+
+// Non-wrapping lattice CSR (network.cpp:31-41, rows in ascending column order
+// as network.cpp:95-113) and Vose alias tables of the dispersal weights.
+void build_network(Sim& s) {
+    const ArboParams& p = s.p;
+    s.row_ptr.assign(1, 0);
+    s.to.clear(); s.thr.clear(); s.alias.clear();
+    for (uint32_t y = 0; y < s.H; ++y) {
+        for (uint32_t x = 0; x < s.W; ++x) {
+            std::vector<uint32_t> cols;
+            std::vector<double> w;
+            for (int dy = -1; dy <= 1; ++dy) {
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int nx = (int)x + dx, ny = (int)y + dy;
+                    if (nx < 0 || ny < 0 || nx >= (int)s.W || ny >= (int)s.H) continue;
+                    int manh = std::abs(dx) + std::abs(dy);
+                    if (manh == 0 && !p.include_self) continue;
+                    if (p.neighbourhood == 4 && manh == 2) continue;
+                    double dist = manh == 0 ? 0.0 : (manh == 1 ? 1.0 : 1.4142135623730951);
+                    double wt = p.dispersal_scale > 0.0f ? exp_d(-dist / (double)p.dispersal_scale) : 1.0;
+                    cols.push_back((uint32_t)ny * s.W + (uint32_t)nx);
+                    w.push_back(wt);
+                }
+            }
+            if (cols.empty()) {  // isolated: every local event stays home
+                cols.push_back(y * s.W + x);
+                w.push_back(1.0);
+            }
+            uint32_t n = (uint32_t)cols.size();
+            double tot = 0.0;
+            for (double v : w) tot = tot + v;
+            std::vector<double> q(n), prob(n, 1.0);
+            std::vector<uint32_t> al(n);
+            for (uint32_t i = 0; i < n; ++i) { q[i] = w[i] / tot * (double)n; al[i] = i; }
+            std::vector<uint32_t> small, large;
+            for (uint32_t i = 0; i < n; ++i) (q[i] < 1.0 ? small : large).push_back(i);
+            while (!small.empty() && !large.empty()) {
+                uint32_t l = small.back(); small.pop_back();
+                uint32_t g = large.back(); large.pop_back();
+                prob[l] = q[l];
+                al[l] = g;
+                q[g] = (q[g] + q[l]) - 1.0;
+                (q[g] < 1.0 ? small : large).push_back(g);
+            }
+            for (uint32_t i = 0; i < n; ++i) {
+                s.to.push_back(cols[i]);
+                s.thr.push_back(to24(prob[i]));
+                s.alias.push_back(al[i]);
+            }
+            s.row_ptr.push_back((uint32_t)s.to.size());
+        }
+    }
+    // in-edges of each destination, ascending source
+    std::vector<std::vector<std::pair<uint32_t, uint32_t>>> in(s.C);
+    for (uint32_t c = 0; c < s.C; ++c)
+        for (uint32_t e = s.row_ptr[c]; e < s.row_ptr[c + 1]; ++e)
+            in[s.to[e]].push_back({c, e - s.row_ptr[c]});
+    s.in_ptr.assign(1, 0);
+    s.in_src.clear(); s.in_slot.clear();
+    for (uint32_t d = 0; d < s.C; ++d) {
+        std::sort(in[d].begin(), in[d].end());
+        for (auto& pr : in[d]) { s.in_src.push_back(pr.first); s.in_slot.push_back(pr.second); }
+        s.in_ptr.push_back((uint32_t)s.in_src.size());
+    }
+}
+
+int validate(ArboEngine* e, const ArboShape& sh, const ArboParams& p) {
+    if (p.n_serotypes < 1 || p.n_serotypes > ARBO_MAX_SEROTYPES) return fail(e, ARBO_E_INVALID, "n_serotypes must be 1..4");
+    if (sh.grid_w < 1 || sh.grid_h < 1 || !sh.humans) return fail(e, ARBO_E_INVALID, "empty lattice");
+    if ((uint64_t)sh.grid_w * sh.grid_h > (1u << 24)) return fail(e, ARBO_E_INVALID, "too many communities");
+    if (p.neighbourhood != 4 && p.neighbourhood != 8) return fail(e, ARBO_E_INVALID, "neighbourhood must be 4 or 8");
+    if (!(p.beta_hm >= 0.0f) || !(p.beta_mh >= 0.0f)) return fail(e, ARBO_E_INVALID, "beta must be >= 0");
+    if (!(p.long_distance >= 0.0f && p.long_distance <= 1.0f)) return fail(e, ARBO_E_INVALID, "long_distance must be in [0,1]");
+    if (!(p.season_amp >= 0.0f && p.season_amp <= 1.0f)) return fail(e, ARBO_E_INVALID, "season_amp must be in [0,1]");
+    if (!(p.recovery >= 0.0f && p.recovery < 1000.0f) || !(p.eip >= 0.0f && p.eip < 1000.0f)) return fail(e, ARBO_E_INVALID, "recovery/eip out of range");
+    if (p.iip_max < p.iip_min || p.iip_max > 1000) return fail(e, ARBO_E_INVALID, "bad incubation range");
+    if (!(p.m_scale > 0.0f) || !(p.m_shape > 0.0f) || !(p.h_scale > 0.0f) || !(p.h_shape > 0.0f)) return fail(e, ARBO_E_INVALID, "bad lifespan parameters");
+    if (!(p.m_ratio >= 0.0f)) return fail(e, ARBO_E_INVALID, "m_ratio must be >= 0");
+    if (!(p.external_infection >= 0.0f)) return fail(e, ARBO_E_INVALID, "external_infection must be >= 0");
+    if (!(p.seed_frac_h >= 0.0f && p.seed_frac_h <= 1.0f) || !(p.seed_frac_m >= 0.0f && p.seed_frac_m <= 1.0f)) return fail(e, ARBO_E_INVALID, "seed fractions must be in [0,1]");
+    if (p.immunity_mode > 1 || !(p.sero_prev >= 0.0f && p.sero_prev <= 1.0f)) return fail(e, ARBO_E_INVALID, "bad immunity settings");
+    if (!(p.dispersal_scale >= 0.0f)) return fail(e, ARBO_E_INVALID, "dispersal_scale must be >= 0");
+    return ARBO_OK;
+}
+
+void init_sim(Sim& s, uint32_t max_days) {
+    const ArboParams& p = s.p;
+    const uint32_t S = s.S;
+    // lifespan and initial-age tables
+    s.Kh = death_table(HUMAN_AGE_TABLE, [&](double a) { return cdf_h(p, a); });
+    s.Km = death_table(MOSQ_AGE_TABLE, [&](double a) { return cdf_m(p, a); });
+    s.Thy = age_table(151, 365, [&](double a) { return cdf_h(p, a); });
+    s.Tmd = age_table(301, 1, [&](double a) { return cdf_m(p, a); });
+    // seasonal density 1 + A sin(2 pi t/365 + phi)
+    s.dens.resize(max_days + 1);
+    for (uint32_t t = 0; t <= max_days; ++t) {
+        double arg = 6.283185307179586 * (double)t / 365.0 + (double)p.season_phase;
+        s.dens[t] = (float)(1.0 + (double)p.season_amp * sin_d(arg));
+    }
+    // immunity thresholds per (community, serotype)
+    s.imm_thr.assign((size_t)s.C * S, 0);
+    for (uint32_t c = 0; c < s.C; ++c)
+        for (uint32_t v = 0; v < S; ++v) {
+            if (p.immunity_mode == 0) {
+                s.imm_thr[(size_t)c * S + v] = to24((double)p.sero_prev);
+            } else {  // Beta(2,5) = 2nd smallest of 6 uniforms
+                W4 a = rnd(s.key, c, DAY_INIT, tag(IMMUNE, 0, v, 0), 0);
+                W4 b = rnd(s.key, c, DAY_INIT, tag(IMMUNE, 0, v, 0), 1);
+                uint32_t u[6] = {a.v[0] >> 8, a.v[1] >> 8, a.v[2] >> 8, a.v[3] >> 8, b.v[0] >> 8, b.v[1] >> 8};
+                std::sort(u, u + 6);
+                s.imm_thr[(size_t)c * S + v] = u[1];
+            }
+        }
+    s.seed_thr_h = to24((double)p.seed_frac_h);
+    s.seed_thr_m = to24((double)p.seed_frac_m);
+    build_network(s);
+
+    s.hw.assign(s.NH, 0); s.hn.assign(s.NH, 0); s.hb.assign(s.NH, 0);
+    s.mw.assign(s.NM, 3); s.mn.assign(s.NM, 0);
+    s.cnt.assign((size_t)s.C * 72, 0);
+    s.Ih.assign((size_t)s.C * S, 0); s.Im.assign((size_t)s.C * S, 0);
+    s.inc.assign((size_t)s.C * S, 0); s.Mc.assign(s.C, 0);
+    s.ldfx.assign((size_t)s.C * 8, 0);
+    s.lam.assign(8, 0);
+    s.n_count.assign(max_days + 1, 0); s.m_count.assign(max_days + 1, 0);
+    s.n_inf.assign((size_t)(max_days + 1) * S, 0);
+    s.n_inc.assign((size_t)(max_days + 1) * S, 0);
+    s.n_one.assign((size_t)(max_days + 1) * S, 0);
+}
+
+// t=0 census of one community (demographicintial.cu:86-297 and
+// epidemicinitial.cu:27-99, with Philox-chosen seed infections).
+void init_community(Sim& s, uint32_t c) {
+    const ArboParams& p = s.p;
+    const uint32_t S = s.S;
+    uint32_t Ih[4] = {0, 0, 0, 0}, Im[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < s.N[c]; ++i) {
+        uint32_t h = (uint32_t)(s.h_off[c] + i);
+        W4 r0 = rnd(s.key, h, DAY_INIT, INIT_H, 0);
+        uint32_t y = pick_index(s.Thy, (r0.v[0] >> 8) + 1);
+        uint32_t a0 = 365 * y + range(r0.v[1], 365);
+        uint32_t da = death_age_given(s.Kh, a0, r0.v[2]);
+        int32_t ddi = (int32_t)da - (int32_t)(a0 + 1);
+        uint32_t dd = ddi < 0 ? 0u : (uint32_t)ddi;
+        W4 r1 = rnd(s.key, h, DAY_INIT, INIT_H, 1);
+        uint32_t imm = 0;
+        for (uint32_t v = 0; v < S; ++v)
+            if ((r1.v[v] >> 8) < s.imm_thr[(size_t)c * S + v]) imm |= 1u << v;
+        uint32_t st = 0, strain = 0;
+        uint16_t nxt = 0;
+        if ((r0.v[3] >> 8) < s.seed_thr_h) {
+            W4 r2 = rnd(s.key, h, DAY_INIT, INIT_H, 2);
+            strain = range(r2.v[0], S);
+            st = 2;
+            imm |= 1u << strain;
+            uint32_t R = sround(p.recovery, r2.v[2]);
+            if (R < 1) R = 1;
+            nxt = (uint16_t)range(r2.v[1], R);
+            Ih[strain]++;
+        }
+        s.hw[h] = (dd << 8) | (imm << 4) | (strain << 2) | st;
+        s.hn[h] = nxt;
+        s.hb[h] = -(int32_t)(a0 + 1);
+    }
+    W4 rt = rnd(s.key, c, DAY_INIT, INIT_MT, 0);
+    uint32_t alive = sround(s.dens[0] * s.base[c], rt.v[0]);
+    if (alive > s.cap[c]) alive = s.cap[c];
+    for (uint32_t j = 0; j < s.cap[c]; ++j) {
+        uint32_t m = (uint32_t)(s.m_off[c] + j);
+        s.mn[m] = 0;
+        if (j >= alive) { s.mw[m] = 3; continue; }
+        W4 r0 = rnd(s.key, m, DAY_INIT, INIT_M, 0);
+        uint32_t a0 = pick_index(s.Tmd, (r0.v[0] >> 8) + 1);
+        uint32_t da = death_age_given(s.Km, a0, r0.v[2]);
+        int32_t ddi = (int32_t)da - (int32_t)(a0 + 1);
+        uint32_t dd12 = (uint32_t)ddi & 0xFFFu;
+        uint32_t st = 0, strain = 0;
+        if ((r0.v[3] >> 8) < s.seed_thr_m) {
+            st = 2;
+            strain = range(r0.v[1], S);
+            Im[strain]++;
+        }
+        s.mw[m] = (uint16_t)((dd12 << 4) | (strain << 2) | st);
+    }
+    s.Mc[c] = alive;
+    for (uint32_t v = 0; v < S; ++v) {
+        s.Ih[(size_t)c * S + v] = Ih[v];
+        s.Im[(size_t)c * S + v] = Im[v];
+        s.inc[(size_t)c * S + v] = 0;
+    }
+}
+
+// Series row t from the per-community counters (datacollect.cu:79-189 as
+// plain sums; community 0 is the reference's h_specialSubPop).
+void record_row(Sim& s, uint32_t t) {
+    const uint32_t S = s.S;
+    uint64_t nc = 0, mc = 0;
+    uint64_t inf[4] = {0, 0, 0, 0}, inc[4] = {0, 0, 0, 0};
+    for (uint32_t c = 0; c < s.C; ++c) {
+        nc += s.N[c];
+        mc += s.Mc[c];
+        for (uint32_t v = 0; v < S; ++v) {
+            inf[v] += s.Ih[(size_t)c * S + v];
+            inc[v] += s.inc[(size_t)c * S + v];
+        }
+    }
+    s.n_count[t] = (uint32_t)nc;
+    s.m_count[t] = (uint32_t)mc;
+    for (uint32_t v = 0; v < S; ++v) {
+        s.n_inf[(size_t)t * S + v] = (uint32_t)inf[v];
+        s.n_inc[(size_t)t * S + v] = (uint32_t)inc[v];
+        s.n_one[(size_t)t * S + v] = s.Ih[v];
+    }
+}
+
+// Phase A of day t for one community: demography, counters, event counts.
+void phase_a(Sim& s, uint32_t c, uint32_t t) {
+    const ArboParams& p = s.p;
+    const uint32_t S = s.S;
+    uint32_t Ih[4] = {0, 0, 0, 0}, Im[4] = {0, 0, 0, 0}, inc[4] = {0, 0, 0, 0};
+    // humans: nBlockBirthDeath (ndemographic.cu:32-105)
+    for (uint32_t i = 0; i < s.N[c]; ++i) {
+        uint32_t h = (uint32_t)(s.h_off[c] + i);
+        uint32_t w = s.hw[h];
+        uint32_t st = w & 3u, strain = (w >> 2) & 3u;
+        if (t >= (w >> 8)) {  // death and instant rebirth
+            W4 r = rnd(s.key, h, t, H_BIRTH, 0);
+            uint32_t da = death_age(s.Kh, (r.v[0] >> 8) + 1);
+            s.hw[h] = (t + da) << 8;
+            s.hb[h] = (int32_t)t;
+            continue;
+        }
+        if (st == 2) {
+            if ((int16_t)(uint16_t)(t - s.hn[h]) > 0) {
+                s.hw[h] = w & ~0xFu;   // recovered: susceptible to the other serotypes
+            } else {
+                Ih[strain]++;
+            }
+        } else if (st == 1) {
+            if ((int16_t)(uint16_t)(t - s.hn[h]) > 0) {
+                W4 r = rnd(s.key, h, t, H_REC, 0);
+                s.hn[h] = (uint16_t)(s.hn[h] + sround(p.recovery, r.v[0]));
+                s.hw[h] = (w & ~3u) | 2u;
+                inc[strain]++;
+                Ih[strain]++;
+            }
+        }
+    }
+    // mosquitoes: mBlockBirthDeath (mdemographic.cu:22-150) per community
+    const uint32_t cap = s.cap[c];
+    thread_local std::vector<uint8_t> dead, died;
+    dead.assign(cap, 0);
+    died.assign(cap, 0);
+    uint32_t alive = 0, ndead = 0;
+    for (uint32_t j = 0; j < cap; ++j) {
+        uint32_t m = (uint32_t)(s.m_off[c] + j);
+        uint32_t w = s.mw[m];
+        uint32_t st = w & 3u;
+        if (st == 3) { dead[j] = 1; ndead++; continue; }
+        if (((t - (w >> 4)) & 0xFFFu) < 2048u) { dead[j] = 1; died[j] = 1; ndead++; continue; }
+        if (st == 1 && (int16_t)(uint16_t)(t - s.mn[m]) > 0) {
+            st = 2;
+            s.mw[m] = (uint16_t)((w & ~3u) | 2u);
+        }
+        alive++;
+        if (st == 2) Im[(w >> 2) & 3u]++;
+    }
+    W4 rt = rnd(s.key, c, t, M_TARGET, 0);
+    uint32_t target = sround(s.dens[t] * s.base[c], rt.v[0]);
+    uint32_t births = target > alive ? target - alive : 0;
+    if (births > ndead) births = ndead;
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < cap; ++j) {
+        if (!dead[j]) continue;
+        uint32_t m = (uint32_t)(s.m_off[c] + j);
+        if (rank < births) {
+            W4 r = rnd(s.key, m, t, M_BIRTH, 0);
+            uint32_t da = death_age(s.Km, (r.v[0] >> 8) + 1);
+            s.mw[m] = (uint16_t)(((t + da) & 0xFFFu) << 4);
+        } else if (died[j]) {
+            s.mw[m] = 3;
+        }
+        rank++;
+    }
+    const uint32_t Mc = alive + births;
+    s.Mc[c] = Mc;
+    for (uint32_t v = 0; v < S; ++v) {
+        s.Ih[(size_t)c * S + v] = Ih[v];
+        s.Im[(size_t)c * S + v] = Im[v];
+        s.inc[(size_t)c * S + v] = inc[v];
+    }
+    // event counts and alias destinations (nVisitingInfected / mVisitingInfected)
+    uint32_t* cnt = &s.cnt[(size_t)c * 72];
+    std::fill(cnt, cnt + 72, 0u);
+    const uint32_t r0 = s.row_ptr[c], rl = s.row_ptr[c + 1] - r0;
+    for (uint32_t dir = 0; dir < 2; ++dir) {
+        for (uint32_t v = 0; v < 4; ++v) {
+            uint64_t fx = 0;
+            if (v < S) {
+                float lam = 0.0f;
+                if (dir == HM && Ih[v] > 0) lam = (float)Mc / (float)s.N[c] * p.beta_hm * (float)Ih[v];
+                if (dir == MH && Im[v] > 0) lam = p.beta_mh * (float)Im[v];
+                if (lam > 0.0f) {
+                    float mu_loc = lam * (1.0f - p.long_distance);
+                    float mu_ld = lam * p.long_distance;
+                    fx = (uint64_t)std::floor((double)mu_ld * 16777216.0);
+                    uint32_t n = poisson_d((double)mu_loc, s.key, c, t, tag(NLOC, dir, v, 0));
+                    for (uint32_t k = 0; k < n; ++k) {
+                        W4 r = rnd(s.key, c, t, tag(DEST, dir, v, 0), k);
+                        uint32_t col = range(r.v[0], rl);
+                        if ((r.v[1] >> 8) >= s.thr[r0 + col]) col = s.alias[r0 + col];
+                        cnt[(dir * 4 + v) * 9 + col]++;
+                    }
+                }
+            }
+            s.ldfx[(size_t)c * 8 + dir * 4 + v] = fx;
+        }
+    }
+}
+
+struct Event { uint32_t s, x0, x2, k; };
+
+// Ordered event list of destination d in direction dir (see file header).
+void events_of(const Sim& s, uint32_t d, uint32_t dir, uint32_t s0,
+               const uint32_t* nld, std::vector<std::vector<Event>>& per_pos) {
+    const uint32_t S = s.S;
+    per_pos.assign(S, {});
+    for (uint32_t pos = 0; pos < S; ++pos) {
+        uint32_t v = (s0 + pos) % S;
+        for (uint32_t e = s.in_ptr[d]; e < s.in_ptr[d + 1]; ++e) {
+            uint32_t src = s.in_src[e], slot = s.in_slot[e];
+            uint32_t n = s.cnt[(size_t)src * 72 + (dir * 4 + v) * 9 + slot];
+            for (uint32_t k = 0; k < n; ++k) per_pos[pos].push_back({v, src, tag(TGT, dir, v, slot), k});
+        }
+        for (uint32_t k = 0; k < nld[v]; ++k) per_pos[pos].push_back({v, d, tag(TGT_LD, dir, v, 0), k});
+    }
+}
+
+void infect_human(Sim& s, uint32_t h, uint32_t v, uint32_t t) {
+    W4 r = rnd(s.key, h, t, H_IIP, 0);
+    uint32_t iip = s.p.iip_min + range(r.v[0], s.p.iip_max - s.p.iip_min + 1);
+    s.hw[h] = (s.hw[h] & ~0xFu) | (1u << (4 + v)) | (v << 2) | 1u;
+    s.hn[h] = (uint16_t)(t + iip);
+}
+
+bool human_eligible(uint32_t w, uint32_t v) { return (w & 3u) == 0 && !((w >> (4 + v)) & 1u); }
+
+// Phase B of day t for destination d: nmTransmission then mnTransmission.
+void phase_b(Sim& s, uint32_t d, uint32_t t) {
+    const ArboParams& p = s.p;
+    const uint32_t S = s.S;
+    uint32_t nld[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    for (uint32_t dir = 0; dir < 2; ++dir)
+        for (uint32_t v = 0; v < S; ++v) {
+            double mu = (double)s.lam[dir * 4 + v] * 0x1p-24 / (double)s.C;
+            nld[dir][v] = poisson_d(mu, s.key, d, t, tag(NLD, dir, v, 0));
+        }
+    std::vector<std::vector<Event>> ev;
+    // onto mosquitoes (mepidemic.cu:143-246)
+    {
+        uint32_t s0 = range(rnd(s.key, d, t, tag(ROT, HM, 0, 0), 0).v[0], S);
+        events_of(s, d, HM, s0, nld[HM], ev);
+        std::vector<uint32_t> alive_slots;
+        for (uint32_t j = 0; j < s.cap[d]; ++j)
+            if ((s.mw[s.m_off[d] + j] & 3u) != 3u) alive_slots.push_back(j);
+        const uint32_t Md = (uint32_t)alive_slots.size();
+        if (Md > 0) {
+            for (uint32_t pos = 0; pos < S; ++pos)
+                for (const Event& E : ev[pos]) {
+                    W4 r = rnd(s.key, E.x0, t, E.x2, E.k);
+                    uint32_t m = (uint32_t)(s.m_off[d] + alive_slots[range(r.v[0], Md)]);
+                    uint32_t w = s.mw[m];
+                    if ((w & 3u) != 0) continue;
+                    W4 r2 = rnd(s.key, m, t, M_EIP, 0);
+                    s.mw[m] = (uint16_t)((w & ~0xFu) | (E.s << 2) | 1u);
+                    s.mn[m] = (uint16_t)(t + sround(p.eip, r2.v[0]));
+                }
+        }
+    }
+    // onto humans (nepidemic.cu:146-319)
+    {
+        uint32_t s0 = range(rnd(s.key, d, t, tag(ROT, MH, 0, 0), 0).v[0], S);
+        events_of(s, d, MH, s0, nld[MH], ev);
+        const uint32_t Nd = s.N[d];
+        for (uint32_t pos = 0; pos < S; ++pos) {
+            uint32_t v = (s0 + pos) % S;
+            for (const Event& E : ev[pos]) {
+                W4 r = rnd(s.key, E.x0, t, E.x2, E.k);
+                uint32_t h = (uint32_t)(s.h_off[d] + range(r.v[0], Nd));
+                if (human_eligible(s.hw[h], v)) infect_human(s, h, v, t);
+            }
+            if (p.external_infection > 0.0f) {  // nepidemic.cu:261-309
+                uint32_t x = sround(p.external_infection / (float)s.C,
+                                    rnd(s.key, d, t, tag(INTRO, 0, v, 0), 0).v[0]);
+                for (uint32_t att = 0; x > 0 && att < 1000; ++att) {
+                    W4 r = rnd(s.key, d, t, tag(INTRO, 0, v, 0), 1 + att);
+                    uint32_t h = (uint32_t)(s.h_off[d] + range(r.v[0], Nd));
+                    if (human_eligible(s.hw[h], v)) { infect_human(s, h, v, t); x--; }
+                }
+            }
+        }
+    }
+}
+
+void day_step(ArboEngine* e, uint32_t t) {
+    for (Sim& s : e->sims) {
+        const int C = (int)s.C;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(e->threads > 0 ? e->threads : omp_get_max_threads())
+        for (int c = 0; c < C; ++c) phase_a(s, (uint32_t)c, t);
+        std::fill(s.lam.begin(), s.lam.end(), 0ull);
+        for (uint32_t c = 0; c < s.C; ++c)
+            for (uint32_t k = 0; k < 8; ++k) s.lam[k] += s.ldfx[(size_t)c * 8 + k];
+        record_row(s, t + 1);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(e->threads > 0 ? e->threads : omp_get_max_threads())
+        for (int d = 0; d < C; ++d) phase_b(s, (uint32_t)d, t);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int arbo_create(const ArboEngineDesc* desc, ArboEngine** out) {
+    if (!out) return ARBO_E_INVALID;
+    *out = nullptr;
+    if (!desc || desc->abi_version != ARBO_ABI_VERSION) return ARBO_E_INVALID;
+    ArboEngine* e = new ArboEngine();
+    e->threads = (int)desc->threads;
+    *out = e;
+    return ARBO_OK;
+}
+
+int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes,
+                    const ArboParams* params, const uint64_t* seeds, uint32_t max_days) {
+    if (!e) return ARBO_E_INVALID;
+    if (n_sims == 0 || !shapes || !params || !seeds) return fail(e, ARBO_E_INVALID, "empty batch");
+    if (max_days == 0 || max_days > (1u << 23)) return fail(e, ARBO_E_INVALID, "max_days must be 1..2^23");
+    e->ready = false;
+    for (uint32_t i = 0; i < n_sims; ++i) {
+        int rc = validate(e, shapes[i], params[i]);
+        if (rc) return rc;
+    }
+    e->sims.clear();
+    e->sims.resize(n_sims);
+    for (uint32_t i = 0; i < n_sims; ++i) {
+        Sim& s = e->sims[i];
+        const ArboShape& sh = shapes[i];
+        s.p = params[i];
+        s.W = sh.grid_w; s.H = sh.grid_h; s.C = s.W * s.H; s.S = s.p.n_serotypes;
+        s.key = {(uint32_t)seeds[i], (uint32_t)(seeds[i] >> 32)};
+        s.N.assign(sh.humans, sh.humans + s.C);
+        s.cap.resize(s.C);
+        s.base.resize(s.C);
+        s.h_off.resize(s.C); s.m_off.resize(s.C);
+        uint64_t nh = 0, nm = 0;
+        for (uint32_t c = 0; c < s.C; ++c) {
+            if (s.N[c] < 1 || s.N[c] > ARBO_MAX_COMMUNITY_HUMANS) return fail(e, ARBO_E_INVALID, "humans per community out of range");
+            s.cap[c] = sh.mosq_slots ? sh.mosq_slots[c]
+                                     : (uint32_t)std::ceil((double)s.p.m_ratio * (1.0 + (double)s.p.season_amp) * (double)s.N[c]);
+            if (s.cap[c] > ARBO_MAX_COMMUNITY_SLOTS) return fail(e, ARBO_E_INVALID, "mosquito slots per community out of range");
+            s.base[c] = s.p.m_ratio * (float)s.N[c];
+            s.h_off[c] = nh; s.m_off[c] = nm;
+            nh += s.N[c]; nm += s.cap[c];
+        }
+        if (nh >= (1ull << 32) || nm >= (1ull << 32)) return fail(e, ARBO_E_INVALID, "simulation too large");
+        s.NH = nh; s.NM = nm;
+        init_sim(s, max_days);
+    }
+    for (Sim& s : e->sims) {
+        const int C = (int)s.C;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(e->threads > 0 ? e->threads : omp_get_max_threads())
+        for (int c = 0; c < C; ++c) init_community(s, (uint32_t)c);
+        record_row(s, 0);
+    }
+    e->day = 0;
+    e->max_days = max_days;
+    e->ready = true;
+    return ARBO_OK;
+}
+
+int arbo_batch_step(ArboEngine* e, uint32_t n_days) {
+    if (!e) return ARBO_E_INVALID;
+    if (!e->ready) return fail(e, ARBO_E_STATE, "arbo_batch_init has not been called");
+    if ((uint64_t)e->day + n_days > e->max_days) return fail(e, ARBO_E_CAPACITY, "series capacity exceeded");
+    for (uint32_t k = 0; k < n_days; ++k) {
+        day_step(e, e->day);
+        e->day++;
+    }
+    return ARBO_OK;
+}
+
+int arbo_sync(ArboEngine* e) { return e ? ARBO_OK : ARBO_E_INVALID; }
+
+int arbo_days_done(const ArboEngine* e, uint32_t* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    *out = e->day;
+    return ARBO_OK;
+}
+
+int arbo_sim_sizes(const ArboEngine* e, uint32_t sim, uint32_t* communities, uint64_t* humans,
+                   uint64_t* slots, uint32_t* serotypes) {
+    if (!e || !e->ready || sim >= e->sims.size()) return ARBO_E_INVALID;
+    const Sim& s = e->sims[sim];
+    if (communities) *communities = s.C;
+    if (humans) *humans = s.NH;
+    if (slots) *slots = s.NM;
+    if (serotypes) *serotypes = s.S;
+    return ARBO_OK;
+}
+
+int arbo_read_series(ArboEngine* e, uint32_t sim, ArboSeries* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    if (!e->ready || sim >= e->sims.size()) return fail(e, ARBO_E_INVALID, "bad simulation index");
+    const Sim& s = e->sims[sim];
+    const size_t rows = e->day + 1;
+    if (out->n_count) std::memcpy(out->n_count, s.n_count.data(), rows * 4);
+    if (out->m_count) std::memcpy(out->m_count, s.m_count.data(), rows * 4);
+    if (out->n_infected) std::memcpy(out->n_infected, s.n_inf.data(), rows * s.S * 4);
+    if (out->n_incidence) std::memcpy(out->n_incidence, s.n_inc.data(), rows * s.S * 4);
+    if (out->n_one_subpop) std::memcpy(out->n_one_subpop, s.n_one.data(), rows * s.S * 4);
+    return ARBO_OK;
+}
+
+int arbo_read_state(ArboEngine* e, uint32_t sim, ArboState* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    if (!e->ready || sim >= e->sims.size()) return fail(e, ARBO_E_INVALID, "bad simulation index");
+    const Sim& s = e->sims[sim];
+    if (out->h_word) std::memcpy(out->h_word, s.hw.data(), s.NH * 4);
+    if (out->h_next) std::memcpy(out->h_next, s.hn.data(), s.NH * 2);
+    if (out->h_birth) std::memcpy(out->h_birth, s.hb.data(), s.NH * 4);
+    if (out->m_word) std::memcpy(out->m_word, s.mw.data(), s.NM * 2);
+    if (out->m_next) std::memcpy(out->m_next, s.mn.data(), s.NM * 2);
+    return ARBO_OK;
+}
+
+int arbo_write_state(ArboEngine* e, uint32_t sim, const ArboState* in) {
+    if (!e || !in) return ARBO_E_INVALID;
+    if (!e->ready || sim >= e->sims.size()) return fail(e, ARBO_E_INVALID, "bad simulation index");
+    if (!in->h_word || !in->h_next || !in->h_birth || !in->m_word || !in->m_next)
+        return fail(e, ARBO_E_INVALID, "every census array is required");
+    Sim& s = e->sims[sim];
+    std::memcpy(s.hw.data(), in->h_word, s.NH * 4);
+    std::memcpy(s.hn.data(), in->h_next, s.NH * 2);
+    std::memcpy(s.hb.data(), in->h_birth, s.NH * 4);
+    std::memcpy(s.mw.data(), in->m_word, s.NM * 2);
+    std::memcpy(s.mn.data(), in->m_next, s.NM * 2);
+    return ARBO_OK;
+}
+
+int arbo_set_day(ArboEngine* e, uint32_t day) {
+    if (!e) return ARBO_E_INVALID;
+    if (!e->ready) return fail(e, ARBO_E_STATE, "arbo_batch_init has not been called");
+    if (day > e->max_days) return fail(e, ARBO_E_CAPACITY, "day beyond series capacity");
+    e->day = day;
+    return ARBO_OK;
+}
+
+int arbo_kernel_time(ArboEngine* e, uint32_t, double* ms, uint64_t* launches) {
+    if (!e) return ARBO_E_INVALID;
+    if (ms) *ms = 0.0;
+    if (launches) *launches = 0;
+    return ARBO_OK;
+}
+
+int arbo_reset_timing(ArboEngine* e) { return e ? ARBO_OK : ARBO_E_INVALID; }
+
+void arbo_destroy(ArboEngine* e) { delete e; }
+
+const char* arbo_last_error(const ArboEngine* e) { return e ? e->err.c_str() : "null engine"; }
+
+// Oracle-only inspection hooks for the tests.
+int oracle_tables(ArboEngine* e, uint32_t sim, uint32_t which, uint32_t* out, uint32_t cap, uint32_t* n) {
+    if (!e || !e->ready || sim >= e->sims.size() || !n) return ARBO_E_INVALID;
+    const Sim& s = e->sims[sim];
+    const std::vector<uint32_t>* v = nullptr;
+    switch (which) {
+        case 0: v = &s.Kh; break;
+        case 1: v = &s.Km; break;
+        case 2: v = &s.Thy; break;
+        case 3: v = &s.Tmd; break;
+        case 4: v = &s.row_ptr; break;
+        case 5: v = &s.to; break;
+        case 6: v = &s.thr; break;
+        case 7: v = &s.alias; break;
+        case 8: v = &s.in_ptr; break;
+        case 9: v = &s.in_src; break;
+        case 10: v = &s.in_slot; break;
+        case 11: v = &s.imm_thr; break;
+        case 12: v = &s.cap; break;
+        case 13: v = &s.cnt; break;
+        default: return ARBO_E_INVALID;
+    }
+    *n = (uint32_t)v->size();
+    if (out) std::memcpy(out, v->data(), std::min<size_t>(cap, v->size()) * 4);
+    return ARBO_OK;
+}
+
+int oracle_dens(ArboEngine* e, uint32_t sim, float* out, uint32_t cap) {
+    if (!e || !e->ready || sim >= e->sims.size()) return ARBO_E_INVALID;
+    const Sim& s = e->sims[sim];
+    std::memcpy(out, s.dens.data(), std::min<size_t>(cap, s.dens.size()) * 4);
+    return ARBO_OK;
+}
+
+// Primitive hooks, so the tests can pin the oracle's own building blocks.
+void oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    W4 r = philox10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+    std::memcpy(out, r.v, 16);
+}
+double oracle_exp(double x) { return exp_d(x); }
+double oracle_log(double x) { return log_d(x); }
+double oracle_sin(double x) { return sin_d(x); }
+double oracle_logfact(double k) { return logfact_d(k); }
+uint32_t oracle_poisson(double mu, uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2) {
+    Seed k{(uint32_t)seed, (uint32_t)(seed >> 32)};
+    return poisson_d(mu, k, c0, c1, c2);
+}
+uint32_t oracle_sround(float x, uint32_t r) { return sround(x, r); }
+
+}  // extern "C"

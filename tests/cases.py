@@ -1,0 +1,70 @@
+"""Parity cases shared by the CPU oracle tests and the GPU parity tests.
+
+Each case is a list of simulations and a number of days.  Together they cover
+the configured workload (config 0 in full), the reference's own parameter set
+(von Neumann lattice, unit weights, one serotype, bi-Weibull humans,
+Weibull(4, 23 d) mosquitoes, external introductions), isolated and ragged
+communities (1 human; the maximum of 8,192 humans and slots; a community with
+no mosquito slots), large Poisson means (the PTRS branch), Beta(2,5) immunity
+and two simulations of the 256-simulation batch.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from arbo_gpu_fit_amd import configs
+from arbo_gpu_fit_amd.engine import SimSpec
+
+
+def case_config0():
+    specs, T = configs.config0()
+    return specs, T
+
+
+def case_tiny_isolated():
+    p = configs.base_params(n_serotypes=1, beta_hm=1.5, beta_mh=1.5, include_self=0, iip_min=1, iip_max=2,
+                            recovery=2.0, eip=3.0, seed_frac_h=0.1, seed_frac_m=0.1, external_infection=2.0)
+    return [SimSpec(1, 1, np.array([50], np.uint32), p, 11)], 120
+
+
+def case_reference_params():
+    p = configs.reference_params(seed_frac_h=0.02, seed_frac_m=0.02, beta_hm=0.4, beta_mh=0.4)
+    return [SimSpec(4, 4, np.full(16, 200, np.uint32), p, 12)], 200
+
+
+def case_ragged():
+    humans = np.array([1, 7, 8192, 13, 64, 65, 255, 256, 257, 1000, 2, 3, 4, 5, 999], np.uint32)
+    slots = np.array([0, 16, 8192, 30, 100, 200, 511, 512, 513, 3000, 1, 8, 9, 7, 2500], np.uint32)
+    p = configs.base_params(n_serotypes=3, beta_hm=0.8, beta_mh=0.8, seed_frac_h=0.02, seed_frac_m=0.02,
+                            long_distance=0.2)
+    return [SimSpec(5, 3, humans, p, 13, mosq_slots=slots)], 60
+
+
+def case_beta25():
+    p = configs.base_params(immunity_mode=1)
+    return [SimSpec(12, 12, np.full(144, 500, np.uint32), p, 14)], 120
+
+
+def case_high_lambda():
+    p = configs.base_params(beta_hm=0.6, beta_mh=0.6, seed_frac_h=0.05, seed_frac_m=0.05, long_distance=0.05)
+    return [SimSpec(3, 3, np.full(9, 2000, np.uint32), p, 15)], 60
+
+
+def case_config2_pair():
+    sims = [configs.config2_sim(i) for i in range(24)]
+    sims.sort(key=lambda s: int(s.humans.sum()))
+    return sims[:2], 40
+
+
+CASES = {
+    "config0": case_config0,
+    "tiny_isolated": case_tiny_isolated,
+    "reference_params": case_reference_params,
+    "ragged": case_ragged,
+    "beta25": case_beta25,
+    "high_lambda": case_high_lambda,
+    "config2_pair": case_config2_pair,
+}
+
+# days at which the full census is compared (plus the last day)
+CHECK_DAYS = (0, 1, 7)
