@@ -4,8 +4,9 @@
 //   human words   u32  h_word[]  (4-element aligned per community)
 //   human next    u16  h_next[]
 //   human birth   i32  h_birth[]   (cold: written at rebirth only)
-//   mosquito word u16  m_word[]  (8-element aligned per community)
+//   mosquito word u16  m_word[]  (32-element aligned per community)
 //   mosquito next u16  m_next[]
+//   alive bitmap  u32  alive[]   (bit j of word m_off/32 + j/32: slot alive after emergence)
 // A community is the unit of work of both day kernels; its descriptor gives
 // the offsets of its members and of its CSR row and in-edges.
 #pragma once
@@ -13,10 +14,10 @@
 
 namespace arbo {
 
-constexpr uint32_t kBlock = 256;          // threads per workgroup (4 wavefronts)
+constexpr uint32_t kWave = 64;            // threads per workgroup: one wavefront per community
 constexpr uint32_t kCntPerComm = 72;      // event counts: [dir 2][serotype 4][row slot 9]
 constexpr uint32_t kHumanAlign = 4;       // humans per 16-byte vector
-constexpr uint32_t kMosqAlign = 8;        // mosquito slots per 16-byte vector
+constexpr uint32_t kMosqAlign = 32;       // mosquito slots per alive-bitmap word (and 4 vectors)
 
 struct CommDesc {
     uint64_t h_off;       // global offset of the first human
@@ -82,7 +83,8 @@ struct EngineView {
     int32_t* h_birth;
     uint16_t* m_word;
     uint16_t* m_next;
-    uint32_t* cnt;                 // [G][72]
+    uint32_t* alive;               // [M/32] alive-slot bitmap of the current day
+    uint32_t* cnt;                 // [G][72] event counts of the current day
     unsigned long long* lam;       // [2 parity][n_sims][8]  long-distance rate, 2^-24 fixed point
     uint32_t* series;
     const uint32_t* csr_to;        // simulation-local destination
@@ -95,7 +97,7 @@ struct EngineView {
     uint32_t n_sims;
     uint32_t n_comm;
     uint32_t max_days;
-    uint32_t pad;
+    uint32_t death_m_lds;          // entries of the largest mosquito death table (LDS staging)
 };
 
 }  // namespace arbo

@@ -25,7 +25,6 @@ hipError_t launch_demography(const EngineView& v, hipStream_t st);
 hipError_t launch_events(const EngineView& v, hipStream_t st);
 hipError_t launch_advance(uint32_t* day, hipStream_t st);
 hipError_t launch_fill(uint32_t* p, uint64_t n, uint32_t value, hipStream_t st);
-size_t events_lds_bytes();
 }  // namespace arbo
 
 using namespace arbo;
@@ -63,7 +62,7 @@ struct ArboEngine {
     // device memory
     DevBuf<CommDesc> comm;
     DevBuf<SimDesc> simd;
-    DevBuf<uint32_t> h_word, cnt, series, csr_thr, in_edge, in_slot, tables, day_ctr;
+    DevBuf<uint32_t> h_word, alive, cnt, series, csr_thr, in_edge, in_slot, tables, day_ctr;
     DevBuf<uint16_t> h_next, m_word, m_next;
     DevBuf<int32_t> h_birth;
     DevBuf<unsigned long long> lam;
@@ -116,7 +115,7 @@ void free_batch(ArboEngine* e) {
     if (e->graph_exec) { (void)hipGraphExecDestroy(e->graph_exec); e->graph_exec = nullptr; }
     if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
     dev_free(e->comm); dev_free(e->simd);
-    dev_free(e->h_word); dev_free(e->cnt); dev_free(e->series); dev_free(e->csr_thr);
+    dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->series); dev_free(e->csr_thr);
     dev_free(e->in_edge); dev_free(e->in_slot); dev_free(e->tables); dev_free(e->day_ctr);
     dev_free(e->h_next); dev_free(e->m_word); dev_free(e->m_next); dev_free(e->h_birth);
     dev_free(e->lam); dev_free(e->dens);
@@ -279,7 +278,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     std::map<std::vector<uint32_t>, uint32_t> seen_u32;
     std::map<std::vector<float>, uint32_t> seen_f32;
     uint64_t h_tot = 0, m_tot = 0, ser_tot = 0;
-    uint32_t max_n = 0, max_cap = 0;
+    uint32_t max_n = 0, max_cap = 0, death_m_lds = 1;
     for (uint32_t i = 0; i < n_sims; ++i) {
         const SimTables& t = tabs[i];
         const ArboParams& p = params[i];
@@ -323,6 +322,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         for (uint32_t a = 1; a < kMosqAgeTable; ++a)
             if (t.death_m[a] >= kOne24) { dmax = a; break; }
         sd.death_m_max = dmax;
+        death_m_lds = std::max(death_m_lds, dmax + 1);
         sd.series_off = sh.series_off;
 
         const uint32_t row_base = (uint32_t)csr_thr.size();
@@ -372,7 +372,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     if ((rc = dev_alloc(e, e->comm, comm.size())) || (rc = dev_alloc(e, e->simd, n_sims)) ||
         (rc = dev_alloc(e, e->h_word, h_tot)) || (rc = dev_alloc(e, e->h_next, h_tot)) ||
         (rc = dev_alloc(e, e->h_birth, h_tot)) || (rc = dev_alloc(e, e->m_word, m_tot)) ||
-        (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->cnt, comm.size() * kCntPerComm)) ||
+        (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kCntPerComm)) ||
         (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->series, ser_tot)) ||
         (rc = dev_alloc(e, e->csr_thr, csr_thr.size())) || (rc = dev_alloc(e, e->in_edge, in_edge.size())) ||
         (rc = dev_alloc(e, e->in_slot, in_slot.size())) || (rc = dev_alloc(e, e->tables, table_pool.size())) ||
@@ -403,6 +403,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.h_birth = e->h_birth.p;
     v.m_word = e->m_word.p;
     v.m_next = e->m_next.p;
+    v.alive = e->alive.p;
     v.cnt = e->cnt.p;
     v.lam = e->lam.p;
     v.series = e->series.p;
@@ -416,6 +417,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.n_sims = n_sims;
     v.n_comm = e->n_comm;
     v.max_days = max_days;
+    v.death_m_lds = death_m_lds;
 
     HIP_TRY(e, launch_init(v, e->stream));
     HIP_TRY(e, hipStreamSynchronize(e->stream));

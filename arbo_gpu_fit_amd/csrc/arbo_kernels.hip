@@ -1,27 +1,33 @@
 // arbo_kernels.hip — the day-loop kernels of the dengue IBM for gfx950.
 //
 // One simulated day is two launches over every community of every simulation
-// of the batch (one 256-thread workgroup = four wave64s per community):
+// of the batch, one wave64 workgroup per community (up to 32 communities in
+// flight per CU, so the dependent-latency chain of one community is hidden by
+// the others):
 //
 //  k_day_demography  ndemographic.cu:32-105 + mdemographic.cu:22-150 +
 //                    nVisitingInfected/mVisitingInfected (nepidemic.cu:28-142,
 //                    mepidemic.cu:28-138) + dataCollect (datacollect.cu:79-189)
-//     streams the community's packed census once (16-byte vector loads of
-//     u32 human words and u16 mosquito words), applies deaths, rebirths and
-//     E->I / I->S transitions, recounts the infectious per serotype with wave
-//     reductions, fills the lowest-index dead mosquito slots up to the seasonal
-//     target with a workgroup prefix scan over the dead-slot bitmap, draws the
-//     Poisson event counts of the community as a source, spreads the local
-//     events over its CSR row with the alias table into an LDS histogram, and
-//     adds its counters into the series row of the next day.
+//     streams the community's packed census once (16-byte vector loads, four
+//     in flight per lane, of u32 human words and u16 mosquito words), applies
+//     deaths, rebirths and E->I / I->S transitions, recounts the infectious
+//     per serotype with wave reductions, fills the lowest-index dead mosquito
+//     slots up to the seasonal target with a wave prefix scan over the
+//     dead-slot bitmap (newborn lifetimes from an LDS copy of the mosquito
+//     lifetime table), publishes the alive-slot bitmap, draws the Poisson
+//     event counts of the community as a source, spreads the local events
+//     over its CSR row with the alias table into an LDS histogram, and adds
+//     its counters into the series row of the next day.
 //
 //  k_day_events      mepidemic.cu:143-246 (nmTransmission) +
 //                    nepidemic.cu:146-319 (mnTransmission)
 //     gathers the event counts of the community's in-edges, draws its
 //     long-distance arrivals, orders all its events (serotype rotation,
-//     source, index) and applies them with an LDS claim-min: the earliest
-//     event whose target was eligible at the start of the pass wins, which is
-//     exactly the serial outcome, with no global atomics on the census.
+//     source, index) and applies them in chunks with an LDS claim-min hash
+//     table: the earliest event whose target was eligible at the start of the
+//     chunk wins, which is exactly the serial outcome.  Mosquito targets are
+//     the q-th alive slot, selected from the alive bitmap without touching the
+//     census.  No global atomics on the census.
 //
 // All randomness is Philox keyed by (seed; entity, day, purpose, index), so a
 // simulation's results do not depend on launch shape or batch composition.
@@ -33,8 +39,19 @@
 
 namespace arbo {
 
-constexpr uint32_t kSlotsPerThread = ARBO_MAX_COMMUNITY_SLOTS / kBlock;  // 32
-static_assert(kSlotsPerThread == 32, "one u32 bitmap word per thread");
+constexpr uint32_t kMaxWords = ARBO_MAX_COMMUNITY_SLOTS / 32;  // alive-bitmap words per community
+constexpr uint32_t kWordsPerLane = kMaxWords / kWave;          // 4
+constexpr uint32_t kUnroll = 4;                                // 16-byte loads in flight per lane
+constexpr uint32_t kChunk = 512;                               // events per claim pass
+constexpr uint32_t kPerLane = kChunk / kWave;                  // 8
+constexpr uint32_t kHash = 1024;                               // claim-table entries (2 x chunk)
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kKeyBits = 14;                              // member index < 2^14
+constexpr uint32_t kKeyMask = (1u << kKeyBits) - 1u;
+constexpr uint32_t kHumanTable = 65536;
+static_assert(kWordsPerLane == 4, "four bitmap words per lane");
+static_assert(ARBO_MAX_COMMUNITY_SLOTS <= (1u << kKeyBits) && ARBO_MAX_COMMUNITY_HUMANS <= (1u << kKeyBits),
+              "member index must fit the claim key");
 
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 #pragma unroll
@@ -42,31 +59,22 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return x;
 }
 
-// Exclusive scan of one value per thread over the 256-thread workgroup.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_wave, uint32_t& total) {
-    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+// Exclusive prefix sum over the wavefront; `total` gets the wave sum.
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, uint32_t& total) {
+    const uint32_t lane = threadIdx.x;
     uint32_t incl = x;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(incl, o, 64);
         if (lane >= (uint32_t)o) incl += y;
     }
-    if (lane == 63) s_wave[wid] = incl;
-    __syncthreads();
-    uint32_t base = 0;
-    total = 0;
-#pragma unroll
-    for (uint32_t w = 0; w < kBlock / 64; ++w) {
-        const uint32_t v = s_wave[w];
-        base += (w < wid) ? v : 0u;
-        total += v;
-    }
-    __syncthreads();
-    return base + incl - x;
+    total = __shfl(incl, 63, 64);
+    return incl - x;
 }
 
 // min{a in [1, hi] : K[a] >= k}; hi when K[hi] < k (hi = table end).
-__device__ __forceinline__ uint32_t death_age(const uint32_t* K, uint32_t hi, uint32_t k) {
+template <class P>
+__device__ __forceinline__ uint32_t death_age(P K, uint32_t hi, uint32_t k) {
     if (K[hi] < k) return hi;
     uint32_t lo = 1;
     while (lo < hi) {
@@ -103,52 +111,67 @@ __device__ __forceinline__ uint32_t unpack_pair(uint32_t lo, uint32_t hi, uint32
     return (v >> ((s & 1u) * 16u)) & 0xFFFFu;
 }
 
-constexpr uint32_t kHumanTable = 65536;
+// Bits of bitmap word w that are slots below cap.
+__device__ __forceinline__ uint32_t valid_bits(uint32_t w, uint32_t cap) {
+    const uint32_t first = 32u * w;
+    if (first + 32u <= cap) return 0xFFFFFFFFu;
+    if (first >= cap) return 0u;
+    return (1u << (cap - first)) - 1u;
+}
 
-struct TableRefs {
-    const uint32_t* death_h;
-    const uint32_t* death_m;
-    uint32_t death_h_max;
-    uint32_t death_m_max;
-};
+// Bits 0 and 16 of m as a 2-bit value (one bit per 16-bit half).
+__device__ __forceinline__ uint32_t pack_halves(uint32_t m) { return (m & 1u) | ((m >> 15) & 2u); }
 
-__device__ __forceinline__ TableRefs table_refs(const EngineView& v, const SimDesc& sd) {
-    TableRefs t;
-    t.death_h = v.tables + sd.death_h_off;
-    t.death_m = v.tables + sd.death_m_off;
-    t.death_h_max = kHumanTable - 1;
-    t.death_m_max = sd.death_m_max;
-    return t;
+// Position of the k-th (0-based) set bit of x; k < popcount(x).
+__device__ __forceinline__ uint32_t select_in_word(uint32_t x, uint32_t k) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t width = 16; width >= 1; width >>= 1) {
+        const uint32_t c = __popc(x & ((1u << width) - 1u));
+        if (k >= c) { k -= c; x >>= width; pos += width; }
+    }
+    return pos;
+}
+
+// The q-th set bit of a bitmap given its words and their exclusive popcount
+// prefix: word index w and bit position pos.
+__device__ __forceinline__ void select_bit(uint32_t q, const uint32_t* word, const uint32_t* wpre, uint32_t nwords,
+                                           uint32_t& w, uint32_t& pos) {
+    uint32_t lo = 0, hi = nwords - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (wpre[mid] <= q) lo = mid; else hi = mid - 1;
+    }
+    w = lo;
+    pos = select_in_word(word[lo], q - wpre[lo]);
 }
 
 // ---------------------------------------------------------------------------
 // t = 0 census: demographicintial.cu:86-297, epidemicinitial.cu:27-99.
-__global__ __launch_bounds__(kBlock) void k_init(EngineView v) {
-    const uint32_t g = blockIdx.x;
+__global__ __launch_bounds__(kWave) void k_init(EngineView v) {
+    const uint32_t g = blockIdx.x, lane = threadIdx.x;
     const CommDesc cd = v.comm[g];
     const SimDesc sd = v.sims[cd.sim];
     const Key key{sd.k0, sd.k1};
-    const uint32_t S = sd.S, tid = threadIdx.x;
-    const TableRefs tb = table_refs(v, sd);
+    const uint32_t S = sd.S;
+    const uint32_t* death_h = v.tables + sd.death_h_off;
+    const uint32_t* death_m = v.tables + sd.death_m_off;
     const uint32_t* init_h = v.tables + sd.init_h_off;
     const uint32_t* init_m = v.tables + sd.init_m_off;
     const uint32_t* immune = v.tables + sd.immune_off + (size_t)cd.c * S;
-    __shared__ uint32_t s_red[8];
-    __shared__ uint32_t s_alive0;
-    if (tid < 8) s_red[tid] = 0;
-    if (tid == 0) {
+    uint32_t alive0;
+    {
         const U4 r = draw(key, cd.c, INIT_DAY, P_INIT_MT, 0);
-        uint32_t a = stoch_round(v.dens[sd.dens_off] * cd.base, r.x);
-        s_alive0 = a < cd.cap ? a : cd.cap;
+        const uint32_t a = stoch_round(v.dens[sd.dens_off] * cd.base, r.x);
+        alive0 = a < cd.cap ? a : cd.cap;
     }
-    __syncthreads();
-    uint32_t ih01 = 0, ih23 = 0, im01 = 0, im23 = 0;
-    for (uint32_t i = tid; i < cd.n; i += kBlock) {
+    uint32_t ih01 = 0, ih23 = 0;
+    for (uint32_t i = lane; i < cd.n; i += kWave) {
         const uint32_t h = cd.h_base + i;
         const U4 r0 = draw(key, h, INIT_DAY, P_INIT_H, 0);
         const uint32_t y = pick_index(init_h, 151, (r0.x >> 8) + 1);
         const uint32_t a0 = 365u * y + mulhi(r0.y, 365);
-        const uint32_t da = death_age_given(tb.death_h, tb.death_h_max, a0, r0.z);
+        const uint32_t da = death_age_given(death_h, kHumanTable - 1, a0, r0.z);
         const int32_t ddi = (int32_t)da - (int32_t)(a0 + 1);
         const uint32_t dd = ddi < 0 ? 0u : (uint32_t)ddi;
         const U4 r1 = draw(key, h, INIT_DAY, P_INIT_H, 1);
@@ -172,195 +195,257 @@ __global__ __launch_bounds__(kBlock) void k_init(EngineView v) {
         v.h_next[gi] = (uint16_t)nxt;
         v.h_birth[gi] = -(int32_t)(a0 + 1);
     }
-    const uint32_t alive0 = s_alive0;
-    for (uint32_t j = tid; j < cd.cap; j += kBlock) {
+    for (uint32_t j = lane; j < cd.cap; j += kWave) {
         const uint64_t gm = cd.m_off + j;
         v.m_next[gm] = 0;
         if (j >= alive0) { v.m_word[gm] = 3; continue; }
         const uint32_t m = cd.m_base + j;
         const U4 r0 = draw(key, m, INIT_DAY, P_INIT_M, 0);
         const uint32_t a0 = pick_index(init_m, 301, (r0.x >> 8) + 1);
-        const uint32_t da = death_age_given(tb.death_m, tb.death_m_max, a0, r0.z);
+        const uint32_t da = death_age_given(death_m, sd.death_m_max, a0, r0.z);
         const uint32_t dd12 = (uint32_t)((int32_t)da - (int32_t)(a0 + 1)) & 0xFFFu;
         uint32_t st = 0, strain = 0;
         if ((r0.w >> 8) < sd.seed_m) {
             st = 2;
             strain = mulhi(r0.y, S);
-            add_pair(im01, im23, strain);
         }
         v.m_word[gm] = (uint16_t)((dd12 << 4) | (strain << 2) | st);
     }
-    ih01 = wave_sum(ih01); ih23 = wave_sum(ih23);
-    if ((tid & 63u) == 0) { atomicAdd(&s_red[0], ih01); atomicAdd(&s_red[1], ih23); }
-    __syncthreads();
+    ih01 = wave_sum(ih01);
+    ih23 = wave_sum(ih23);
     const SeriesIdx si = series_index(sd.series_off, v.max_days, S);
-    if (tid == 0) {
+    if (lane == 0) {
         atomicAdd(&v.series[si.n_count], cd.n);
         atomicAdd(&v.series[si.m_count], alive0);
     }
-    if (tid < S) {
-        const uint32_t ih = unpack_pair(s_red[0], s_red[1], tid);
-        atomicAdd(&v.series[si.n_inf + tid], ih);
-        if (cd.c == 0) v.series[si.n_one + tid] = ih;
+    if (lane < S) {
+        const uint32_t ih = unpack_pair(ih01, ih23, lane);
+        atomicAdd(&v.series[si.n_inf + lane], ih);
+        if (cd.c == 0) v.series[si.n_one + lane] = ih;
     }
-    (void)im01; (void)im23;
 }
 
 // ---------------------------------------------------------------------------
 // Day kernel A: demography, counters, event counts, series row t+1.
-__global__ __launch_bounds__(kBlock) void k_day_demography(EngineView v) {
-    const uint32_t g = blockIdx.x;
+
+struct HumanCount { uint32_t ih01, ih23, inc01, inc23; };
+
+// One human of ndemographic.cu:57-103 on the packed word w.
+__device__ __forceinline__ void human_day(const EngineView& v, const CommDesc& cd, const SimDesc& sd, Key key,
+                                          uint32_t t, uint32_t i, uint32_t w, HumanCount& hc) {
+    const uint32_t st = w & 3u;
+    const uint64_t gi = cd.h_off + i;
+    if (t >= (w >> 8)) {  // death and instant rebirth
+        const U4 r = draw(key, cd.h_base + i, t, P_H_BIRTH, 0);
+        const uint32_t da = death_age(v.tables + sd.death_h_off, kHumanTable - 1, (r.x >> 8) + 1);
+        v.h_word[gi] = (t + da) << 8;
+        v.h_birth[gi] = (int32_t)t;
+        return;
+    }
+    if (st == 0) return;
+    const uint32_t strain = (w >> 2) & 3u;
+    const uint16_t nx = v.h_next[gi];
+    const bool due = (int16_t)(uint16_t)(t - nx) > 0;
+    if (st == 2) {
+        if (due) v.h_word[gi] = w & ~0xFu;   // recovered
+        else add_pair(hc.ih01, hc.ih23, strain);
+    } else if (due) {                        // E -> I
+        const U4 r = draw(key, cd.h_base + i, t, P_H_REC, 0);
+        v.h_next[gi] = (uint16_t)(nx + stoch_round(sd.recovery, r.x));
+        v.h_word[gi] = (w & ~3u) | 2u;
+        add_pair(hc.inc01, hc.inc23, strain);
+        add_pair(hc.ih01, hc.ih23, strain);
+    }
+}
+
+__global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
+    const uint32_t g = blockIdx.x, lane = threadIdx.x;
     const uint32_t t = *v.day;
     const CommDesc cd = v.comm[g];
     const SimDesc sd = v.sims[cd.sim];
     const Key key{sd.k0, sd.k1};
-    const uint32_t S = sd.S, tid = threadIdx.x;
-    const TableRefs tb = table_refs(v, sd);
+    const uint32_t S = sd.S;
 
-    __shared__ uint32_t s_red[8];     // ih01 ih23 inc01 inc23 im01 im23 alive dead
-    __shared__ uint8_t s_dead[ARBO_MAX_COMMUNITY_SLOTS / 8];
-    __shared__ uint8_t s_died[ARBO_MAX_COMMUNITY_SLOTS / 8];
-    __shared__ uint32_t s_wave[kBlock / 64];
-    __shared__ uint32_t s_births;
+    extern __shared__ uint32_t s_dtab[];                            // mosquito lifetime thresholds
+    __shared__ uint32_t s_dead[kMaxWords];                          // dead-slot bitmap
+    __shared__ uint32_t s_died[kMaxWords];                          // died today
+    __shared__ uint32_t s_dpre[kMaxWords];                          // dead slots before each word
     __shared__ uint32_t s_n[8];
     __shared__ uint32_t s_hist[kCntPerComm];
-    if (tid < 8) s_red[tid] = 0;
-    if (tid < kCntPerComm) s_hist[tid] = 0;
-    __syncthreads();
+    uint8_t* s_dead8 = reinterpret_cast<uint8_t*>(s_dead);
+    uint8_t* s_died8 = reinterpret_cast<uint8_t*>(s_died);
 
-    // ---- humans: death/rebirth, I->S, E->I (ndemographic.cu:57-103) ----
-    uint32_t ih01 = 0, ih23 = 0, inc01 = 0, inc23 = 0;
+    for (uint32_t a = lane; a <= sd.death_m_max; a += kWave) s_dtab[a] = v.tables[sd.death_m_off + a];
+    for (uint32_t i = lane; i < kCntPerComm; i += kWave) s_hist[i] = 0;
+
+    // ---- humans ----
+    HumanCount hc{0, 0, 0, 0};
     {
         const uint4* w4p = reinterpret_cast<const uint4*>(v.h_word + cd.h_off);
         const uint32_t nvec = (cd.n + 3) >> 2;
-        for (uint32_t q = tid; q < nvec; q += kBlock) {
-            const uint4 w4 = w4p[q];
-            const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+        for (uint32_t q0 = 0; q0 < nvec; q0 += kUnroll * kWave) {
+            uint4 w4[kUnroll];
 #pragma unroll
-            for (uint32_t e = 0; e < 4; ++e) {
-                const uint32_t i = 4 * q + e;
-                if (i >= cd.n) break;
-                const uint32_t w = ws[e];
-                const uint32_t st = w & 3u;
-                const uint64_t gi = cd.h_off + i;
-                if (t >= (w >> 8)) {
-                    const U4 r = draw(key, cd.h_base + i, t, P_H_BIRTH, 0);
-                    const uint32_t da = death_age(tb.death_h, tb.death_h_max, (r.x >> 8) + 1);
-                    v.h_word[gi] = (t + da) << 8;
-                    v.h_birth[gi] = (int32_t)t;
-                } else if (st != 0) {
-                    const uint32_t strain = (w >> 2) & 3u;
-                    const uint16_t nx = v.h_next[gi];
-                    const bool due = (int16_t)(uint16_t)(t - nx) > 0;
-                    if (st == 2) {
-                        if (due) v.h_word[gi] = w & ~0xFu;
-                        else add_pair(ih01, ih23, strain);
-                    } else if (due) {
-                        const U4 r = draw(key, cd.h_base + i, t, P_H_REC, 0);
-                        v.h_next[gi] = (uint16_t)(nx + stoch_round(sd.recovery, r.x));
-                        v.h_word[gi] = (w & ~3u) | 2u;
-                        add_pair(inc01, inc23, strain);
-                        add_pair(ih01, ih23, strain);
-                    }
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                const uint32_t q = q0 + u * kWave + lane;
+                if (q < nvec) w4[u] = w4p[q];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                const uint32_t q = q0 + u * kWave + lane;
+                if (q >= nvec) continue;
+                const uint32_t ws[4] = {w4[u].x, w4[u].y, w4[u].z, w4[u].w};
+#pragma unroll
+                for (uint32_t e = 0; e < 4; ++e) {
+                    const uint32_t i = 4 * q + e;
+                    if (i < cd.n) human_day(v, cd, sd, key, t, i, ws[e], hc);
                 }
             }
         }
     }
 
-    // ---- mosquitoes: death and E->I (mdemographic.cu:68-98) ----
-    uint32_t im01 = 0, im23 = 0, alive = 0, ndead = 0;
+    // ---- mosquitoes: death and E->I (mdemographic.cu:68-98), two slots per u32 op ----
+    uint32_t im01 = 0, im23 = 0, alive = 0;
     const uint32_t mvec = (cd.cap + 7) >> 3;
+    const uint32_t nwords = (cd.cap + 31) >> 5;
     {
+        // per 16-bit half: t mod 4096 plus a 4096 guard, so (t - death day) mod 4096
+        // is the half's low 12 bits with no borrow between halves
+        const uint32_t T2 = ((t & 0xFFFu) * 0x00010001u) | 0x10001000u;
         const uint4* m4p = reinterpret_cast<const uint4*>(v.m_word + cd.m_off);
-        for (uint32_t q = tid; q < mvec; q += kBlock) {
-            const uint4 raw = m4p[q];
-            const uint32_t rw[4] = {raw.x, raw.y, raw.z, raw.w};
-            uint32_t dmask = 0, kmask = 0;
+        for (uint32_t q0 = 0; q0 < mvec; q0 += kUnroll * kWave) {
+            uint4 raw[kUnroll];
 #pragma unroll
-            for (uint32_t e = 0; e < 8; ++e) {
-                const uint32_t j = 8 * q + e;
-                if (j >= cd.cap) break;
-                const uint32_t w = (rw[e >> 1] >> ((e & 1u) * 16u)) & 0xFFFFu;
-                uint32_t st = w & 3u;
-                if (st == 3) { dmask |= 1u << e; continue; }
-                if (((t - (w >> 4)) & 0xFFFu) < 2048u) { dmask |= 1u << e; kmask |= 1u << e; continue; }
-                if (st == 1) {
-                    const uint64_t gm = cd.m_off + j;
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                const uint32_t q = q0 + u * kWave + lane;
+                if (q < mvec) raw[u] = m4p[q];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kUnroll; ++u) {
+                const uint32_t q = q0 + u * kWave + lane;
+                if (q >= mvec) continue;
+                const uint32_t rw[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
+                uint32_t dead8 = 0, died8 = 0, e8 = 0, i8 = 0, sa8 = 0, sb8 = 0;
+#pragma unroll
+                for (uint32_t p = 0; p < 4; ++p) {
+                    const uint32_t x = rw[p];
+                    const uint32_t d0 = x & (x >> 1) & 0x00010001u;                    // status 3: empty slot
+                    const uint32_t dying = (~(T2 - ((x >> 4) & 0x0FFF0FFFu)) >> 11) & 0x00010001u & ~d0;
+                    const uint32_t live = 0x00010001u & ~(d0 | dying);
+                    dead8 |= pack_halves(d0 | dying) << (2 * p);
+                    died8 |= pack_halves(dying) << (2 * p);
+                    e8 |= pack_halves(x & ~(x >> 1) & live) << (2 * p);               // status 1
+                    i8 |= pack_halves((x >> 1) & ~x & live) << (2 * p);               // status 2
+                    sa8 |= pack_halves(x >> 2) << (2 * p);
+                    sb8 |= pack_halves(x >> 3) << (2 * p);
+                }
+                const uint32_t valid8 = (8 * q + 8 <= cd.cap) ? 0xFFu : ((1u << (cd.cap - 8 * q)) - 1u);
+                dead8 &= valid8;
+                died8 &= valid8;
+                e8 &= valid8;
+                i8 &= valid8;
+                while (e8) {  // E -> I when the extrinsic incubation is over
+                    const uint32_t e = __ffs(e8) - 1;
+                    e8 &= e8 - 1;
+                    const uint64_t gm = cd.m_off + 8 * q + e;
                     if ((int16_t)(uint16_t)(t - v.m_next[gm]) > 0) {
-                        st = 2;
+                        const uint32_t w = (rw[e >> 1] >> ((e & 1u) * 16u)) & 0xFFFFu;
                         v.m_word[gm] = (uint16_t)((w & ~3u) | 2u);
+                        i8 |= 1u << e;
                     }
                 }
-                alive++;
-                if (st == 2) add_pair(im01, im23, (w >> 2) & 3u);
+                alive += __popc(valid8 & ~dead8);
+                im01 += __popc(i8 & ~sa8 & ~sb8) + (__popc(i8 & sa8 & ~sb8) << 16);
+                im23 += __popc(i8 & ~sa8 & sb8) + (__popc(i8 & sa8 & sb8) << 16);
+                s_dead8[q] = (uint8_t)dead8;
+                s_died8[q] = (uint8_t)died8;
             }
-            s_dead[q] = (uint8_t)dmask;
-            s_died[q] = (uint8_t)kmask;
-            ndead += __popc(dmask);
         }
+        // the tail bytes of the last bitmap word
+        for (uint32_t q = mvec + lane; q < 4 * nwords; q += kWave) { s_dead8[q] = 0; s_died8[q] = 0; }
     }
-    ih01 = wave_sum(ih01); ih23 = wave_sum(ih23);
-    inc01 = wave_sum(inc01); inc23 = wave_sum(inc23);
-    im01 = wave_sum(im01); im23 = wave_sum(im23);
-    alive = wave_sum(alive); ndead = wave_sum(ndead);
-    if ((tid & 63u) == 0) {
-        atomicAdd(&s_red[0], ih01); atomicAdd(&s_red[1], ih23);
-        atomicAdd(&s_red[2], inc01); atomicAdd(&s_red[3], inc23);
-        atomicAdd(&s_red[4], im01); atomicAdd(&s_red[5], im23);
-        atomicAdd(&s_red[6], alive); atomicAdd(&s_red[7], ndead);
-    }
-    __syncthreads();
-    const uint32_t alive_c = s_red[6], dead_c = s_red[7];
+    const uint32_t ih01 = wave_sum(hc.ih01), ih23 = wave_sum(hc.ih23);
+    const uint32_t inc01 = wave_sum(hc.inc01), inc23 = wave_sum(hc.inc23);
+    im01 = wave_sum(im01);
+    im23 = wave_sum(im23);
+    const uint32_t alive_c = wave_sum(alive);
+    const uint32_t dead_c = cd.cap - alive_c;
 
     // ---- emergence into the lowest-index dead slots (mdemographic.cu:104-142) ----
-    if (tid == 0) {
+    uint32_t births;
+    {
         const U4 r = draw(key, cd.c, t, P_M_TARGET, 0);
         const uint32_t target = stoch_round(v.dens[sd.dens_off + t] * cd.base, r.x);
-        uint32_t b = target > alive_c ? target - alive_c : 0u;
-        s_births = b < dead_c ? b : dead_c;
+        const uint32_t b = target > alive_c ? target - alive_c : 0u;
+        births = b < dead_c ? b : dead_c;
     }
     __syncthreads();
-    const uint32_t births = s_births;
-    if (dead_c > 0) {
-        // thread tid owns bitmap bytes [4 tid, 4 tid + 4): slots [32 tid, 32 tid + 32)
-        uint32_t dword = 0, kword = 0;
+    uint32_t dw[kWordsPerLane], kw[kWordsPerLane], wpre[kWordsPerLane];
+    {
+        uint32_t cnt = 0;
 #pragma unroll
-        for (uint32_t b = 0; b < 4; ++b) {
-            const uint32_t q = 4 * tid + b;
-            if (q < mvec) {
-                dword |= (uint32_t)s_dead[q] << (8 * b);
-                kword |= (uint32_t)s_died[q] << (8 * b);
-            }
+        for (uint32_t k = 0; k < kWordsPerLane; ++k) {
+            const uint32_t w = kWordsPerLane * lane + k;
+            dw[k] = w < nwords ? s_dead[w] : 0u;
+            kw[k] = w < nwords ? s_died[w] : 0u;
+            cnt += __popc(dw[k]);
         }
         uint32_t total;
-        uint32_t rank = block_exclusive_scan(__popc(dword), s_wave, total);
-        if ((rank < births) || kword) {
-            uint32_t bits = dword;
-            while (bits) {
-                const uint32_t e = __ffs(bits) - 1;
-                bits &= bits - 1;
-                const uint32_t jj = 32 * tid + e;
-                const uint64_t gm = cd.m_off + jj;
-                if (rank < births) {
-                    const U4 r = draw(key, cd.m_base + jj, t, P_M_BIRTH, 0);
-                    const uint32_t da = death_age(tb.death_m, tb.death_m_max, (r.x >> 8) + 1);
-                    v.m_word[gm] = (uint16_t)(((t + da) & 0xFFFu) << 4);
-                } else if ((kword >> e) & 1u) {
-                    v.m_word[gm] = 3;
-                }
-                rank++;
-            }
+        uint32_t pre = wave_exclusive_scan(cnt, total);
+#pragma unroll
+        for (uint32_t k = 0; k < kWordsPerLane; ++k) {
+            const uint32_t w = kWordsPerLane * lane + k;
+            wpre[k] = pre;
+            if (w < nwords) s_dpre[w] = pre;
+            pre += __popc(dw[k]);
         }
+    }
+    __syncthreads();
+    // births i = 0..births-1 fill the i-th dead slot; births 4g..4g+3 share one Philox block
+    for (uint32_t g4 = lane; 4 * g4 < births; g4 += kWave) {
+        const U4 r = draw(key, cd.c, t, P_M_BIRTH, g4);
+        const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
+        uint32_t w, pos;
+        select_bit(4 * g4, s_dead, s_dpre, nwords, w, pos);
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+            if (4 * g4 + j >= births) break;
+            if (j > 0) {  // next dead slot
+                uint32_t bits = s_dead[w] & ~((2u << pos) - 1u);
+                while (!bits) bits = s_dead[++w];
+                pos = __ffs(bits) - 1;
+            }
+            const uint32_t da = death_age(s_dtab, sd.death_m_max, (rr[j] >> 8) + 1);
+            v.m_word[cd.m_off + 32u * w + pos] = (uint16_t)(((t + da) & 0xFFFu) << 4);
+        }
+    }
+    // slots that died today and stay empty; the alive bitmap of the day
+#pragma unroll
+    for (uint32_t k = 0; k < kWordsPerLane; ++k) {
+        const uint32_t w = kWordsPerLane * lane + k;
+        if (w >= nwords) continue;
+        const uint32_t d = dw[k], c = __popc(d);
+        uint32_t reborn;
+        if (wpre[k] + c <= births) reborn = d;
+        else if (wpre[k] >= births) reborn = 0u;
+        else reborn = d & ((1u << select_in_word(d, births - wpre[k])) - 1u);
+        uint32_t stay = kw[k] & ~reborn;
+        while (stay) {
+            const uint32_t e = __ffs(stay) - 1;
+            stay &= stay - 1;
+            v.m_word[cd.m_off + 32u * w + e] = 3;
+        }
+        v.alive[(cd.m_off >> 5) + w] = ~(d & ~reborn) & valid_bits(w, cd.cap);
     }
     const uint32_t Mc = alive_c + births;
 
     // ---- event counts of this community as a source (nepidemic.cu:49-136) ----
-    if (tid < 8) {
-        const uint32_t dir = tid >> 2, s = tid & 3u;
+    if (lane < 8) {
+        const uint32_t dir = lane >> 2, s = lane & 3u;
         uint32_t n = 0;
         unsigned long long fx = 0;
         if (s < S) {
-            const uint32_t I = dir == DIR_HM ? unpack_pair(s_red[0], s_red[1], s) : unpack_pair(s_red[4], s_red[5], s);
+            const uint32_t I = dir == DIR_HM ? unpack_pair(ih01, ih23, s) : unpack_pair(im01, im23, s);
             if (I > 0) {
                 const float lam = dir == DIR_HM ? (float)Mc / (float)cd.n * sd.beta_hm * (float)I
                                                 : sd.beta_mh * (float)I;
@@ -371,8 +456,8 @@ __global__ __launch_bounds__(kBlock) void k_day_demography(EngineView v) {
                 n = poisson((double)mu_loc, st);
             }
         }
-        s_n[tid] = n;
-        if (fx) atomicAdd(&v.lam[((size_t)(t & 1u) * v.n_sims + cd.sim) * 8 + tid], fx);
+        s_n[lane] = n;
+        if (fx) atomicAdd(&v.lam[((size_t)(t & 1u) * v.n_sims + cd.sim) * 8 + lane], fx);
     }
     __syncthreads();
     uint32_t pre[9];
@@ -380,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_day_demography(EngineView v) {
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) pre[k + 1] = pre[k] + s_n[k];
     const uint32_t E = pre[8];
-    for (uint32_t ev = tid; ev < E; ev += kBlock) {
+    for (uint32_t ev = lane; ev < E; ev += kWave) {
         uint32_t k = 0;
 #pragma unroll
         for (uint32_t q = 1; q < 8; ++q) k += (ev >= pre[q]) ? 1u : 0u;
@@ -392,21 +477,21 @@ __global__ __launch_bounds__(kBlock) void k_day_demography(EngineView v) {
         atomicAdd(&s_hist[k * 9 + col], 1u);
     }
     __syncthreads();
-    if (tid < kCntPerComm) v.cnt[(size_t)g * kCntPerComm + tid] = s_hist[tid];
+    for (uint32_t i = lane; i < kCntPerComm; i += kWave) v.cnt[(size_t)g * kCntPerComm + i] = s_hist[i];
 
     // ---- series row t+1 (datacollect.cu:79-189) ----
     const SeriesIdx si = series_index(sd.series_off, v.max_days, S);
     const uint32_t row = t + 1;
-    if (tid == 0) {
+    if (lane == 0) {
         atomicAdd(&v.series[si.n_count + row], cd.n);
         atomicAdd(&v.series[si.m_count + row], Mc);
     }
-    if (tid < S) {
-        const uint32_t ih = unpack_pair(s_red[0], s_red[1], tid);
-        const uint32_t ic = unpack_pair(s_red[2], s_red[3], tid);
-        atomicAdd(&v.series[si.n_inf + (size_t)row * S + tid], ih);
-        if (ic) atomicAdd(&v.series[si.n_inc + (size_t)row * S + tid], ic);
-        if (cd.c == 0) v.series[si.n_one + (size_t)row * S + tid] = ih;
+    if (lane < S) {
+        const uint32_t ih = unpack_pair(ih01, ih23, lane);
+        const uint32_t ic = unpack_pair(inc01, inc23, lane);
+        if (ih) atomicAdd(&v.series[si.n_inf + (size_t)row * S + lane], ih);
+        if (ic) atomicAdd(&v.series[si.n_inc + (size_t)row * S + lane], ic);
+        if (cd.c == 0) v.series[si.n_one + (size_t)row * S + lane] = ih;
     }
 }
 
@@ -436,118 +521,161 @@ __device__ __forceinline__ EventRef decode_event(uint32_t r, const uint32_t* seg
     return ev;
 }
 
-__global__ __launch_bounds__(kBlock) void k_day_events(EngineView v) {
-    const uint32_t g = blockIdx.x;
+// The q-th alive slot of the community, from its bitmap words and their
+// exclusive popcount prefix.
+__device__ __forceinline__ uint32_t select_alive(uint32_t q, const uint32_t* word, const uint32_t* wpre,
+                                                 uint32_t nwords) {
+    uint32_t w, pos;
+    select_bit(q, word, wpre, nwords, w, pos);
+    return 32u * w + pos;
+}
+
+__device__ __forceinline__ uint32_t claim_slot(uint32_t key) { return (key * 2654435761u) >> 22; }
+
+// Record event `rank` of the chunk on member `key`; the table keeps the
+// smallest rank per member (open addressing, at most kChunk keys in kHash).
+__device__ __forceinline__ void claim(uint32_t* tab, uint32_t key, uint32_t rank) {
+    const uint32_t packed = (rank << kKeyBits) | key;
+    uint32_t h = claim_slot(key);
+    while (true) {
+        uint32_t cur = tab[h];
+        if (cur == kEmpty) {
+            cur = atomicCAS(&tab[h], kEmpty, packed);
+            if (cur == kEmpty) return;
+        }
+        if ((cur & kKeyMask) == key) {
+            atomicMin(&tab[h], packed);
+            return;
+        }
+        h = (h + 1) & (kHash - 1);
+    }
+}
+
+__device__ __forceinline__ uint32_t claimed_rank(const uint32_t* tab, uint32_t key) {
+    uint32_t h = claim_slot(key);
+    while (tab[h] == kEmpty || (tab[h] & kKeyMask) != key) h = (h + 1) & (kHash - 1);
+    return tab[h] >> kKeyBits;
+}
+
+constexpr uint32_t kEligible = 1u << 31;
+
+__global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
+    const uint32_t g = blockIdx.x, lane = threadIdx.x;
     const uint32_t t = *v.day;
     const CommDesc cd = v.comm[g];
     const SimDesc sd = v.sims[cd.sim];
     const Key key{sd.k0, sd.k1};
-    const uint32_t S = sd.S, tid = threadIdx.x;
+    const uint32_t S = sd.S;
     const uint32_t in_len = cd.in_len;
     const uint32_t nseg = S * (in_len + 1);
 
-    extern __shared__ uint32_t s_dyn[];
-    uint32_t* s_claim = s_dyn;                                         // [max(n, cap)]
-    uint16_t* s_alive = reinterpret_cast<uint16_t*>(s_dyn + ARBO_MAX_COMMUNITY_SLOTS);  // [cap]
     __shared__ uint32_t s_src[9], s_slot[9];
-    __shared__ uint32_t s_ecnt[2][4][9];
+    __shared__ uint32_t s_ecnt[8][9];
     __shared__ uint32_t s_nld[8];
     __shared__ uint32_t s_rot[2];
     __shared__ uint32_t s_seg[2][4 * 10 + 1];
-    __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint32_t s_word[kMaxWords];
+    __shared__ uint32_t s_wpre[kMaxWords];
+    __shared__ uint32_t s_tab[kHash];
 
-    if (tid < in_len) {
-        const uint32_t src = v.in_edge[cd.in_off + tid];
-        s_src[tid] = src;
-        s_slot[tid] = v.in_slot[cd.in_off + tid];
+    if (lane < in_len) {
+        s_src[lane] = v.in_edge[cd.in_off + lane];
+        s_slot[lane] = v.in_slot[cd.in_off + lane];
     }
     __syncthreads();
-    for (uint32_t i = tid; i < 8 * in_len; i += kBlock) {
+    for (uint32_t i = lane; i < 8 * in_len; i += kWave) {
         const uint32_t k = i / in_len, e = i % in_len;      // k = dir*4 + s
-        s_ecnt[k >> 2][k & 3u][e] = ((k & 3u) < S) ? v.cnt[(size_t)s_src[e] * kCntPerComm + k * 9 + s_slot[e]] : 0u;
+        s_ecnt[k][e] = ((k & 3u) < S) ? v.cnt[(size_t)s_src[e] * kCntPerComm + k * 9 + s_slot[e]] : 0u;
     }
-    if (tid < 8) {
-        const uint32_t dir = tid >> 2, s = tid & 3u;
+    if (lane < 8) {
+        const uint32_t dir = lane >> 2, s = lane & 3u;
         uint32_t n = 0;
         if (s < S) {
-            const unsigned long long fx = v.lam[((size_t)(t & 1u) * v.n_sims + cd.sim) * 8 + tid];
+            const unsigned long long fx = v.lam[((size_t)(t & 1u) * v.n_sims + cd.sim) * 8 + lane];
             const double mu = (double)fx * 0x1p-24 / (double)sd.C;
             const Stream st{key, cd.c, t, ctr_tag(P_NLD, dir, s, 0)};
             n = poisson(mu, st);
         }
-        s_nld[tid] = n;
+        s_nld[lane] = n;
     }
-    if (tid < 2) s_rot[tid] = mulhi(draw(key, cd.c, t, ctr_tag(P_ROT, tid, 0, 0), 0).x, S);
-    if (cd.c == 0 && tid < 8) v.lam[((size_t)((t + 1) & 1u) * v.n_sims + cd.sim) * 8 + tid] = 0ull;
+    if (lane < 2) s_rot[lane] = mulhi(draw(key, cd.c, t, ctr_tag(P_ROT, lane, 0, 0), 0).x, S);
+    if (cd.c == 0 && lane < 8) v.lam[((size_t)((t + 1) & 1u) * v.n_sims + cd.sim) * 8 + lane] = 0ull;
     __syncthreads();
-    if (tid < 2) {
-        const uint32_t dir = tid;
-        uint32_t acc = 0, idx = 0;
-        for (uint32_t pos = 0; pos < S; ++pos) {
+    // segment starts: for each direction, in rotation order, the in-edges then
+    // the long-distance arrivals (one lane per segment, then a wave scan)
+#pragma unroll
+    for (uint32_t dir = 0; dir < 2; ++dir) {
+        uint32_t cnt = 0;
+        if (lane < nseg) {
+            const uint32_t pos = lane / (in_len + 1), e = lane % (in_len + 1);
             const uint32_t s = (s_rot[dir] + pos) % S;
-            for (uint32_t e = 0; e <= in_len; ++e) {
-                s_seg[dir][idx++] = acc;
-                acc += (e < in_len) ? s_ecnt[dir][s][e] : s_nld[dir * 4 + s];
-            }
+            cnt = (e < in_len) ? s_ecnt[dir * 4 + s][e] : s_nld[dir * 4 + s];
         }
-        s_seg[dir][idx] = acc;
+        uint32_t total;
+        const uint32_t start = wave_exclusive_scan(cnt, total);
+        if (lane < nseg) s_seg[dir][lane] = start;
+        if (lane == 0) s_seg[dir][nseg] = total;
     }
-    if (tid < in_len) s_src[tid] -= sd.comm_off;   // simulation-local source ids
+    if (lane < in_len) s_src[lane] -= sd.comm_off;   // simulation-local source ids
     __syncthreads();
 
     // ---- onto mosquitoes: uniform over alive slots, success iff susceptible ----
     const uint32_t tot_m = s_seg[DIR_HM][nseg];
     if (tot_m > 0) {
-        uint32_t aw[kSlotsPerThread / 2];
-        uint32_t nalive = 0;
-        const uint32_t j0 = kSlotsPerThread * tid;
-        {
-            const uint4* m4p = reinterpret_cast<const uint4*>(v.m_word + cd.m_off + j0);
+        const uint32_t nwords = (cd.cap + 31) >> 5;
+        uint32_t wv[kWordsPerLane], c = 0;
 #pragma unroll
-            for (uint32_t b = 0; b < kSlotsPerThread / 8; ++b) {
-                uint4 raw = make_uint4(0x00030003u, 0x00030003u, 0x00030003u, 0x00030003u);
-                if (j0 + 8 * b < cd.cap) raw = m4p[b];
-                aw[4 * b + 0] = raw.x; aw[4 * b + 1] = raw.y; aw[4 * b + 2] = raw.z; aw[4 * b + 3] = raw.w;
-            }
-#pragma unroll
-            for (uint32_t e = 0; e < kSlotsPerThread; ++e) {
-                const uint32_t w = (aw[e >> 1] >> ((e & 1u) * 16u)) & 0xFFFFu;
-                if (j0 + e < cd.cap && (w & 3u) != 3u) nalive++;
-            }
+        for (uint32_t k = 0; k < kWordsPerLane; ++k) {
+            const uint32_t w = kWordsPerLane * lane + k;
+            wv[k] = w < nwords ? v.alive[(cd.m_off >> 5) + w] : 0u;
+            c += __popc(wv[k]);
         }
         uint32_t Md;
-        uint32_t q = block_exclusive_scan(nalive, s_wave, Md);
+        uint32_t pre = wave_exclusive_scan(c, Md);
 #pragma unroll
-        for (uint32_t e = 0; e < kSlotsPerThread; ++e) {
-            const uint32_t w = (aw[e >> 1] >> ((e & 1u) * 16u)) & 0xFFFFu;
-            if (j0 + e < cd.cap && (w & 3u) != 3u) {
-                s_alive[q] = (uint16_t)((j0 + e) | ((w & 3u) == 0 ? 0x8000u : 0u));
-                s_claim[q] = 0xFFFFFFFFu;
-                q++;
-            }
+        for (uint32_t k = 0; k < kWordsPerLane; ++k) {
+            const uint32_t w = kWordsPerLane * lane + k;
+            if (w < nwords) { s_word[w] = wv[k]; s_wpre[w] = pre; }
+            pre += __popc(wv[k]);
         }
-        __syncthreads();
         if (Md > 0) {
-            for (uint32_t r = tid; r < tot_m; r += kBlock) {
-                const EventRef ev = decode_event(r, s_seg[DIR_HM], nseg, in_len, s_rot[DIR_HM], S, DIR_HM, s_src, s_slot, cd.c);
-                const uint32_t qq = mulhi(draw(key, ev.x0, t, ev.x2, ev.k).x, Md);
-                if (s_alive[qq] & 0x8000u) atomicMin(&s_claim[qq], r);
-            }
-            __syncthreads();
-            for (uint32_t r = tid; r < tot_m; r += kBlock) {
-                const EventRef ev = decode_event(r, s_seg[DIR_HM], nseg, in_len, s_rot[DIR_HM], S, DIR_HM, s_src, s_slot, cd.c);
-                const uint32_t qq = mulhi(draw(key, ev.x0, t, ev.x2, ev.k).x, Md);
-                if (s_claim[qq] == r) {
-                    const uint32_t jj = s_alive[qq] & 0x7FFFu;
-                    const uint64_t gm = cd.m_off + jj;
+            for (uint32_t cb = 0; cb < tot_m; cb += kChunk) {
+                const uint32_t ce = tot_m - cb < kChunk ? tot_m : cb + kChunk;
+                for (uint32_t i = lane; i < kHash; i += kWave) s_tab[i] = kEmpty;
+                __syncthreads();
+                uint32_t cache[kPerLane];
+#pragma unroll
+                for (uint32_t u = 0; u < kPerLane; ++u) {
+                    const uint32_t r = cb + u * kWave + lane;
+                    cache[u] = 0;
+                    if (r < ce) {
+                        const EventRef ev = decode_event(r, s_seg[DIR_HM], nseg, in_len, s_rot[DIR_HM], S, DIR_HM,
+                                                         s_src, s_slot, cd.c);
+                        const uint32_t q = mulhi(draw(key, ev.x0, t, ev.x2, ev.k).x, Md);
+                        const uint32_t slot = select_alive(q, s_word, s_wpre, nwords);
+                        if ((v.m_word[cd.m_off + slot] & 3u) == 0) {
+                            claim(s_tab, slot, r - cb);
+                            cache[u] = kEligible | (ev.s << kKeyBits) | slot;
+                        }
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (uint32_t u = 0; u < kPerLane; ++u) {
+                    if (!(cache[u] & kEligible)) continue;
+                    const uint32_t r = cb + u * kWave + lane;
+                    const uint32_t slot = cache[u] & kKeyMask;
+                    if (claimed_rank(s_tab, slot) != r - cb) continue;
+                    const uint32_t s = (cache[u] >> kKeyBits) & 3u;
+                    const uint64_t gm = cd.m_off + slot;
                     const uint32_t w = v.m_word[gm];
-                    v.m_word[gm] = (uint16_t)((w & ~0xFu) | (ev.s << 2) | 1u);
-                    const U4 r2 = draw(key, cd.m_base + jj, t, P_M_EIP, 0);
+                    v.m_word[gm] = (uint16_t)((w & ~0xFu) | (s << 2) | 1u);
+                    const U4 r2 = draw(key, cd.m_base + slot, t, P_M_EIP, 0);
                     v.m_next[gm] = (uint16_t)(t + stoch_round(sd.eip, r2.x));
                 }
+                __syncthreads();
             }
         }
-        __syncthreads();
     }
 
     // ---- onto humans: uniform target, success iff S and never infected by s ----
@@ -558,31 +686,43 @@ __global__ __launch_bounds__(kBlock) void k_day_events(EngineView v) {
             const uint32_t s = (s_rot[DIR_MH] + pos) % S;
             const uint32_t rb = s_seg[DIR_MH][pos * (in_len + 1)];
             const uint32_t re = s_seg[DIR_MH][(pos + 1) * (in_len + 1)];
-            if (re > rb) {
-                for (uint32_t h = tid; h < N; h += kBlock) s_claim[h] = 0xFFFFFFFFu;
+            for (uint32_t cb = rb; cb < re; cb += kChunk) {
+                const uint32_t ce = re - cb < kChunk ? re : cb + kChunk;
+                for (uint32_t i = lane; i < kHash; i += kWave) s_tab[i] = kEmpty;
                 __syncthreads();
-                for (uint32_t r = rb + tid; r < re; r += kBlock) {
-                    const EventRef ev = decode_event(r, s_seg[DIR_MH], nseg, in_len, s_rot[DIR_MH], S, DIR_MH, s_src, s_slot, cd.c);
-                    const uint32_t h = mulhi(draw(key, ev.x0, t, ev.x2, ev.k).x, N);
-                    const uint32_t w = v.h_word[cd.h_off + h];
-                    if ((w & 3u) == 0 && !((w >> (4 + s)) & 1u)) atomicMin(&s_claim[h], r);
+                uint32_t cache[kPerLane];
+#pragma unroll
+                for (uint32_t u = 0; u < kPerLane; ++u) {
+                    const uint32_t r = cb + u * kWave + lane;
+                    cache[u] = 0;
+                    if (r < ce) {
+                        const EventRef ev = decode_event(r, s_seg[DIR_MH], nseg, in_len, s_rot[DIR_MH], S, DIR_MH,
+                                                         s_src, s_slot, cd.c);
+                        const uint32_t h = mulhi(draw(key, ev.x0, t, ev.x2, ev.k).x, N);
+                        const uint32_t w = v.h_word[cd.h_off + h];
+                        if ((w & 3u) == 0 && !((w >> (4 + s)) & 1u)) {
+                            claim(s_tab, h, r - cb);
+                            cache[u] = kEligible | h;
+                        }
+                    }
                 }
                 __syncthreads();
-                for (uint32_t r = rb + tid; r < re; r += kBlock) {
-                    const EventRef ev = decode_event(r, s_seg[DIR_MH], nseg, in_len, s_rot[DIR_MH], S, DIR_MH, s_src, s_slot, cd.c);
-                    const uint32_t h = mulhi(draw(key, ev.x0, t, ev.x2, ev.k).x, N);
-                    if (s_claim[h] == r) {
-                        const uint64_t gi = cd.h_off + h;
-                        const uint32_t w = v.h_word[gi];
-                        v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
-                        const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
-                        v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
-                    }
+#pragma unroll
+                for (uint32_t u = 0; u < kPerLane; ++u) {
+                    if (!(cache[u] & kEligible)) continue;
+                    const uint32_t r = cb + u * kWave + lane;
+                    const uint32_t h = cache[u] & kKeyMask;
+                    if (claimed_rank(s_tab, h) != r - cb) continue;
+                    const uint64_t gi = cd.h_off + h;
+                    const uint32_t w = v.h_word[gi];
+                    v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
+                    const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
+                    v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
                 }
                 __syncthreads();
             }
             if (sd.intro > 0.0f) {  // external introductions (nepidemic.cu:261-309)
-                if (tid == 0) {
+                if (lane == 0) {
                     const uint32_t tagi = ctr_tag(P_INTRO, 0, s, 0);
                     uint32_t x = stoch_round(sd.intro / (float)sd.C, draw(key, cd.c, t, tagi, 0).x);
                     for (uint32_t att = 0; x > 0 && att < 1000; ++att) {
@@ -614,22 +754,18 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t value) {
 
 // ---------------------------------------------------------------------------
 // Launchers (host).
-size_t events_lds_bytes() {
-    return (size_t)ARBO_MAX_COMMUNITY_SLOTS * 4 + (size_t)ARBO_MAX_COMMUNITY_SLOTS * 2;
-}
-
 hipError_t launch_init(const EngineView& v, hipStream_t st) {
-    hipLaunchKernelGGL(k_init, dim3(v.n_comm), dim3(kBlock), 0, st, v);
+    hipLaunchKernelGGL(k_init, dim3(v.n_comm), dim3(kWave), 0, st, v);
     return hipGetLastError();
 }
 
 hipError_t launch_demography(const EngineView& v, hipStream_t st) {
-    hipLaunchKernelGGL(k_day_demography, dim3(v.n_comm), dim3(kBlock), 0, st, v);
+    hipLaunchKernelGGL(k_day_demography, dim3(v.n_comm), dim3(kWave), (size_t)v.death_m_lds * 4, st, v);
     return hipGetLastError();
 }
 
 hipError_t launch_events(const EngineView& v, hipStream_t st) {
-    hipLaunchKernelGGL(k_day_events, dim3(v.n_comm), dim3(kBlock), events_lds_bytes(), st, v);
+    hipLaunchKernelGGL(k_day_events, dim3(v.n_comm), dim3(kWave), 0, st, v);
     return hipGetLastError();
 }
 

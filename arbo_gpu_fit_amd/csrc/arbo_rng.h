@@ -50,7 +50,7 @@ enum Purpose : uint32_t {
     P_IMMUNE = 4,      // per-(community, serotype) Beta(2,5) immunity fraction
     P_H_BIRTH = 10,    // human rebirth: death age of the newborn
     P_H_REC = 11,      // human E->I: length of the infectious period
-    P_M_BIRTH = 12,    // mosquito emergence into a dead slot: death age
+    P_M_BIRTH = 12,    // mosquito emergence: death ages of births 4i..4i+3 of a community (x3 = i)
     P_M_TARGET = 13,   // community emergence target rounding
     P_NLOC = 20,       // local event count of a source (community, dir, serotype)
     P_DEST = 21,       // alias-table destination of the k-th local event

@@ -445,8 +445,9 @@ void phase_a(Sim& s, uint32_t c, uint32_t t) {
         if (!dead[j]) continue;
         uint32_t m = (uint32_t)(s.m_off[c] + j);
         if (rank < births) {
-            W4 r = rnd(s.key, m, t, M_BIRTH, 0);
-            uint32_t da = death_age(s.Km, (r.v[0] >> 8) + 1);
+            // birth `rank` of the community draws word rank%4 of block rank/4
+            W4 r = rnd(s.key, c, t, M_BIRTH, rank >> 2);
+            uint32_t da = death_age(s.Km, (r.v[rank & 3u] >> 8) + 1);
             s.mw[m] = (uint16_t)(((t + da) & 0xFFFu) << 4);
         } else if (died[j]) {
             s.mw[m] = 3;
