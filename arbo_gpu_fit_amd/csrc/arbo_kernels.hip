@@ -375,7 +375,9 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     uint32_t births;
     {
         const U4 r = draw(key, cd.c, t, P_M_TARGET, 0);
-        const uint32_t target = stoch_round(v.dens[sd.dens_off + t] * cd.base, r.x);
+        // the target of day t is the next day's expected population E[t+1]
+        // (simulation.cpp:129-132)
+        const uint32_t target = stoch_round(v.dens[sd.dens_off + t + 1] * cd.base, r.x);
         const uint32_t b = target > alive_c ? target - alive_c : 0u;
         births = b < dead_c ? b : dead_c;
     }

@@ -436,8 +436,10 @@ void phase_a(Sim& s, uint32_t c, uint32_t t) {
         alive++;
         if (st == 2) Im[(w >> 2) & 3u]++;
     }
+    // the day-t target is the expected population of the next day, E[t+1]
+    // (simulation.cpp:129-132 computes it before the day's demography)
     W4 rt = rnd(s.key, c, t, M_TARGET, 0);
-    uint32_t target = sround(s.dens[t] * s.base[c], rt.v[0]);
+    uint32_t target = sround(s.dens[t + 1] * s.base[c], rt.v[0]);
     uint32_t births = target > alive ? target - alive : 0;
     if (births > ndead) births = ndead;
     uint32_t rank = 0;
@@ -574,17 +576,31 @@ void phase_b(Sim& s, uint32_t d, uint32_t t) {
     }
 }
 
+void sim_day(Sim& s, uint32_t t, int threads) {
+    const int C = (int)s.C;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads) if (threads > 1)
+    for (int c = 0; c < C; ++c) phase_a(s, (uint32_t)c, t);
+    std::fill(s.lam.begin(), s.lam.end(), 0ull);
+    for (uint32_t c = 0; c < s.C; ++c)
+        for (uint32_t k = 0; k < 8; ++k) s.lam[k] += s.ldfx[(size_t)c * 8 + k];
+    record_row(s, t + 1);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads) if (threads > 1)
+    for (int d = 0; d < C; ++d) phase_b(s, (uint32_t)d, t);
+}
+
+int thread_count(const ArboEngine* e) { return e->threads > 0 ? e->threads : omp_get_max_threads(); }
+
+// Simulations are independent: many small ones run one per thread, a few
+// large ones run with their communities spread over the threads.  Both orders
+// give identical results.
 void day_step(ArboEngine* e, uint32_t t) {
-    for (Sim& s : e->sims) {
-        const int C = (int)s.C;
-#pragma omp parallel for schedule(dynamic, 16) num_threads(e->threads > 0 ? e->threads : omp_get_max_threads())
-        for (int c = 0; c < C; ++c) phase_a(s, (uint32_t)c, t);
-        std::fill(s.lam.begin(), s.lam.end(), 0ull);
-        for (uint32_t c = 0; c < s.C; ++c)
-            for (uint32_t k = 0; k < 8; ++k) s.lam[k] += s.ldfx[(size_t)c * 8 + k];
-        record_row(s, t + 1);
-#pragma omp parallel for schedule(dynamic, 16) num_threads(e->threads > 0 ? e->threads : omp_get_max_threads())
-        for (int d = 0; d < C; ++d) phase_b(s, (uint32_t)d, t);
+    const int nt = thread_count(e);
+    const int n = (int)e->sims.size();
+    if (n >= 2 * nt) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+        for (int i = 0; i < n; ++i) sim_day(e->sims[i], t, 1);
+    } else {
+        for (Sim& s : e->sims) sim_day(s, t, nt);
     }
 }
 
@@ -636,13 +652,24 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes,
         }
         if (nh >= (1ull << 32) || nm >= (1ull << 32)) return fail(e, ARBO_E_INVALID, "simulation too large");
         s.NH = nh; s.NM = nm;
-        init_sim(s, max_days);
     }
-    for (Sim& s : e->sims) {
-        const int C = (int)s.C;
-#pragma omp parallel for schedule(dynamic, 16) num_threads(e->threads > 0 ? e->threads : omp_get_max_threads())
-        for (int c = 0; c < C; ++c) init_community(s, (uint32_t)c);
-        record_row(s, 0);
+    const int nt = thread_count(e);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+    for (int i = 0; i < (int)n_sims; ++i) init_sim(e->sims[i], max_days);
+    if ((int)n_sims >= 2 * nt) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+        for (int i = 0; i < (int)n_sims; ++i) {
+            Sim& s = e->sims[i];
+            for (uint32_t c = 0; c < s.C; ++c) init_community(s, c);
+            record_row(s, 0);
+        }
+    } else {
+        for (Sim& s : e->sims) {
+            const int C = (int)s.C;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+            for (int c = 0; c < C; ++c) init_community(s, (uint32_t)c);
+            record_row(s, 0);
+        }
     }
     e->day = 0;
     e->max_days = max_days;

@@ -1,0 +1,79 @@
+"""The C ABI boundary (CPU): libarbo_hip.so loads and exports every entry point
+include/arbo_c.h declares (no compute is run), the oracle exports the same
+ABI, and the ctypes mirror matches the C struct layout (sizes and offsets
+printed by a program compiled from the header)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from arbo_gpu_fit_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "arbo_c.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"ARBO_API\s+[\w\s\*]+?\b(arbo_\w+)\s*\(", text)))
+
+
+def test_header_declares_the_abi():
+    names = declared_functions()
+    assert set(names) == set(abi.PROTOTYPES), (set(names) ^ set(abi.PROTOTYPES))
+
+
+def test_hip_library_exports_every_symbol():
+    if not os.path.exists(abi.HIP_LIB_PATH):
+        pytest.skip("libarbo_hip.so not built")
+    lib = C.CDLL(abi.HIP_LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    abi.bind(lib)
+    out = subprocess.run(["nm", "-D", "--defined-only", abi.HIP_LIB_PATH], capture_output=True, text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    assert {n for n in exported if n.startswith("arbo_")} == set(declared_functions())
+
+
+def test_oracle_exports_the_same_abi(oracle_lib):
+    for name in declared_functions():
+        assert hasattr(oracle_lib, name), name
+
+
+def test_create_rejects_bad_descriptor(oracle_lib):
+    h = abi.EnginePtr()
+    desc = abi.EngineDesc(abi.ABI_VERSION + 7, 0, 0, 0)
+    assert oracle_lib.arbo_create(C.byref(desc), C.byref(h)) == abi.E_INVALID
+    assert oracle_lib.arbo_create(None, C.byref(h)) == abi.E_INVALID
+
+
+LAYOUT_PROGRAM = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "arbo_c.h"
+#define F(T, f) printf("%s.%s %zu\n", #T, #f, offsetof(T, f));
+int main(void) {
+    printf("ArboEngineDesc %zu\nArboParams %zu\nArboShape %zu\nArboSeries %zu\nArboState %zu\n",
+           sizeof(ArboEngineDesc), sizeof(ArboParams), sizeof(ArboShape), sizeof(ArboSeries), sizeof(ArboState));
+    FIELDS
+    return 0;
+}
+"""
+
+
+def test_ctypes_layout_matches_header(tmp_path):
+    structs = {"ArboEngineDesc": abi.EngineDesc, "ArboParams": abi.Params, "ArboShape": abi.Shape,
+               "ArboSeries": abi.Series, "ArboState": abi.State}
+    fields = "".join(f"F({cname}, {f})" for cname, cls in structs.items() for f, _ in cls._fields_)
+    src = tmp_path / "layout.c"
+    src.write_text(LAYOUT_PROGRAM.replace("FIELDS", fields))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(ln.rsplit(" ", 1) for ln in lines if ln)
+    for cname, cls in structs.items():
+        assert int(got[cname]) == C.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)

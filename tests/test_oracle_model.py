@@ -1,0 +1,279 @@
+"""Pins the oracle's day loop (CPU): census invariants and recounts every day,
+closed-form special cases, serial == OpenMP, checkpoint/resume, batch
+invariance, error codes, and the claim-min == serial-application theorem the
+GPU event kernel relies on (brute force over small random instances)."""
+import math
+
+import numpy as np
+import pytest
+from hypothesis import given, settings
+from hypothesis import strategies as st
+
+from arbo_gpu_fit_amd import abi, configs
+from arbo_gpu_fit_amd.engine import ArboError, Engine, SimSpec, weekly_incidence
+
+
+def small_spec(**over):
+    p = configs.base_params(seed_frac_h=0.01, seed_frac_m=0.01, **over)
+    return SimSpec(5, 5, np.full(25, 400, np.uint32), p, 21)
+
+
+def human_fields(w):
+    return w & 3, (w >> 2) & 3, (w >> 4) & 15, w >> 8
+
+
+def i16(x):
+    return ((x.astype(np.int64) + 32768) % 65536) - 32768
+
+
+def test_invariants_and_recounts(oracle_lib):
+    spec = small_spec(beta_hm=0.5, beta_mh=0.5)
+    T = 90
+    eng = Engine(oracle_lib, threads=4)
+    eng.init([spec], T)
+    N = int(spec.humans.sum())
+    S = spec.params.n_serotypes
+    comm_of = np.repeat(np.arange(25), 400)
+    total_inc = 0
+    for t in range(T):
+        before = eng.state(0)
+        eng.step(1)
+        after = eng.state(0)
+        ser = eng.series(0)
+        st_b, sn_b, _, dd_b = human_fields(before["h_word"])
+        # incidence of day t = E humans that survive day t and are past their incubation
+        due = (st_b == 1) & (dd_b > t) & (i16(np.uint64(t) - before["h_next"]) > 0)
+        exp_inc = np.bincount(sn_b[due], minlength=4)[:S]
+        assert np.array_equal(ser["n_incidence"][t + 1], exp_inc), t
+        total_inc += exp_inc.sum()
+        st_a, sn_a, imm_a, dd_a = human_fields(after["h_word"])
+        isI = st_a == 2
+        assert np.array_equal(ser["n_infected"][t + 1], np.bincount(sn_a[isI], minlength=4)[:S])
+        assert np.array_equal(ser["n_one_subpop"][t + 1],
+                              np.bincount(sn_a[isI & (comm_of == 0)], minlength=4)[:S])
+        assert ser["n_count"][t + 1] == N
+        mst = after["m_word"] & 3
+        assert ser["m_count"][t + 1] == int((mst != 3).sum())
+        # infected humans are immune to their strain; nobody is overdue
+        inf = st_a != 0
+        assert np.all((imm_a[inf] >> sn_a[inf]) & 1)
+        assert np.all(dd_a > t)
+        assert np.all(i16(np.uint64(t) - after["h_next"][inf]) <= 0)
+        alive = mst != 3
+        dd12 = (after["m_word"] >> 4).astype(np.int64)
+        assert np.all(((t - dd12) % 4096)[alive] >= 2048)
+        me = mst == 1
+        assert np.all(i16(np.uint64(t) - after["m_next"][me]) <= 0)
+    assert total_inc > 100
+    eng.close()
+
+
+def test_no_transmission_without_biting(oracle_lib):
+    spec = small_spec(beta_hm=0.0, beta_mh=0.0)
+    eng = Engine(oracle_lib)
+    eng.init([spec], 40)
+    eng.step(40)
+    s = eng.series(0)
+    assert s["n_incidence"].sum() == 0
+    assert s["n_infected"][0].sum() > 0
+    assert s["n_infected"][6:].sum() == 0        # seeded infections last at most 5 days
+    st = eng.state(0)
+    assert np.all((st["h_word"] & 3) == 0)
+    eng.close()
+
+
+def test_mosquito_daily_death_probability(oracle_lib):
+    """Exponential lifetimes with mean 10 d: each alive mosquito dies on a given
+    day with probability 1 - exp(-1/10), whatever its age."""
+    p = configs.base_params(seed_frac_h=0.0, seed_frac_m=0.0, season_amp=0.0)
+    spec = SimSpec(6, 6, np.full(36, 1000, np.uint32), p, 31)
+    eng = Engine(oracle_lib, threads=4)
+    eng.init([spec], 100)
+    eng.step(60)
+    for t in (60, 75, 99):
+        eng.step(t - eng.day) if t > eng.day else None
+        mw = eng.state(0)["m_word"]
+        alive = (mw & 3) != 3
+        due = ((mw >> 4) == (t & 0xFFF)) & alive
+        frac = due.sum() / alive.sum()
+        q = 1.0 - math.exp(-0.1)
+        assert abs(frac - q) < 5 * math.sqrt(q * (1 - q) / alive.sum())
+    eng.close()
+
+
+def test_human_renewal_rate(oracle_lib):
+    """A stationary human population renews at 1/mean lifespan per year."""
+    p = configs.base_params(seed_frac_h=0.0, seed_frac_m=0.0)
+    spec = SimSpec(10, 10, np.full(100, 2000, np.uint32), p, 41)
+    eng = Engine(oracle_lib, threads=8)
+    eng.init([spec], 365)
+    eng.step(365)
+    births = int((eng.state(0)["h_birth"] >= 0).sum())
+    expect = 2e5 / 70.0
+    assert abs(births - expect) < 5 * math.sqrt(expect)
+    eng.close()
+
+
+def test_seasonal_mosquito_target(oracle_lib):
+    """Alive mosquitoes at the start of day t track E[t] = 2 N (1 + A sin(2 pi t/365))
+    while the target exceeds the survivors: emergence on day t-1 fills dead
+    slots up to E[t] (simulation.cpp:129-132 computes E[t+1] before day t)."""
+    p = configs.base_params(seed_frac_h=0.0, seed_frac_m=0.0, season_amp=0.5)
+    spec = SimSpec(4, 4, np.full(16, 1000, np.uint32), p, 51)
+    eng = Engine(oracle_lib)
+    eng.init([spec], 365)
+    eng.step(365)
+    m = eng.series(0)["m_count"].astype(float)
+    t = np.arange(366)
+    target = 2 * 16000 * (1 + 0.5 * np.sin(2 * np.pi * t / 365))
+    rising = np.cos(2 * np.pi * t / 365) > 0.1
+    assert np.all(np.abs(m[rising] - target[rising]) <= 16)     # one rounding per community
+    assert np.all(m <= 16 * math.ceil(2 * 1.5 * 1000))
+    eng.close()
+
+
+def test_serial_equals_openmp(oracle_lib):
+    spec = small_spec(beta_hm=0.6, beta_mh=0.6)
+    res = []
+    for threads in (1, 8):
+        eng = Engine(oracle_lib, threads=threads)
+        eng.init([spec], 50)
+        eng.step(50)
+        res.append((eng.state(0), eng.series(0)))
+        eng.close()
+    for k in res[0][0]:
+        assert np.array_equal(res[0][0][k], res[1][0][k])
+    for k in res[0][1]:
+        assert np.array_equal(res[0][1][k], res[1][1][k])
+
+
+def test_resume_and_batch_invariance(oracle_lib):
+    a = small_spec(beta_hm=0.6, beta_mh=0.6)
+    b = SimSpec(3, 4, np.arange(1, 13, dtype=np.uint32) * 50,
+                configs.base_params(n_serotypes=2, seed_frac_h=0.05, seed_frac_m=0.05), 99)
+    T = 40
+    full = Engine(oracle_lib)
+    full.init([a, b], T)
+    full.step(T // 2)
+    mid = [full.state(i) for i in range(2)]
+    full.step(T - T // 2)
+    res = Engine(oracle_lib)
+    res.init([a, b], T)
+    res.set_day(T // 2)
+    for i in range(2):
+        res.write_state(i, mid[i])
+    res.step(T - T // 2)
+    solo = Engine(oracle_lib)
+    solo.init([b], T)
+    solo.step(T)
+    for k, v in full.state(1).items():
+        assert np.array_equal(v, res.state(1)[k])
+        assert np.array_equal(v, solo.state(0)[k])
+    for k, v in full.series(1).items():
+        assert np.array_equal(v[T // 2 + 1:], res.series(1)[k][T // 2 + 1:])
+        assert np.array_equal(v, solo.series(0)[k])
+    for e in (full, res, solo):
+        e.close()
+
+
+@pytest.mark.parametrize("field,value,code", [
+    ("n_serotypes", 5, abi.E_INVALID), ("n_serotypes", 0, abi.E_INVALID), ("neighbourhood", 6, abi.E_INVALID),
+    ("beta_hm", -1.0, abi.E_INVALID), ("beta_mh", float("nan"), abi.E_INVALID),
+    ("long_distance", 1.5, abi.E_INVALID), ("season_amp", 2.0, abi.E_INVALID), ("iip_max", 1, abi.E_INVALID),
+    ("m_scale", 0.0, abi.E_INVALID), ("seed_frac_h", -0.1, abi.E_INVALID), ("immunity_mode", 3, abi.E_INVALID),
+])
+def test_invalid_parameters(oracle_lib, field, value, code):
+    spec = small_spec()
+    setattr(spec.params, field, value)
+    eng = Engine(oracle_lib)
+    with pytest.raises(ArboError) as ei:
+        eng.init([spec], 10)
+    assert ei.value.code == code and str(ei.value)
+    eng.close()
+
+
+def test_invalid_shapes_and_order(oracle_lib):
+    eng = Engine(oracle_lib)
+    with pytest.raises(ArboError) as ei:
+        eng.step(1)
+    assert ei.value.code == abi.E_STATE
+    for humans in (np.array([0] * 4, np.uint32), np.array([9000] * 4, np.uint32)):
+        with pytest.raises(ArboError):
+            eng.init([SimSpec(2, 2, humans, configs.base_params(), 1)], 10)
+    with pytest.raises(ArboError):
+        eng.init([SimSpec(2, 2, np.full(4, 5, np.uint32), configs.base_params(), 1,
+                          mosq_slots=np.full(4, 9000, np.uint32))], 10)
+    with pytest.raises(ArboError):
+        eng.init([small_spec()], 0)
+    eng.init([small_spec()], 5)
+    eng.step(5)
+    with pytest.raises(ArboError) as ei:
+        eng.step(1)
+    assert ei.value.code == abi.E_CAPACITY
+    with pytest.raises(ArboError):
+        eng.series(3)
+    eng.close()
+
+
+def test_weekly_binning_matches_reference_rule():
+    """MCMC::storeIncidence (MCMC.cpp:199-209) for one serotype:
+    week w = h[7w] + sum_{t=1..6, 7w+t <= maxTime} h[7w+t]."""
+    rng = np.random.default_rng(3)
+    h = rng.integers(0, 50, (30, 1)).astype(np.uint32)
+    max_time = 29
+    max_week = 5
+    ref = [int(h[7 * w, 0]) + sum(int(h[7 * w + t, 0]) for t in range(1, 7) if 7 * w + t <= max_time)
+           for w in range(max_week)]
+    assert list(weekly_incidence(h, max_week)) == ref
+
+
+# ---------------------------------------------------------------------------
+# Claim-min: the event kernel applies a destination's events in parallel.  Each
+# eligible event records its rank on its target, the minimum wins.  With
+# eligibility evaluated at the start of a chunk and chunks processed in order,
+# this equals applying the events one by one (the reference's serial loops).
+
+def serial_apply(state, events, eligible):
+    state = list(state)
+    won = []
+    for r, (tgt, s) in enumerate(events):
+        if eligible(state[tgt], s):
+            state[tgt] = ("E", s)
+            won.append(r)
+    return state, won
+
+
+def claim_apply(state, events, eligible, chunk):
+    state = list(state)
+    won = []
+    for cb in range(0, len(events), chunk):
+        claims = {}
+        for r in range(cb, min(len(events), cb + chunk)):
+            tgt, s = events[r]
+            if eligible(state[tgt], s):
+                claims[tgt] = min(claims.get(tgt, r), r)
+        for tgt, r in claims.items():
+            state[tgt] = ("E", events[r][1])
+            won.append(r)
+    return state, sorted(won)
+
+
+def human_eligible(x, s):
+    return x[0] == "S" and not (x[1] >> s) & 1
+
+
+def mosquito_eligible(x, s):
+    return x[0] == "S"
+
+
+@settings(max_examples=300, deadline=None)
+@given(st.integers(1, 6), st.lists(st.tuples(st.integers(0, 5), st.integers(0, 3)), max_size=30),
+       st.lists(st.tuples(st.sampled_from(["S", "E", "I"]), st.integers(0, 15)), min_size=6, max_size=6),
+       st.integers(1, 7))
+def test_claim_min_equals_serial(n, events, init, chunk):
+    events = [(t % n, s) for t, s in events]
+    state = init[:n]
+    for elig in (human_eligible, mosquito_eligible):
+        a, wa = serial_apply(state, events, elig)
+        b, wb = claim_apply(state, events, elig, chunk)
+        assert a == b and wa == wb
