@@ -50,7 +50,7 @@ struct SimDesc {
     uint32_t immune_off;          // [C*S] thresholds
     uint32_t seed_h, seed_m;
     uint32_t death_m_max;         // largest death age the mosquito table can return
-    uint32_t pad;
+    uint32_t death_m_guide_off;   // [256] smallest death age of each 2^16-wide PLE bucket
     uint64_t series_off;          // this simulation's block in the series array
 };
 
@@ -85,6 +85,7 @@ struct EngineView {
     uint16_t* m_next;
     uint32_t* alive;               // [M/32] alive-slot bitmap of the current day
     uint32_t* cnt;                 // [G][72] event counts of the current day
+    uint32_t* nld;                 // [G][8] long-distance arrivals of the current day
     unsigned long long* lam;       // [2 parity][n_sims][8]  long-distance rate, 2^-24 fixed point
     uint32_t* series;
     const uint32_t* csr_to;        // simulation-local destination

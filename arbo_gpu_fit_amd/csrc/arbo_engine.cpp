@@ -23,6 +23,7 @@ namespace arbo {
 hipError_t launch_init(const EngineView& v, hipStream_t st);
 hipError_t launch_demography(const EngineView& v, hipStream_t st);
 hipError_t launch_events(const EngineView& v, hipStream_t st);
+hipError_t launch_longdist(const EngineView& v, hipStream_t st);
 hipError_t launch_advance(uint32_t* day, hipStream_t st);
 hipError_t launch_fill(uint32_t* p, uint64_t n, uint32_t value, hipStream_t st);
 }  // namespace arbo
@@ -62,7 +63,7 @@ struct ArboEngine {
     // device memory
     DevBuf<CommDesc> comm;
     DevBuf<SimDesc> simd;
-    DevBuf<uint32_t> h_word, alive, cnt, series, csr_thr, in_edge, in_slot, tables, day_ctr;
+    DevBuf<uint32_t> h_word, alive, cnt, nld, series, csr_thr, in_edge, in_slot, tables, day_ctr;
     DevBuf<uint16_t> h_next, m_word, m_next;
     DevBuf<int32_t> h_birth;
     DevBuf<unsigned long long> lam;
@@ -115,7 +116,7 @@ void free_batch(ArboEngine* e) {
     if (e->graph_exec) { (void)hipGraphExecDestroy(e->graph_exec); e->graph_exec = nullptr; }
     if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
     dev_free(e->comm); dev_free(e->simd);
-    dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->series); dev_free(e->csr_thr);
+    dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->series); dev_free(e->csr_thr);
     dev_free(e->in_edge); dev_free(e->in_slot); dev_free(e->tables); dev_free(e->day_ctr);
     dev_free(e->h_next); dev_free(e->m_word); dev_free(e->m_next); dev_free(e->h_birth);
     dev_free(e->lam); dev_free(e->dens);
@@ -167,6 +168,7 @@ int drain_timing(ArboEngine* e) {
 
 int launch_day(ArboEngine* e) {
     HIP_TRY(e, launch_demography(e->view, e->stream));
+    HIP_TRY(e, launch_longdist(e->view, e->stream));
     HIP_TRY(e, launch_events(e->view, e->stream));
     HIP_TRY(e, launch_advance(e->day_ctr.p, e->stream));
     return ARBO_OK;
@@ -178,6 +180,7 @@ int launch_day_timed(ArboEngine* e) {
     if (rc) return rc;
     HIP_TRY(e, launch_demography(e->view, e->stream));
     if ((rc = record_timing_event(e, &b))) return rc;
+    HIP_TRY(e, launch_longdist(e->view, e->stream));
     HIP_TRY(e, launch_events(e->view, e->stream));
     if ((rc = record_timing_event(e, &c))) return rc;
     HIP_TRY(e, launch_advance(e->day_ctr.p, e->stream));
@@ -318,11 +321,9 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         sd.dens_off = pool_add(dens_pool, seen_f32, t.dens);
         sd.seed_h = t.seed_h;
         sd.seed_m = t.seed_m;
-        uint32_t dmax = kMosqAgeTable - 1;
-        for (uint32_t a = 1; a < kMosqAgeTable; ++a)
-            if (t.death_m[a] >= kOne24) { dmax = a; break; }
-        sd.death_m_max = dmax;
-        death_m_lds = std::max(death_m_lds, dmax + 1);
+        sd.death_m_max = t.death_m_max;
+        sd.death_m_guide_off = pool_add(table_pool, seen_u32, t.death_m_guide);
+        death_m_lds = std::max(death_m_lds, t.death_m_max + 1);
         sd.series_off = sh.series_off;
 
         const uint32_t row_base = (uint32_t)csr_thr.size();
@@ -372,7 +373,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     if ((rc = dev_alloc(e, e->comm, comm.size())) || (rc = dev_alloc(e, e->simd, n_sims)) ||
         (rc = dev_alloc(e, e->h_word, h_tot)) || (rc = dev_alloc(e, e->h_next, h_tot)) ||
         (rc = dev_alloc(e, e->h_birth, h_tot)) || (rc = dev_alloc(e, e->m_word, m_tot)) ||
-        (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kCntPerComm)) ||
+        (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kCntPerComm)) || (rc = dev_alloc(e, e->nld, comm.size() * 8)) ||
         (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->series, ser_tot)) ||
         (rc = dev_alloc(e, e->csr_thr, csr_thr.size())) || (rc = dev_alloc(e, e->in_edge, in_edge.size())) ||
         (rc = dev_alloc(e, e->in_slot, in_slot.size())) || (rc = dev_alloc(e, e->tables, table_pool.size())) ||
@@ -390,6 +391,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     HIP_TRY(e, launch_fill(e->series.p, ser_tot, 0u, e->stream));
     HIP_TRY(e, launch_fill(reinterpret_cast<uint32_t*>(e->lam.p), (uint64_t)2 * n_sims * 8 * 2, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->day_ctr.p, 1, 0u, e->stream));
+    HIP_TRY(e, launch_fill(e->nld.p, comm.size() * 8, 0u, e->stream));
     // padding words: humans beyond a community are never read as members, but
     // keep the whole array defined
     HIP_TRY(e, launch_fill(e->h_word.p, h_tot, 0u, e->stream));
@@ -405,6 +407,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.m_next = e->m_next.p;
     v.alive = e->alive.p;
     v.cnt = e->cnt.p;
+    v.nld = e->nld.p;
     v.lam = e->lam.p;
     v.series = e->series.p;
     v.csr_to = nullptr;

@@ -49,6 +49,8 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kKeyBits = 14;                              // member index < 2^14
 constexpr uint32_t kKeyMask = (1u << kKeyBits) - 1u;
 constexpr uint32_t kHumanTable = 65536;
+constexpr uint32_t kMaxRowLen = 9;                             // Moore-8 + self
+constexpr uint32_t kGuide = 256;                               // death-age guide buckets (2^16 PLE values each)
 static_assert(kWordsPerLane == 4, "four bitmap words per lane");
 static_assert(ARBO_MAX_COMMUNITY_SLOTS <= (1u << kKeyBits) && ARBO_MAX_COMMUNITY_HUMANS <= (1u << kKeyBits),
               "member index must fit the claim key");
@@ -82,6 +84,15 @@ __device__ __forceinline__ uint32_t death_age(P K, uint32_t hi, uint32_t k) {
         if (K[mid] >= k) hi = mid; else lo = mid + 1;
     }
     return lo;
+}
+
+// The same minimum, started from the smallest death age of k's 2^16-wide
+// bucket: guide[b] = min{a >= 1 : K[a] >= b 2^16 + 1} <= result.
+__device__ __forceinline__ uint32_t death_age_guided(const uint32_t* K, const uint32_t* guide, uint32_t hi,
+                                                     uint32_t k) {
+    uint32_t a = guide[(k - 1) >> 16];
+    while (a < hi && K[a] < k) ++a;
+    return a;
 }
 
 // Death age given survival to age a0: PLE uniform on (F(a0-1), 1].
@@ -118,9 +129,6 @@ __device__ __forceinline__ uint32_t valid_bits(uint32_t w, uint32_t cap) {
     if (first >= cap) return 0u;
     return (1u << (cap - first)) - 1u;
 }
-
-// Bits 0 and 16 of m as a 2-bit value (one bit per 16-bit half).
-__device__ __forceinline__ uint32_t pack_halves(uint32_t m) { return (m & 1u) | ((m >> 15) & 2u); }
 
 // Position of the k-th (0-based) set bit of x; k < popcount(x).
 __device__ __forceinline__ uint32_t select_in_word(uint32_t x, uint32_t k) {
@@ -230,31 +238,51 @@ __global__ __launch_bounds__(kWave) void k_init(EngineView v) {
 
 struct HumanCount { uint32_t ih01, ih23, inc01, inc23; };
 
-// One human of ndemographic.cu:57-103 on the packed word w.
-__device__ __forceinline__ void human_day(const EngineView& v, const CommDesc& cd, const SimDesc& sd, Key key,
-                                          uint32_t t, uint32_t i, uint32_t w, HumanCount& hc) {
-    const uint32_t st = w & 3u;
-    const uint64_t gi = cd.h_off + i;
-    if (t >= (w >> 8)) {  // death and instant rebirth
-        const U4 r = draw(key, cd.h_base + i, t, P_H_BIRTH, 0);
-        const uint32_t da = death_age(v.tables + sd.death_h_off, kHumanTable - 1, (r.x >> 8) + 1);
-        v.h_word[gi] = (t + da) << 8;
-        v.h_birth[gi] = (int32_t)t;
-        return;
+// Four humans of ndemographic.cu:57-103 (one 16-byte vector of packed words,
+// index 4q..4q+3).  The next-event days of the vector are read with one
+// 8-byte load, only when one of its members is infected and survives.
+__device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& cd, const SimDesc& sd, Key key,
+                                          uint32_t t, uint32_t q, uint4 w4, HumanCount& hc) {
+    const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+    uint32_t dying = 0, need = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+        if (4 * q + e >= cd.n) break;
+        if (t >= (ws[e] >> 8)) dying |= 1u << e;
+        else if (ws[e] & 3u) need |= 1u << e;
     }
-    if (st == 0) return;
-    const uint32_t strain = (w >> 2) & 3u;
-    const uint16_t nx = v.h_next[gi];
-    const bool due = (int16_t)(uint16_t)(t - nx) > 0;
-    if (st == 2) {
-        if (due) v.h_word[gi] = w & ~0xFu;   // recovered
-        else add_pair(hc.ih01, hc.ih23, strain);
-    } else if (due) {                        // E -> I
-        const U4 r = draw(key, cd.h_base + i, t, P_H_REC, 0);
-        v.h_next[gi] = (uint16_t)(nx + stoch_round(sd.recovery, r.x));
-        v.h_word[gi] = (w & ~3u) | 2u;
-        add_pair(hc.inc01, hc.inc23, strain);
-        add_pair(hc.ih01, hc.ih23, strain);
+    if (!(dying | need)) return;
+    uint32_t nxs[4] = {0, 0, 0, 0};
+    if (need) {
+        const uint2 nx2 = *reinterpret_cast<const uint2*>(v.h_next + cd.h_off + 4 * q);
+        nxs[0] = nx2.x & 0xFFFFu; nxs[1] = nx2.x >> 16; nxs[2] = nx2.y & 0xFFFFu; nxs[3] = nx2.y >> 16;
+    }
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+        const uint32_t i = 4 * q + e;
+        const uint64_t gi = cd.h_off + i;
+        const uint32_t w = ws[e];
+        if ((dying >> e) & 1u) {  // death and instant rebirth
+            const U4 r = draw(key, cd.h_base + i, t, P_H_BIRTH, 0);
+            const uint32_t da = death_age(v.tables + sd.death_h_off, kHumanTable - 1, (r.x >> 8) + 1);
+            v.h_word[gi] = (t + da) << 8;
+            v.h_birth[gi] = (int32_t)t;
+            continue;
+        }
+        if (!((need >> e) & 1u)) continue;
+        const uint32_t st = w & 3u, strain = (w >> 2) & 3u;
+        const uint32_t nx = nxs[e];
+        const bool due = (int16_t)(uint16_t)(t - nx) > 0;
+        if (st == 2) {
+            if (due) v.h_word[gi] = w & ~0xFu;   // recovered
+            else add_pair(hc.ih01, hc.ih23, strain);
+        } else if (due) {                        // E -> I
+            const U4 r = draw(key, cd.h_base + i, t, P_H_REC, 0);
+            v.h_next[gi] = (uint16_t)(nx + stoch_round(sd.recovery, r.x));
+            v.h_word[gi] = (w & ~3u) | 2u;
+            add_pair(hc.inc01, hc.inc23, strain);
+            add_pair(hc.ih01, hc.ih23, strain);
+        }
     }
 }
 
@@ -262,6 +290,22 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     const uint32_t g = blockIdx.x, lane = threadIdx.x;
     const uint32_t t = *v.day;
     const CommDesc cd = v.comm[g];
+    // the first census batches go out before anything else
+    const uint32_t nvec = (cd.n + 3) >> 2;
+    const uint32_t mvec = (cd.cap + 7) >> 3;
+    const uint4* w4p = reinterpret_cast<const uint4*>(v.h_word + cd.h_off);
+    const uint4* m4p = reinterpret_cast<const uint4*>(v.m_word + cd.m_off);
+    uint4 hb[kUnroll], mb[kUnroll];
+#pragma unroll
+    for (uint32_t u = 0; u < kUnroll; ++u) {
+        const uint32_t q = u * kWave + lane;
+        if (q < nvec) hb[u] = w4p[q];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kUnroll; ++u) {
+        const uint32_t q = u * kWave + lane;
+        if (q < mvec) mb[u] = m4p[q];
+    }
     const SimDesc sd = v.sims[cd.sim];
     const Key key{sd.k0, sd.k1};
     const uint32_t S = sd.S;
@@ -270,95 +314,115 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     __shared__ uint32_t s_dead[kMaxWords];                          // dead-slot bitmap
     __shared__ uint32_t s_died[kMaxWords];                          // died today
     __shared__ uint32_t s_dpre[kMaxWords];                          // dead slots before each word
+    __shared__ uint32_t s_mguide[kGuide];                           // death-age guide of the lifetime table
+    __shared__ uint32_t s_row[kMaxRowLen];                          // alias row of the community
     __shared__ uint32_t s_n[8];
     __shared__ uint32_t s_hist[kCntPerComm];
     uint8_t* s_dead8 = reinterpret_cast<uint8_t*>(s_dead);
     uint8_t* s_died8 = reinterpret_cast<uint8_t*>(s_died);
 
-    for (uint32_t a = lane; a <= sd.death_m_max; a += kWave) s_dtab[a] = v.tables[sd.death_m_off + a];
+    // tables for the births, kept in registers until the census is scanned
+    constexpr uint32_t kStage = kGuide / kWave;                     // 4
+    uint32_t gstage[kStage];
+#pragma unroll
+    for (uint32_t k = 0; k < kStage; ++k) gstage[k] = v.tables[sd.death_m_guide_off + k * kWave + lane];
+    const uint32_t rowent = lane < cd.row_len ? v.csr_thr[cd.row_off + lane] : 0u;
     for (uint32_t i = lane; i < kCntPerComm; i += kWave) s_hist[i] = 0;
+    // emergence target of the day (independent of the census): the next
+    // day's expected population E[t+1] (simulation.cpp:129-132)
+    uint32_t target;
+    {
+        const U4 r = draw(key, cd.c, t, P_M_TARGET, 0);
+        target = stoch_round(v.dens[sd.dens_off + t + 1] * cd.base, r.x);
+    }
 
     // ---- humans ----
     HumanCount hc{0, 0, 0, 0};
-    {
-        const uint4* w4p = reinterpret_cast<const uint4*>(v.h_word + cd.h_off);
-        const uint32_t nvec = (cd.n + 3) >> 2;
-        for (uint32_t q0 = 0; q0 < nvec; q0 += kUnroll * kWave) {
-            uint4 w4[kUnroll];
 #pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-                const uint32_t q = q0 + u * kWave + lane;
-                if (q < nvec) w4[u] = w4p[q];
-            }
+    for (uint32_t u = 0; u < kUnroll; ++u) {
+        const uint32_t q = u * kWave + lane;
+        if (q < nvec) human_vec(v, cd, sd, key, t, q, hb[u], hc);
+    }
+    for (uint32_t q0 = kUnroll * kWave; q0 < nvec; q0 += kUnroll * kWave) {
+        uint4 w4[kUnroll];
 #pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-                const uint32_t q = q0 + u * kWave + lane;
-                if (q >= nvec) continue;
-                const uint32_t ws[4] = {w4[u].x, w4[u].y, w4[u].z, w4[u].w};
+        for (uint32_t u = 0; u < kUnroll; ++u) {
+            const uint32_t q = q0 + u * kWave + lane;
+            if (q < nvec) w4[u] = w4p[q];
+        }
 #pragma unroll
-                for (uint32_t e = 0; e < 4; ++e) {
-                    const uint32_t i = 4 * q + e;
-                    if (i < cd.n) human_day(v, cd, sd, key, t, i, ws[e], hc);
-                }
-            }
+        for (uint32_t u = 0; u < kUnroll; ++u) {
+            const uint32_t q = q0 + u * kWave + lane;
+            if (q < nvec) human_vec(v, cd, sd, key, t, q, w4[u], hc);
         }
     }
 
-    // ---- mosquitoes: death and E->I (mdemographic.cu:68-98), two slots per u32 op ----
+    // ---- mosquitoes: death and E->I (mdemographic.cu:68-98), eight slots per u32 op ----
+    // A 16-byte vector holds slots 0..7 as four u32 pairs.  The status and
+    // strain fields of all eight are gathered into one word each, slot e at
+    // bit position 16*(e&1) + 2*(e>>1), and tested together.
     uint32_t im01 = 0, im23 = 0, alive = 0;
-    const uint32_t mvec = (cd.cap + 7) >> 3;
     const uint32_t nwords = (cd.cap + 31) >> 5;
     {
         // per 16-bit half: t mod 4096 plus a 4096 guard, so (t - death day) mod 4096
         // is the half's low 12 bits with no borrow between halves
         const uint32_t T2 = ((t & 0xFFFu) * 0x00010001u) | 0x10001000u;
-        const uint4* m4p = reinterpret_cast<const uint4*>(v.m_word + cd.m_off);
         for (uint32_t q0 = 0; q0 < mvec; q0 += kUnroll * kWave) {
             uint4 raw[kUnroll];
+            if (q0 == 0) {
 #pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
-                const uint32_t q = q0 + u * kWave + lane;
-                if (q < mvec) raw[u] = m4p[q];
+                for (uint32_t u = 0; u < kUnroll; ++u) raw[u] = mb[u];
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < kUnroll; ++u) {
+                    const uint32_t q = q0 + u * kWave + lane;
+                    if (q < mvec) raw[u] = m4p[q];
+                }
             }
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u) {
                 const uint32_t q = q0 + u * kWave + lane;
                 if (q >= mvec) continue;
                 const uint32_t rw[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
-                uint32_t dead8 = 0, died8 = 0, e8 = 0, i8 = 0, sa8 = 0, sb8 = 0;
+                uint32_t st = 0, sn = 0, dy = 0;
 #pragma unroll
                 for (uint32_t p = 0; p < 4; ++p) {
                     const uint32_t x = rw[p];
-                    const uint32_t d0 = x & (x >> 1) & 0x00010001u;                    // status 3: empty slot
-                    const uint32_t dying = (~(T2 - ((x >> 4) & 0x0FFF0FFFu)) >> 11) & 0x00010001u & ~d0;
-                    const uint32_t live = 0x00010001u & ~(d0 | dying);
-                    dead8 |= pack_halves(d0 | dying) << (2 * p);
-                    died8 |= pack_halves(dying) << (2 * p);
-                    e8 |= pack_halves(x & ~(x >> 1) & live) << (2 * p);               // status 1
-                    i8 |= pack_halves((x >> 1) & ~x & live) << (2 * p);               // status 2
-                    sa8 |= pack_halves(x >> 2) << (2 * p);
-                    sb8 |= pack_halves(x >> 3) << (2 * p);
+                    st |= (x & 0x00030003u) << (2 * p);
+                    sn |= ((x >> 2) & 0x00030003u) << (2 * p);
+                    dy |= ((~(T2 - ((x >> 4) & 0x0FFF0FFFu)) >> 11) & 0x00010001u) << (2 * p);
                 }
-                const uint32_t valid8 = (8 * q + 8 <= cd.cap) ? 0xFFu : ((1u << (cd.cap - 8 * q)) - 1u);
-                dead8 &= valid8;
-                died8 &= valid8;
-                e8 &= valid8;
-                i8 &= valid8;
-                while (e8) {  // E -> I when the extrinsic incubation is over
-                    const uint32_t e = __ffs(e8) - 1;
-                    e8 &= e8 - 1;
-                    const uint64_t gm = cd.m_off + 8 * q + e;
-                    if ((int16_t)(uint16_t)(t - v.m_next[gm]) > 0) {
-                        const uint32_t w = (rw[e >> 1] >> ((e & 1u) * 16u)) & 0xFFFFu;
-                        v.m_word[gm] = (uint16_t)((w & ~3u) | 2u);
-                        i8 |= 1u << e;
+                const uint32_t left = cd.cap - 8 * q;
+                const uint32_t vbyte = left >= 8 ? 0xFFu : ((1u << left) - 1u);
+                const uint32_t valid = (vbyte & 0x55u) | ((vbyte & 0xAAu) << 15);
+                const uint32_t b0 = st & 0x00550055u, b1 = (st >> 1) & 0x00550055u;
+                const uint32_t empty = b0 & b1;                       // status 3
+                const uint32_t died = dy & ~empty & valid;
+                const uint32_t dead = (empty | died) & valid;
+                const uint32_t live = valid & ~dead;
+                uint32_t em = b0 & ~b1 & live;                        // status 1
+                uint32_t im = b1 & ~b0 & live;                        // status 2
+                if (em) {  // E -> I when the extrinsic incubation is over; one 16-byte read of the next days
+                    const uint4 nx4 = *reinterpret_cast<const uint4*>(v.m_next + cd.m_off + 8 * q);
+                    const uint32_t nw[4] = {nx4.x, nx4.y, nx4.z, nx4.w};
+                    while (em) {
+                        const uint32_t b = __ffs(em) - 1;
+                        em &= em - 1;
+                        const uint32_t e = b < 16 ? b : b - 15;           // slot within the vector
+                        const uint32_t nx = (nw[e >> 1] >> ((e & 1u) * 16u)) & 0xFFFFu;
+                        if ((int16_t)(uint16_t)(t - nx) > 0) {
+                            const uint32_t w = (rw[e >> 1] >> ((e & 1u) * 16u)) & 0xFFFFu;
+                            v.m_word[cd.m_off + 8 * q + e] = (uint16_t)((w & ~3u) | 2u);
+                            im |= 1u << b;
+                        }
                     }
                 }
-                alive += __popc(valid8 & ~dead8);
-                im01 += __popc(i8 & ~sa8 & ~sb8) + (__popc(i8 & sa8 & ~sb8) << 16);
-                im23 += __popc(i8 & ~sa8 & sb8) + (__popc(i8 & sa8 & sb8) << 16);
-                s_dead8[q] = (uint8_t)dead8;
-                s_died8[q] = (uint8_t)died8;
+                const uint32_t sa = sn & 0x00550055u, sb = (sn >> 1) & 0x00550055u;
+                alive += __popc(live);
+                im01 += __popc(im & ~sa & ~sb) + (__popc(im & sa & ~sb) << 16);
+                im23 += __popc(im & ~sa & sb) + (__popc(im & sa & sb) << 16);
+                s_dead8[q] = (uint8_t)((dead & 0x55u) | ((dead >> 15) & 0xAAu));
+                s_died8[q] = (uint8_t)((died & 0x55u) | ((died >> 15) & 0xAAu));
             }
         }
         // the tail bytes of the last bitmap word
@@ -374,13 +438,13 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     // ---- emergence into the lowest-index dead slots (mdemographic.cu:104-142) ----
     uint32_t births;
     {
-        const U4 r = draw(key, cd.c, t, P_M_TARGET, 0);
-        // the target of day t is the next day's expected population E[t+1]
-        // (simulation.cpp:129-132)
-        const uint32_t target = stoch_round(v.dens[sd.dens_off + t + 1] * cd.base, r.x);
         const uint32_t b = target > alive_c ? target - alive_c : 0u;
         births = b < dead_c ? b : dead_c;
     }
+    for (uint32_t a = lane; a <= sd.death_m_max; a += kWave) s_dtab[a] = v.tables[sd.death_m_off + a];
+#pragma unroll
+    for (uint32_t k = 0; k < kStage; ++k) s_mguide[k * kWave + lane] = gstage[k];
+    if (lane < cd.row_len) s_row[lane] = rowent;
     __syncthreads();
     uint32_t dw[kWordsPerLane], kw[kWordsPerLane], wpre[kWordsPerLane];
     {
@@ -417,7 +481,7 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
                 while (!bits) bits = s_dead[++w];
                 pos = __ffs(bits) - 1;
             }
-            const uint32_t da = death_age(s_dtab, sd.death_m_max, (rr[j] >> 8) + 1);
+            const uint32_t da = death_age_guided(s_dtab, s_mguide, sd.death_m_max, (rr[j] >> 8) + 1);
             v.m_word[cd.m_off + 32u * w + pos] = (uint16_t)(((t + da) & 0xFFFu) << 4);
         }
     }
@@ -462,21 +526,32 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
         if (fx) atomicAdd(&v.lam[((size_t)(t & 1u) * v.n_sims + cd.sim) * 8 + lane], fx);
     }
     __syncthreads();
-    uint32_t pre[9];
-    pre[0] = 0;
+    // Destinations: event i of source segment k takes word i%4 of Philox
+    // block i/4 (one block per lane and quad of events); the product
+    // word * row length gives the column (high half) and the alias test
+    // uniform (low half).
+    const uint32_t rl = cd.row_len;
+    uint32_t qpre[9];
+    qpre[0] = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) pre[k + 1] = pre[k] + s_n[k];
-    const uint32_t E = pre[8];
-    for (uint32_t ev = lane; ev < E; ev += kWave) {
+    for (uint32_t k = 0; k < 8; ++k) qpre[k + 1] = qpre[k] + ((s_n[k] + 3) >> 2);
+    for (uint32_t gq = lane; gq < qpre[8]; gq += kWave) {
         uint32_t k = 0;
 #pragma unroll
-        for (uint32_t q = 1; q < 8; ++q) k += (ev >= pre[q]) ? 1u : 0u;
-        const uint32_t i = ev - pre[k];
-        const U4 r = draw(key, cd.c, t, ctr_tag(P_DEST, k >> 2, k & 3u, 0), i);
-        uint32_t col = mulhi(r.x, cd.row_len);
-        const uint32_t ent = v.csr_thr[cd.row_off + col];
-        if ((r.y >> 8) >= (ent & 0x1FFFFFFu)) col = ent >> 27;
-        atomicAdd(&s_hist[k * 9 + col], 1u);
+        for (uint32_t q = 1; q < 8; ++q) k += (gq >= qpre[q]) ? 1u : 0u;
+        const uint32_t j = gq - qpre[k];
+        const uint32_t n = s_n[k];
+        const U4 r = draw(key, cd.c, t, ctr_tag(P_DEST, k >> 2, k & 3u, 0), j);
+        const uint32_t w4[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+            if (4 * j + e >= n) break;
+            const uint64_t prod = (uint64_t)w4[e] * rl;
+            uint32_t col = (uint32_t)(prod >> 32);
+            const uint32_t en = s_row[col];
+            if ((((uint32_t)prod) >> 8) >= (en & 0x1FFFFFFu)) col = en >> 27;
+            atomicAdd(&s_hist[k * 9 + col], 1u);
+        }
     }
     __syncthreads();
     for (uint32_t i = lane; i < kCntPerComm; i += kWave) v.cnt[(size_t)g * kCntPerComm + i] = s_hist[i];
@@ -499,28 +574,48 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
 
 // ---------------------------------------------------------------------------
 // Day kernel B: application of the day's events at one destination.
-struct EventRef { uint32_t s, x0, x2, k; };
+//
+// The events of a destination and direction form segments in a fixed order:
+// for each serotype of the rotation, one segment per in-edge (ascending
+// source) and then the long-distance arrivals.  Event k of a segment draws its
+// target from word k%4 of the segment's Philox block k/4, so a lane handles a
+// quad of consecutive events with one block.  Quads are applied in chunks of
+// kChunkQuads with an LDS claim-min table.
 
-__device__ __forceinline__ EventRef decode_event(uint32_t r, const uint32_t* seg, uint32_t nseg, uint32_t in_len,
-                                                 uint32_t rot, uint32_t S, uint32_t dir, const uint32_t* src_local,
-                                                 const uint32_t* slot, uint32_t self_c) {
+constexpr uint32_t kChunkQuads = kChunk / 4;   // 128 quads = up to 512 events
+constexpr uint32_t kQuadsPerLane = kChunkQuads / kWave;  // 2
+
+struct Quad {
+    uint32_t s;       // serotype
+    uint32_t x0, x2;  // Philox counter words of the segment
+    uint32_t j;       // quad index within the segment (Philox block)
+    uint32_t rank;    // rank of the quad's first event within the direction
+    uint32_t count;   // events in the quad (1..4)
+};
+
+__device__ __forceinline__ Quad decode_quad(uint32_t gq, const uint32_t* seg, const uint32_t* qseg, uint32_t nseg,
+                                            uint32_t in_len, uint32_t rot, uint32_t S, uint32_t dir,
+                                            const uint32_t* src_local, const uint32_t* slot, uint32_t self_c) {
     uint32_t lo = 0, hi = nseg - 1;
     while (lo < hi) {
         const uint32_t mid = (lo + hi + 1) >> 1;
-        if (seg[mid] <= r) lo = mid; else hi = mid - 1;
+        if (qseg[mid] <= gq) lo = mid; else hi = mid - 1;
     }
     const uint32_t pos = lo / (in_len + 1), e = lo % (in_len + 1);
-    EventRef ev;
-    ev.s = (rot + pos) % S;
-    ev.k = r - seg[lo];
+    Quad q;
+    q.s = (rot + pos) % S;
+    q.j = gq - qseg[lo];
+    q.rank = seg[lo] + 4 * q.j;
+    const uint32_t left = seg[lo + 1] - q.rank;
+    q.count = left < 4 ? left : 4;
     if (e < in_len) {
-        ev.x0 = src_local[e];
-        ev.x2 = ctr_tag(P_TGT, dir, ev.s, slot[e]);
+        q.x0 = src_local[e];
+        q.x2 = ctr_tag(P_TGT, dir, q.s, slot[e]);
     } else {
-        ev.x0 = self_c;
-        ev.x2 = ctr_tag(P_TGT_LD, dir, ev.s, 0);
+        q.x0 = self_c;
+        q.x2 = ctr_tag(P_TGT_LD, dir, q.s, 0);
     }
-    return ev;
+    return q;
 }
 
 // The q-th alive slot of the community, from its bitmap words and their
@@ -576,6 +671,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     __shared__ uint32_t s_nld[8];
     __shared__ uint32_t s_rot[2];
     __shared__ uint32_t s_seg[2][4 * 10 + 1];
+    __shared__ uint32_t s_qseg[2][4 * 10 + 1];
     __shared__ uint32_t s_word[kMaxWords];
     __shared__ uint32_t s_wpre[kMaxWords];
     __shared__ uint32_t s_tab[kHash];
@@ -584,27 +680,27 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
         s_src[lane] = v.in_edge[cd.in_off + lane];
         s_slot[lane] = v.in_slot[cd.in_off + lane];
     }
+    uint32_t tot_dir[2] = {0, 0};
+    if (lane < 8) {  // long-distance arrivals of the day, consumed
+        const uint32_t n = v.nld[(size_t)g * 8 + lane];
+        s_nld[lane] = n;
+        if (n) v.nld[(size_t)g * 8 + lane] = 0;
+        tot_dir[lane >> 2] += n;
+    }
+    if (cd.c == 0 && lane < 8) v.lam[((size_t)((t + 1) & 1u) * v.n_sims + cd.sim) * 8 + lane] = 0ull;
     __syncthreads();
     for (uint32_t i = lane; i < 8 * in_len; i += kWave) {
         const uint32_t k = i / in_len, e = i % in_len;      // k = dir*4 + s
-        s_ecnt[k][e] = ((k & 3u) < S) ? v.cnt[(size_t)s_src[e] * kCntPerComm + k * 9 + s_slot[e]] : 0u;
+        const uint32_t n = ((k & 3u) < S) ? v.cnt[(size_t)s_src[e] * kCntPerComm + k * 9 + s_slot[e]] : 0u;
+        s_ecnt[k][e] = n;
+        tot_dir[k >> 2] += n;
     }
-    if (lane < 8) {
-        const uint32_t dir = lane >> 2, s = lane & 3u;
-        uint32_t n = 0;
-        if (s < S) {
-            const unsigned long long fx = v.lam[((size_t)(t & 1u) * v.n_sims + cd.sim) * 8 + lane];
-            const double mu = (double)fx * 0x1p-24 / (double)sd.C;
-            const Stream st{key, cd.c, t, ctr_tag(P_NLD, dir, s, 0)};
-            n = poisson(mu, st);
-        }
-        s_nld[lane] = n;
-    }
+    tot_dir[0] = wave_sum(tot_dir[0]);
+    tot_dir[1] = wave_sum(tot_dir[1]);
+    if (tot_dir[0] == 0 && tot_dir[1] == 0 && !(sd.intro > 0.0f)) return;   // nothing arrives here today
     if (lane < 2) s_rot[lane] = mulhi(draw(key, cd.c, t, ctr_tag(P_ROT, lane, 0, 0), 0).x, S);
-    if (cd.c == 0 && lane < 8) v.lam[((size_t)((t + 1) & 1u) * v.n_sims + cd.sim) * 8 + lane] = 0ull;
     __syncthreads();
-    // segment starts: for each direction, in rotation order, the in-edges then
-    // the long-distance arrivals (one lane per segment, then a wave scan)
+    // segment starts in events and in quads (one lane per segment, wave scans)
 #pragma unroll
     for (uint32_t dir = 0; dir < 2; ++dir) {
         uint32_t cnt = 0;
@@ -613,17 +709,18 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
             const uint32_t s = (s_rot[dir] + pos) % S;
             cnt = (e < in_len) ? s_ecnt[dir * 4 + s][e] : s_nld[dir * 4 + s];
         }
-        uint32_t total;
+        uint32_t total, qtotal;
         const uint32_t start = wave_exclusive_scan(cnt, total);
-        if (lane < nseg) s_seg[dir][lane] = start;
-        if (lane == 0) s_seg[dir][nseg] = total;
+        const uint32_t qstart = wave_exclusive_scan((cnt + 3) >> 2, qtotal);
+        if (lane < nseg) { s_seg[dir][lane] = start; s_qseg[dir][lane] = qstart; }
+        if (lane == 0) { s_seg[dir][nseg] = total; s_qseg[dir][nseg] = qtotal; }
     }
     if (lane < in_len) s_src[lane] -= sd.comm_off;   // simulation-local source ids
     __syncthreads();
 
     // ---- onto mosquitoes: uniform over alive slots, success iff susceptible ----
-    const uint32_t tot_m = s_seg[DIR_HM][nseg];
-    if (tot_m > 0) {
+    const uint32_t tq_m = s_qseg[DIR_HM][nseg];
+    if (tq_m > 0) {
         const uint32_t nwords = (cd.cap + 31) >> 5;
         uint32_t wv[kWordsPerLane], c = 0;
 #pragma unroll
@@ -641,34 +738,42 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
             pre += __popc(wv[k]);
         }
         if (Md > 0) {
-            for (uint32_t cb = 0; cb < tot_m; cb += kChunk) {
-                const uint32_t ce = tot_m - cb < kChunk ? tot_m : cb + kChunk;
+            for (uint32_t cq = 0; cq < tq_m; cq += kChunkQuads) {
+                const uint32_t cqe = tq_m - cq < kChunkQuads ? tq_m : cq + kChunkQuads;
                 for (uint32_t i = lane; i < kHash; i += kWave) s_tab[i] = kEmpty;
                 __syncthreads();
-                uint32_t cache[kPerLane];
+                const uint32_t base = decode_quad(cq, s_seg[DIR_HM], s_qseg[DIR_HM], nseg, in_len, s_rot[DIR_HM], S,
+                                                  DIR_HM, s_src, s_slot, cd.c).rank;
+                uint32_t cache[kPerLane], qrank[kQuadsPerLane];
 #pragma unroll
-                for (uint32_t u = 0; u < kPerLane; ++u) {
-                    const uint32_t r = cb + u * kWave + lane;
-                    cache[u] = 0;
-                    if (r < ce) {
-                        const EventRef ev = decode_event(r, s_seg[DIR_HM], nseg, in_len, s_rot[DIR_HM], S, DIR_HM,
-                                                         s_src, s_slot, cd.c);
-                        const uint32_t q = mulhi(draw(key, ev.x0, t, ev.x2, ev.k).x, Md);
-                        const uint32_t slot = select_alive(q, s_word, s_wpre, nwords);
+                for (uint32_t u = 0; u < kQuadsPerLane; ++u) {
+                    const uint32_t gq = cq + u * kWave + lane;
+                    qrank[u] = 0;
+#pragma unroll
+                    for (uint32_t e = 0; e < 4; ++e) cache[4 * u + e] = 0;
+                    if (gq >= cqe) continue;
+                    const Quad q = decode_quad(gq, s_seg[DIR_HM], s_qseg[DIR_HM], nseg, in_len, s_rot[DIR_HM], S,
+                                               DIR_HM, s_src, s_slot, cd.c);
+                    qrank[u] = q.rank - base;
+                    const U4 r = draw(key, q.x0, t, q.x2, q.j);
+                    const uint32_t w4[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+                    for (uint32_t e = 0; e < 4; ++e) {
+                        if (e >= q.count) break;
+                        const uint32_t slot = select_alive(mulhi(w4[e], Md), s_word, s_wpre, nwords);
                         if ((v.m_word[cd.m_off + slot] & 3u) == 0) {
-                            claim(s_tab, slot, r - cb);
-                            cache[u] = kEligible | (ev.s << kKeyBits) | slot;
+                            claim(s_tab, slot, qrank[u] + e);
+                            cache[4 * u + e] = kEligible | (q.s << kKeyBits) | slot;
                         }
                     }
                 }
                 __syncthreads();
 #pragma unroll
-                for (uint32_t u = 0; u < kPerLane; ++u) {
-                    if (!(cache[u] & kEligible)) continue;
-                    const uint32_t r = cb + u * kWave + lane;
-                    const uint32_t slot = cache[u] & kKeyMask;
-                    if (claimed_rank(s_tab, slot) != r - cb) continue;
-                    const uint32_t s = (cache[u] >> kKeyBits) & 3u;
+                for (uint32_t i = 0; i < kPerLane; ++i) {
+                    if (!(cache[i] & kEligible)) continue;
+                    const uint32_t slot = cache[i] & kKeyMask;
+                    if (claimed_rank(s_tab, slot) != qrank[i >> 2] + (i & 3u)) continue;
+                    const uint32_t s = (cache[i] >> kKeyBits) & 3u;
                     const uint64_t gm = cd.m_off + slot;
                     const uint32_t w = v.m_word[gm];
                     v.m_word[gm] = (uint16_t)((w & ~0xFu) | (s << 2) | 1u);
@@ -681,40 +786,48 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     }
 
     // ---- onto humans: uniform target, success iff S and never infected by s ----
-    const uint32_t tot_h = s_seg[DIR_MH][nseg];
-    if (tot_h > 0 || sd.intro > 0.0f) {
+    if (s_qseg[DIR_MH][nseg] > 0 || sd.intro > 0.0f) {
         const uint32_t N = cd.n;
         for (uint32_t pos = 0; pos < S; ++pos) {
             const uint32_t s = (s_rot[DIR_MH] + pos) % S;
-            const uint32_t rb = s_seg[DIR_MH][pos * (in_len + 1)];
-            const uint32_t re = s_seg[DIR_MH][(pos + 1) * (in_len + 1)];
-            for (uint32_t cb = rb; cb < re; cb += kChunk) {
-                const uint32_t ce = re - cb < kChunk ? re : cb + kChunk;
+            const uint32_t qb = s_qseg[DIR_MH][pos * (in_len + 1)];
+            const uint32_t qe = s_qseg[DIR_MH][(pos + 1) * (in_len + 1)];
+            for (uint32_t cq = qb; cq < qe; cq += kChunkQuads) {
+                const uint32_t cqe = qe - cq < kChunkQuads ? qe : cq + kChunkQuads;
                 for (uint32_t i = lane; i < kHash; i += kWave) s_tab[i] = kEmpty;
                 __syncthreads();
-                uint32_t cache[kPerLane];
+                const uint32_t base = decode_quad(cq, s_seg[DIR_MH], s_qseg[DIR_MH], nseg, in_len, s_rot[DIR_MH], S,
+                                                  DIR_MH, s_src, s_slot, cd.c).rank;
+                uint32_t cache[kPerLane], qrank[kQuadsPerLane];
 #pragma unroll
-                for (uint32_t u = 0; u < kPerLane; ++u) {
-                    const uint32_t r = cb + u * kWave + lane;
-                    cache[u] = 0;
-                    if (r < ce) {
-                        const EventRef ev = decode_event(r, s_seg[DIR_MH], nseg, in_len, s_rot[DIR_MH], S, DIR_MH,
-                                                         s_src, s_slot, cd.c);
-                        const uint32_t h = mulhi(draw(key, ev.x0, t, ev.x2, ev.k).x, N);
+                for (uint32_t u = 0; u < kQuadsPerLane; ++u) {
+                    const uint32_t gq = cq + u * kWave + lane;
+                    qrank[u] = 0;
+#pragma unroll
+                    for (uint32_t e = 0; e < 4; ++e) cache[4 * u + e] = 0;
+                    if (gq >= cqe) continue;
+                    const Quad q = decode_quad(gq, s_seg[DIR_MH], s_qseg[DIR_MH], nseg, in_len, s_rot[DIR_MH], S,
+                                               DIR_MH, s_src, s_slot, cd.c);
+                    qrank[u] = q.rank - base;
+                    const U4 r = draw(key, q.x0, t, q.x2, q.j);
+                    const uint32_t w4[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+                    for (uint32_t e = 0; e < 4; ++e) {
+                        if (e >= q.count) break;
+                        const uint32_t h = mulhi(w4[e], N);
                         const uint32_t w = v.h_word[cd.h_off + h];
                         if ((w & 3u) == 0 && !((w >> (4 + s)) & 1u)) {
-                            claim(s_tab, h, r - cb);
-                            cache[u] = kEligible | h;
+                            claim(s_tab, h, qrank[u] + e);
+                            cache[4 * u + e] = kEligible | h;
                         }
                     }
                 }
                 __syncthreads();
 #pragma unroll
-                for (uint32_t u = 0; u < kPerLane; ++u) {
-                    if (!(cache[u] & kEligible)) continue;
-                    const uint32_t r = cb + u * kWave + lane;
-                    const uint32_t h = cache[u] & kKeyMask;
-                    if (claimed_rank(s_tab, h) != r - cb) continue;
+                for (uint32_t i = 0; i < kPerLane; ++i) {
+                    if (!(cache[i] & kEligible)) continue;
+                    const uint32_t h = cache[i] & kKeyMask;
+                    if (claimed_rank(s_tab, h) != qrank[i >> 2] + (i & 3u)) continue;
                     const uint64_t gi = cd.h_off + h;
                     const uint32_t w = v.h_word[gi];
                     v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
@@ -745,6 +858,46 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Long-distance events of the day (nepidemic.cu:118-126, mepidemic.cu:117-122:
+// a long-distance event lands on a uniform random community).  Per simulation,
+// direction and serotype the events are L ~ Poisson(summed long-distance rate
+// of all sources); event l lands on community mulhi(Philox(l), C).  Only the
+// number landing on each destination matters (their targets are drawn at the
+// destination), so the destinations are counted with integer atomics.
+constexpr uint32_t kLdSplit = 8;   // workgroups per simulation
+
+__global__ __launch_bounds__(256) void k_day_longdist(EngineView v) {
+    const uint32_t sim = blockIdx.x / kLdSplit, part = blockIdx.x % kLdSplit, tid = threadIdx.x;
+    const uint32_t t = *v.day;
+    const SimDesc sd = v.sims[sim];
+    const Key key{sd.k0, sd.k1};
+    __shared__ uint32_t s_L[8];
+    if (tid < 8) {
+        const uint32_t dir = tid >> 2, s = tid & 3u;
+        uint32_t L = 0;
+        if (s < sd.S) {
+            const unsigned long long fx = v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + tid];
+            const Stream st{key, INIT_DAY, t, ctr_tag(P_NLD, dir, s, 0)};
+            L = poisson((double)fx * 0x1p-24, st);
+        }
+        s_L[tid] = L;
+    }
+    __syncthreads();
+    uint32_t pre[9];
+    pre[0] = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) pre[k + 1] = pre[k] + s_L[k];
+    for (uint32_t l = part * 256 + tid; l < pre[8]; l += kLdSplit * 256) {
+        uint32_t k = 0;
+#pragma unroll
+        for (uint32_t q = 1; q < 8; ++q) k += (l >= pre[q]) ? 1u : 0u;
+        const uint32_t idx = l - pre[k];
+        const uint32_t d = mulhi(draw(key, idx, t, ctr_tag(P_LD_DEST, k >> 2, k & 3u, 0), 0).x, sd.C);
+        atomicAdd(&v.nld[(size_t)(sd.comm_off + d) * 8 + k], 1u);
+    }
+}
+
 __global__ void k_advance_day(uint32_t* day) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *day = *day + 1u;
 }
@@ -768,6 +921,11 @@ hipError_t launch_demography(const EngineView& v, hipStream_t st) {
 
 hipError_t launch_events(const EngineView& v, hipStream_t st) {
     hipLaunchKernelGGL(k_day_events, dim3(v.n_comm), dim3(kWave), 0, st, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_longdist(const EngineView& v, hipStream_t st) {
+    hipLaunchKernelGGL(k_day_longdist, dim3(v.n_sims * kLdSplit), dim3(256), 0, st, v);
     return hipGetLastError();
 }
 

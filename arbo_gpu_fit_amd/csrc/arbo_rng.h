@@ -54,13 +54,14 @@ enum Purpose : uint32_t {
     P_M_TARGET = 13,   // community emergence target rounding
     P_NLOC = 20,       // local event count of a source (community, dir, serotype)
     P_DEST = 21,       // alias-table destination of the k-th local event
-    P_NLD = 22,        // long-distance arrivals at a destination
+    P_NLD = 22,        // long-distance events of a simulation (x0 = INIT_DAY)
     P_ROT = 23,        // serotype rotation start at a destination
     P_TGT = 24,        // target of the k-th local event on CSR slot j
     P_TGT_LD = 25,     // target of the k-th long-distance arrival
     P_H_IIP = 26,      // newly infected human: intrinsic incubation
     P_M_EIP = 27,      // newly infected mosquito: extrinsic incubation
     P_INTRO = 28,      // external introductions (x3 = 0 count, 1+a attempt a)
+    P_LD_DEST = 29,    // destination of the l-th long-distance event (x0 = l)
 };
 
 // Direction of a transmission event.

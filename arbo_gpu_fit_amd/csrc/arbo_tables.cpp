@@ -193,6 +193,18 @@ void build_tables(const ArboShape& sh, const ArboParams& p, uint64_t seed, uint3
     }
     threshold_table(kHumanAgeTable, [&](double a) { return human_cdf(p, a); }, t.death_h);
     threshold_table(kMosqAgeTable, [&](double a) { return mosq_cdf(p, a); }, t.death_m);
+    // the largest death age the mosquito table returns, and the smallest
+    // death age of every 2^16-wide bucket of 24-bit PLE values
+    t.death_m_max = kMosqAgeTable - 1;
+    for (uint32_t a = 1; a < kMosqAgeTable; ++a)
+        if (t.death_m[a] >= kOne24) { t.death_m_max = a; break; }
+    t.death_m_guide.resize(kGuideBuckets);
+    for (uint32_t b = 0; b < kGuideBuckets; ++b) {
+        const uint32_t k = b * 65536u + 1u;
+        uint32_t a = 1;
+        while (a < t.death_m_max && t.death_m[a] < k) ++a;
+        t.death_m_guide[b] = a;
+    }
     initial_age_table(kHumanYears, 365, [&](double a) { return human_cdf(p, a); }, t.init_h);
     initial_age_table(kMosqInitDays, 1, [&](double a) { return mosq_cdf(p, a); }, t.init_m);
     t.dens.resize((size_t)max_days + 1);

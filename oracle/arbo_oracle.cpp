@@ -20,14 +20,16 @@
 //     F tabulated once in double, PLE a 24-bit uniform;
 //   - Poisson event counts (local ~ Poisson(lambda(1-w)), long-distance
 //     ~ Poisson(lambda w)) instead of decimalResolve splits; local
-//     destinations from per-row alias tables, long-distance arrivals drawn at
-//     the destination from the summed long-distance rate (Poisson thinning);
+//     destinations from per-row alias tables; the long-distance events of a
+//     simulation are drawn as one Poisson total (superposition) with uniform
+//     destinations, so only their count per destination is kept;
 //   - emergence per community into its lowest-index dead slots;
 //   - mosquito targets uniform over alive slots (the reference retries dead
 //     picks until an alive one is found: the same law);
 //   - fixed event order per destination: serotype rotation from a keyed
 //     start, local events by source community then event index, then the
-//     long-distance arrivals.
+//     long-distance arrivals; the k-th event of a (source, serotype, slot)
+//     segment draws its target from word k%4 of Philox block k/4.
 // Serial semantics throughout: each event checks the target's current state.
 #include "arbo_c.h"
 #include "oracle_rng.h"
@@ -76,6 +78,7 @@ struct Sim {
     std::vector<uint32_t> Ih, Im, inc, Mc;
     std::vector<uint64_t> ldfx;     // [C][2][4] long-distance rate, 2^-24 fixed point
     std::vector<uint64_t> lam;      // [2][4] summed long-distance rate
+    std::vector<uint32_t> nld;      // [C][2][4] long-distance arrivals of the day
     // series
     std::vector<uint32_t> n_count, m_count, n_inf, n_inc, n_one;
 };
@@ -293,6 +296,7 @@ void init_sim(Sim& s, uint32_t max_days) {
     s.inc.assign((size_t)s.C * S, 0); s.Mc.assign(s.C, 0);
     s.ldfx.assign((size_t)s.C * 8, 0);
     s.lam.assign(8, 0);
+    s.nld.assign((size_t)s.C * 8, 0);
     s.n_count.assign(max_days + 1, 0); s.m_count.assign(max_days + 1, 0);
     s.n_inf.assign((size_t)(max_days + 1) * S, 0);
     s.n_inc.assign((size_t)(max_days + 1) * S, 0);
@@ -479,10 +483,14 @@ void phase_a(Sim& s, uint32_t c, uint32_t t) {
                     float mu_ld = lam * p.long_distance;
                     fx = (uint64_t)std::floor((double)mu_ld * 16777216.0);
                     uint32_t n = poisson_d((double)mu_loc, s.key, c, t, tag(NLOC, dir, v, 0));
+                    // event k takes word k%4 of block k/4; the product word * row
+                    // length gives the column (high half) and the alias test
+                    // uniform (low half)
                     for (uint32_t k = 0; k < n; ++k) {
-                        W4 r = rnd(s.key, c, t, tag(DEST, dir, v, 0), k);
-                        uint32_t col = range(r.v[0], rl);
-                        if ((r.v[1] >> 8) >= s.thr[r0 + col]) col = s.alias[r0 + col];
+                        W4 r = rnd(s.key, c, t, tag(DEST, dir, v, 0), k >> 2);
+                        const uint64_t prod = (uint64_t)r.v[k & 3] * rl;
+                        uint32_t col = (uint32_t)(prod >> 32);
+                        if (((uint32_t)prod >> 8) >= s.thr[r0 + col]) col = s.alias[r0 + col];
                         cnt[(dir * 4 + v) * 9 + col]++;
                     }
                 }
@@ -523,12 +531,8 @@ bool human_eligible(uint32_t w, uint32_t v) { return (w & 3u) == 0 && !((w >> (4
 void phase_b(Sim& s, uint32_t d, uint32_t t) {
     const ArboParams& p = s.p;
     const uint32_t S = s.S;
-    uint32_t nld[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-    for (uint32_t dir = 0; dir < 2; ++dir)
-        for (uint32_t v = 0; v < S; ++v) {
-            double mu = (double)s.lam[dir * 4 + v] * 0x1p-24 / (double)s.C;
-            nld[dir][v] = poisson_d(mu, s.key, d, t, tag(NLD, dir, v, 0));
-        }
+    uint32_t nld[2][4];
+    for (uint32_t k = 0; k < 8; ++k) nld[k >> 2][k & 3] = s.nld[(size_t)d * 8 + k];
     std::vector<std::vector<Event>> ev;
     // onto mosquitoes (mepidemic.cu:143-246)
     {
@@ -541,8 +545,8 @@ void phase_b(Sim& s, uint32_t d, uint32_t t) {
         if (Md > 0) {
             for (uint32_t pos = 0; pos < S; ++pos)
                 for (const Event& E : ev[pos]) {
-                    W4 r = rnd(s.key, E.x0, t, E.x2, E.k);
-                    uint32_t m = (uint32_t)(s.m_off[d] + alive_slots[range(r.v[0], Md)]);
+                    W4 r = rnd(s.key, E.x0, t, E.x2, E.k >> 2);
+                    uint32_t m = (uint32_t)(s.m_off[d] + alive_slots[range(r.v[E.k & 3], Md)]);
                     uint32_t w = s.mw[m];
                     if ((w & 3u) != 0) continue;
                     W4 r2 = rnd(s.key, m, t, M_EIP, 0);
@@ -559,8 +563,8 @@ void phase_b(Sim& s, uint32_t d, uint32_t t) {
         for (uint32_t pos = 0; pos < S; ++pos) {
             uint32_t v = (s0 + pos) % S;
             for (const Event& E : ev[pos]) {
-                W4 r = rnd(s.key, E.x0, t, E.x2, E.k);
-                uint32_t h = (uint32_t)(s.h_off[d] + range(r.v[0], Nd));
+                W4 r = rnd(s.key, E.x0, t, E.x2, E.k >> 2);
+                uint32_t h = (uint32_t)(s.h_off[d] + range(r.v[E.k & 3], Nd));
                 if (human_eligible(s.hw[h], v)) infect_human(s, h, v, t);
             }
             if (p.external_infection > 0.0f) {  // nepidemic.cu:261-309
@@ -576,6 +580,24 @@ void phase_b(Sim& s, uint32_t d, uint32_t t) {
     }
 }
 
+// Long-distance arrivals of day t (nepidemic.cu:118-126: each long-distance
+// event goes to a uniform random community).  The simulation's long-distance
+// events of one direction and serotype are L ~ Poisson(sum of the sources'
+// long-distance rates); event l lands on a uniform community.
+void long_distance(Sim& s, uint32_t t) {
+    std::fill(s.nld.begin(), s.nld.end(), 0u);
+    for (uint32_t dir = 0; dir < 2; ++dir)
+        for (uint32_t v = 0; v < s.S; ++v) {
+            const uint32_t k = dir * 4 + v;
+            const double mu = (double)s.lam[k] * 0x1p-24;
+            const uint32_t L = poisson_d(mu, s.key, DAY_INIT, t, tag(NLD, dir, v, 0));
+            for (uint32_t l = 0; l < L; ++l) {
+                const uint32_t d = range(rnd(s.key, l, t, tag(LD_DEST, dir, v, 0), 0).v[0], s.C);
+                s.nld[(size_t)d * 8 + k]++;
+            }
+        }
+}
+
 void sim_day(Sim& s, uint32_t t, int threads) {
     const int C = (int)s.C;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads) if (threads > 1)
@@ -584,6 +606,7 @@ void sim_day(Sim& s, uint32_t t, int threads) {
     for (uint32_t c = 0; c < s.C; ++c)
         for (uint32_t k = 0; k < 8; ++k) s.lam[k] += s.ldfx[(size_t)c * 8 + k];
     record_row(s, t + 1);
+    long_distance(s, t);
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads) if (threads > 1)
     for (int d = 0; d < C; ++d) phase_b(s, (uint32_t)d, t);
 }
