@@ -15,7 +15,8 @@
 namespace arbo {
 
 constexpr uint32_t kWave = 64;            // threads per workgroup: one wavefront per community
-constexpr uint32_t kCntPerComm = 72;      // event counts: [dir 2][serotype 4][row slot 9]
+constexpr uint32_t kCntPerComm = 72;      // event counts of a source: [dir 2][serotype 4][row slot 9]
+constexpr uint32_t kInboxEdges = 9;       // in-edges per destination (Moore-8 + self)
 constexpr uint32_t kHumanAlign = 4;       // humans per 16-byte vector
 constexpr uint32_t kMosqAlign = 32;       // mosquito slots per alive-bitmap word (and 4 vectors)
 
@@ -84,11 +85,11 @@ struct EngineView {
     uint16_t* m_word;
     uint16_t* m_next;
     uint32_t* alive;               // [M/32] alive-slot bitmap of the current day
-    uint32_t* cnt;                 // [G][72] event counts of the current day
+    uint32_t* cnt;                 // [G][9 in-edges][8 = dir*4 + serotype] local events arriving today
     uint32_t* nld;                 // [G][8] long-distance arrivals of the current day
     unsigned long long* lam;       // [2 parity][n_sims][8]  long-distance rate, 2^-24 fixed point
     uint32_t* series;
-    const uint32_t* csr_to;        // simulation-local destination
+    const uint32_t* csr_inbox;     // per CSR entry: inbox row (destination * 9 + in-edge index)
     const uint32_t* csr_thr;       // bits 0..24 threshold, 27..31 alias slot
     const uint32_t* in_edge;       // global source community
     const uint32_t* in_slot;       // slot of the destination in the source's row

@@ -63,7 +63,7 @@ struct ArboEngine {
     // device memory
     DevBuf<CommDesc> comm;
     DevBuf<SimDesc> simd;
-    DevBuf<uint32_t> h_word, alive, cnt, nld, series, csr_thr, in_edge, in_slot, tables, day_ctr;
+    DevBuf<uint32_t> h_word, alive, cnt, nld, series, csr_thr, csr_inbox, in_edge, in_slot, tables, day_ctr;
     DevBuf<uint16_t> h_next, m_word, m_next;
     DevBuf<int32_t> h_birth;
     DevBuf<unsigned long long> lam;
@@ -116,7 +116,7 @@ void free_batch(ArboEngine* e) {
     if (e->graph_exec) { (void)hipGraphExecDestroy(e->graph_exec); e->graph_exec = nullptr; }
     if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
     dev_free(e->comm); dev_free(e->simd);
-    dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->series); dev_free(e->csr_thr);
+    dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->series); dev_free(e->csr_thr); dev_free(e->csr_inbox);
     dev_free(e->in_edge); dev_free(e->in_slot); dev_free(e->tables); dev_free(e->day_ctr);
     dev_free(e->h_next); dev_free(e->m_word); dev_free(e->m_next); dev_free(e->h_birth);
     dev_free(e->lam); dev_free(e->dens);
@@ -276,7 +276,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     e->sims.assign(n_sims, SimHost());
     e->sim_desc.assign(n_sims, SimDesc());
     std::vector<CommDesc> comm;
-    std::vector<uint32_t> csr_thr, in_edge, in_slot, table_pool;
+    std::vector<uint32_t> csr_thr, csr_inbox, in_edge, in_slot, table_pool;
     std::vector<float> dens_pool;
     std::map<std::vector<uint32_t>, uint32_t> seen_u32;
     std::map<std::vector<float>, uint32_t> seen_f32;
@@ -328,6 +328,13 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
 
         const uint32_t row_base = (uint32_t)csr_thr.size();
         for (uint32_t k = 0; k < t.thr.size(); ++k) csr_thr.push_back(t.thr[k] | (t.alias[k] << 27));
+        // inbox row of every CSR entry: (global destination) * 9 + (index of
+        // the source among the destination's in-edges)
+        csr_inbox.resize(csr_thr.size());
+        for (uint32_t d = 0; d < t.C; ++d)
+            for (uint32_t e = t.in_ptr[d]; e < t.in_ptr[d + 1]; ++e)
+                csr_inbox[row_base + t.row_ptr[t.in_src[e]] + t.in_slot[e]] =
+                    (sh.comm_off + d) * kInboxEdges + (e - t.in_ptr[d]);
         const uint32_t in_base = (uint32_t)in_edge.size();
         for (uint32_t k = 0; k < t.in_src.size(); ++k) {
             in_edge.push_back(t.in_src[k] + sh.comm_off);
@@ -373,9 +380,9 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     if ((rc = dev_alloc(e, e->comm, comm.size())) || (rc = dev_alloc(e, e->simd, n_sims)) ||
         (rc = dev_alloc(e, e->h_word, h_tot)) || (rc = dev_alloc(e, e->h_next, h_tot)) ||
         (rc = dev_alloc(e, e->h_birth, h_tot)) || (rc = dev_alloc(e, e->m_word, m_tot)) ||
-        (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kCntPerComm)) || (rc = dev_alloc(e, e->nld, comm.size() * 8)) ||
+        (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kInboxEdges * 8)) || (rc = dev_alloc(e, e->nld, comm.size() * 8)) ||
         (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->series, ser_tot)) ||
-        (rc = dev_alloc(e, e->csr_thr, csr_thr.size())) || (rc = dev_alloc(e, e->in_edge, in_edge.size())) ||
+        (rc = dev_alloc(e, e->csr_thr, csr_thr.size())) || (rc = dev_alloc(e, e->csr_inbox, csr_inbox.size())) || (rc = dev_alloc(e, e->in_edge, in_edge.size())) ||
         (rc = dev_alloc(e, e->in_slot, in_slot.size())) || (rc = dev_alloc(e, e->tables, table_pool.size())) ||
         (rc = dev_alloc(e, e->dens, dens_pool.size())) || (rc = dev_alloc(e, e->day_ctr, 1))) {
         free_batch(e);
@@ -384,6 +391,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     HIP_TRY(e, hipMemcpy(e->comm.p, comm.data(), comm.size() * sizeof(CommDesc), hipMemcpyHostToDevice));
     HIP_TRY(e, hipMemcpy(e->simd.p, e->sim_desc.data(), n_sims * sizeof(SimDesc), hipMemcpyHostToDevice));
     HIP_TRY(e, hipMemcpy(e->csr_thr.p, csr_thr.data(), csr_thr.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(e, hipMemcpy(e->csr_inbox.p, csr_inbox.data(), csr_inbox.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(e, hipMemcpy(e->in_edge.p, in_edge.data(), in_edge.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(e, hipMemcpy(e->in_slot.p, in_slot.data(), in_slot.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(e, hipMemcpy(e->tables.p, table_pool.data(), table_pool.size() * 4, hipMemcpyHostToDevice));
@@ -410,7 +418,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.nld = e->nld.p;
     v.lam = e->lam.p;
     v.series = e->series.p;
-    v.csr_to = nullptr;
+    v.csr_inbox = e->csr_inbox.p;
     v.csr_thr = e->csr_thr.p;
     v.in_edge = e->in_edge.p;
     v.in_slot = e->in_slot.p;

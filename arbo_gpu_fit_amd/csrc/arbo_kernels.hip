@@ -42,9 +42,9 @@ namespace arbo {
 constexpr uint32_t kMaxWords = ARBO_MAX_COMMUNITY_SLOTS / 32;  // alive-bitmap words per community
 constexpr uint32_t kWordsPerLane = kMaxWords / kWave;          // 4
 constexpr uint32_t kUnroll = 4;                                // 16-byte loads in flight per lane
-constexpr uint32_t kChunk = 512;                               // events per claim pass
-constexpr uint32_t kPerLane = kChunk / kWave;                  // 8
-constexpr uint32_t kHash = 1024;                               // claim-table entries (2 x chunk)
+constexpr uint32_t kChunk = 256;                               // events per claim pass
+constexpr uint32_t kPerLane = kChunk / kWave;                  // 4
+constexpr uint32_t kHash = 512;                                // claim-table entries (2 x chunk)
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kKeyBits = 14;                              // member index < 2^14
 constexpr uint32_t kKeyMask = (1u << kKeyBits) - 1u;
@@ -143,7 +143,8 @@ __device__ __forceinline__ uint32_t select_in_word(uint32_t x, uint32_t k) {
 
 // The q-th set bit of a bitmap given its words and their exclusive popcount
 // prefix: word index w and bit position pos.
-__device__ __forceinline__ void select_bit(uint32_t q, const uint32_t* word, const uint32_t* wpre, uint32_t nwords,
+template <class Pre>
+__device__ __forceinline__ void select_bit(uint32_t q, const uint32_t* word, const Pre* wpre, uint32_t nwords,
                                            uint32_t& w, uint32_t& pos) {
     uint32_t lo = 0, hi = nwords - 1;
     while (lo < hi) {
@@ -554,7 +555,11 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
         }
     }
     __syncthreads();
-    for (uint32_t i = lane; i < kCntPerComm; i += kWave) v.cnt[(size_t)g * kCntPerComm + i] = s_hist[i];
+    // push the counts into each destination's inbox: 8 contiguous words per (destination, in-edge)
+    for (uint32_t i = lane; i < 8 * rl; i += kWave) {
+        const uint32_t col = i >> 3, k = i & 7u;
+        v.cnt[(size_t)v.csr_inbox[cd.row_off + col] * 8 + k] = s_hist[k * 9 + col];
+    }
 
     // ---- series row t+1 (datacollect.cu:79-189) ----
     const SeriesIdx si = series_index(sd.series_off, v.max_days, S);
@@ -582,8 +587,8 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
 // quad of consecutive events with one block.  Quads are applied in chunks of
 // kChunkQuads with an LDS claim-min table.
 
-constexpr uint32_t kChunkQuads = kChunk / 4;   // 128 quads = up to 512 events
-constexpr uint32_t kQuadsPerLane = kChunkQuads / kWave;  // 2
+constexpr uint32_t kChunkQuads = kChunk / 4;   // 64 quads = up to 256 events
+constexpr uint32_t kQuadsPerLane = kChunkQuads / kWave;  // 1
 
 struct Quad {
     uint32_t s;       // serotype
@@ -620,14 +625,14 @@ __device__ __forceinline__ Quad decode_quad(uint32_t gq, const uint32_t* seg, co
 
 // The q-th alive slot of the community, from its bitmap words and their
 // exclusive popcount prefix.
-__device__ __forceinline__ uint32_t select_alive(uint32_t q, const uint32_t* word, const uint32_t* wpre,
+__device__ __forceinline__ uint32_t select_alive(uint32_t q, const uint32_t* word, const uint16_t* wpre,
                                                  uint32_t nwords) {
     uint32_t w, pos;
     select_bit(q, word, wpre, nwords, w, pos);
     return 32u * w + pos;
 }
 
-__device__ __forceinline__ uint32_t claim_slot(uint32_t key) { return (key * 2654435761u) >> 22; }
+__device__ __forceinline__ uint32_t claim_slot(uint32_t key) { return (key * 2654435761u) >> 23; }
 
 // Record event `rank` of the chunk on member `key`; the table keeps the
 // smallest rank per member (open addressing, at most kChunk keys in kHash).
@@ -673,7 +678,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     __shared__ uint32_t s_seg[2][4 * 10 + 1];
     __shared__ uint32_t s_qseg[2][4 * 10 + 1];
     __shared__ uint32_t s_word[kMaxWords];
-    __shared__ uint32_t s_wpre[kMaxWords];
+    __shared__ uint16_t s_wpre[kMaxWords];
     __shared__ uint32_t s_tab[kHash];
 
     if (lane < in_len) {
@@ -691,7 +696,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     __syncthreads();
     for (uint32_t i = lane; i < 8 * in_len; i += kWave) {
         const uint32_t k = i / in_len, e = i % in_len;      // k = dir*4 + s
-        const uint32_t n = ((k & 3u) < S) ? v.cnt[(size_t)s_src[e] * kCntPerComm + k * 9 + s_slot[e]] : 0u;
+        const uint32_t n = ((k & 3u) < S) ? v.cnt[((size_t)g * kInboxEdges + e) * 8 + k] : 0u;
         s_ecnt[k][e] = n;
         tot_dir[k >> 2] += n;
     }
@@ -734,7 +739,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
 #pragma unroll
         for (uint32_t k = 0; k < kWordsPerLane; ++k) {
             const uint32_t w = kWordsPerLane * lane + k;
-            if (w < nwords) { s_word[w] = wv[k]; s_wpre[w] = pre; }
+            if (w < nwords) { s_word[w] = wv[k]; s_wpre[w] = (uint16_t)pre; }
             pre += __popc(wv[k]);
         }
         if (Md > 0) {
