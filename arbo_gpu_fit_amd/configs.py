@@ -126,18 +126,22 @@ def reference_params(**over) -> abi.Params:
     return p
 
 
-def config0(seed: int = 1) -> tuple[list[SimSpec], int]:
+def lattice10(seed: int = 1) -> tuple[list[SimSpec], int]:
+    """One simulation on a 10x10 lattice, 1,000 humans per community, 365
+    days."""
     spec = SimSpec(10, 10, np.full(100, 1000, np.uint32), base_params(), seed)
     return [spec], 365
 
 
-def config1(seed: int = 1, grid: int = 100) -> tuple[list[SimSpec], int]:
+def lattice100(seed: int = 1, grid: int = 100) -> tuple[list[SimSpec], int]:
+    """One simulation on a 100x100 lattice, 1,000 humans per community
+    (1e7), Beta(2,5) initial immunity, 3,650 days."""
     spec = SimSpec(grid, grid, np.full(grid * grid, 1000, np.uint32),
                    base_params(immunity_mode=1), seed)
     return [spec], 3650
 
 
-def config2_sim(i: int, seed: int = 2) -> SimSpec:
+def batch_sim(i: int, seed: int = 2) -> SimSpec:
     """Simulation i of the 256-simulation batch."""
     r = philox4x32(i, 1, np.arange(4), 0, seed, 0)
     u = _u53(r[0], r[1])
@@ -158,13 +162,16 @@ def config2_sim(i: int, seed: int = 2) -> SimSpec:
     return SimSpec(g, g, humans, p, (i << 32) | (seed & 0xFFFFFFFF))
 
 
-def config2(n_sims: int = 256, seed: int = 2) -> tuple[list[SimSpec], int]:
-    return [config2_sim(i, seed) for i in range(n_sims)], 1825
+def batch256(n_sims: int = 256, seed: int = 2) -> tuple[list[SimSpec], int]:
+    """The batch of 256 simulations (parameters and sizes drawn per
+    simulation), 1,825 days."""
+    return [batch_sim(i, seed) for i in range(n_sims)], 1825
 
 
 def mcmc_shape(total_humans: int = 1_000_000, communities: int = 1000):
-    """Config 3/4 model shape: communities squared like squareLattice, humans
-    spread as evenly as possible."""
+    """Model shape of the MCMC fit (1e6 humans over 1,000 requested
+    communities): communities squared like squareLattice, humans spread as
+    evenly as possible."""
     C = square_lattice(float(communities))
     g = int(round(C ** 0.5))
     base = total_humans // C
@@ -174,13 +181,9 @@ def mcmc_shape(total_humans: int = 1_000_000, communities: int = 1000):
     return g, humans
 
 
-def config3_spec(theta: dict | None = None, seed: int = 3, **kw) -> SimSpec:
+def mcmc_spec(theta: dict | None = None, seed: int = 3, **kw) -> SimSpec:
     g, humans = mcmc_shape(**kw)
     p = base_params()
     for k, v in (theta or {}).items():
         setattr(p, k, v)
     return SimSpec(g, g, humans, p, seed)
-
-
-def config(n: int):
-    return {0: config0, 1: config1, 2: config2}[n]()

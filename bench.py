@@ -12,12 +12,22 @@ days (W + K = 1,825 by default, the whole configured run).  Inputs are
 synthetic and resident in HBM before timing starts.
 
 The JSON line carries:
-  roofline      the demography kernel against HBM bandwidth: algorithmic bytes
-                per launch (4 B per human word read, 2 B per mosquito word read
-                and 2 B written back) over its mean launch time, measured with
-                HIP events on the engine's stream;
-  cpu_baseline  the CPU oracle (OpenMP) on a bounded sample of the same
-                workload, on this machine's host cores.
+  roofline       the demography kernel (the dominant launch of the day) against
+                 HBM bandwidth.  Algorithmic bytes per launch = 4 B per human
+                 (the packed census word, read) + 4 B per mosquito slot (the
+                 packed 16-bit word, read and written back) + 32 B per
+                 (community, serotype) of counters; `achieved` divides them by
+                 the kernel's mean launch time, measured with HIP events on the
+                 engine's stream.  `traffic` is the kernel's measured HBM bytes
+                 per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated
+                 against a plain copy), read from the committed profile summary
+                 of this same command;
+  step_roofline  the whole day step priced at 7 B per human and per mosquito
+                 slot + 32 B per (community, serotype), the per-person-day
+                 figure of a census that stores an age, over the measured
+                 ms_per_step;
+  cpu_baseline   the CPU oracle (OpenMP, and serial beside it) on a bounded
+                 sample of the same workload, on this machine's host cores.
 """
 from __future__ import annotations
 
@@ -33,33 +43,60 @@ if ROOT not in sys.path:
 
 METRIC = "person-days/sec on 1 MI355X (plain, batched); MCMC ESS/hour plain vs surrogate"
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join("profiles", "k_day_demography_batch256.md")
+
+
+def profiled_traffic() -> float | None:
+    """HBM bytes per launch of the demography kernel from the committed
+    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated)."""
+    try:
+        with open(os.path.join(ROOT, PROFILE)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(specs, days: int, threads: int) -> dict:
-    """The CPU oracle on a bounded sample: the first simulations of the batch
-    for the first `days` days."""
+def _oracle_rate(lib, specs, days: int, threads: int):
     from arbo_gpu_fit_amd.engine import Engine
-    from tests.oracle_lib import load_oracle
 
-    lib = load_oracle()
     eng = Engine(lib, threads=threads)
     eng.init(specs, days)
     t0 = time.perf_counter()
     eng.step(days)
     dt = time.perf_counter() - t0
-    humans = sum(int(s.humans.sum()) for s in specs)
     eng.close()
+    humans = sum(int(s.humans.sum()) for s in specs)
+    return humans * days / dt, humans, dt
+
+
+def cpu_baseline(specs, days: int, threads: int, serial_days: int) -> dict:
+    """The CPU oracle on a bounded sample: the first simulations of the batch
+    for the first `days` days on `threads` OpenMP threads, and simulation 0
+    alone on one thread for `serial_days` days."""
+    from tests.oracle_lib import load_oracle
+
+    lib = load_oracle()
+    rate, humans, dt = _oracle_rate(lib, specs, days, threads)
+    srate, shumans, sdt = _oracle_rate(lib, specs[:1], serial_days, 1)
     return {
-        "value": humans * days / dt,
+        "value": rate,
         "unit": "person-days/s",
         "cores": threads,
         "kind": "port",
         "sample": f"CPU oracle (g++ -O2, OpenMP {threads} threads): simulations 0..{len(specs) - 1} of the "
                   f"batch ({humans} humans), days 0..{days - 1}, {dt:.1f} s",
+        "serial": {
+            "value": srate,
+            "cores": 1,
+            "sample": f"simulation 0 ({shumans} humans), days 0..{serial_days - 1}, one thread, {sdt:.1f} s",
+        },
     }
 
 
@@ -70,8 +107,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=25)
     ap.add_argument("--sims", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sims", type=int, default=4)
-    ap.add_argument("--cpu-days", type=int, default=60)
+    ap.add_argument("--cpu-sims", type=int, default=8)
+    ap.add_argument("--cpu-days", type=int, default=365)
+    ap.add_argument("--cpu-serial-days", type=int, default=365)
     ap.add_argument("--graph", action="store_true", help="replay the day loop as hipGraphs (no per-kernel timing)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this size (counter calibration)")
@@ -99,9 +137,10 @@ def main():
 
     lib = abi.load_hip_library()
     t0 = time.perf_counter()
-    specs, days_cfg = configs.config2(args.sims)
+    specs, days_cfg = configs.batch256(args.sims)
     humans = sum(int(s.humans.sum()) for s in specs)
     comms = sum(s.communities for s in specs)
+    sero_comms = sum(s.communities * s.params.n_serotypes for s in specs)
     log(f"[bench] generated {len(specs)} simulations, {comms} communities, {humans} humans "
         f"in {time.perf_counter() - t0:.1f} s")
     days = args.warmup + args.steps
@@ -163,10 +202,15 @@ def main():
             "graph": bool(args.graph),
         },
     }
+    step_bytes = 7 * (humans + slots) + 32 * sero_comms
+    step_gbs = step_bytes / (elapsed / args.steps) / 1e9
+    out["step_roofline"] = {"bytes_per_step": step_bytes, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS}
     if not args.graph:
+        traffic = profiled_traffic() if args.sims == 256 else None
         ms_a, n_a = eng.kernel_time(0)
         ms_b, n_b = eng.kernel_time(1)
-        bytes_a = 4 * humans + 4 * slots
+        bytes_a = 4 * humans + 4 * slots + 32 * sero_comms
         mean_a = ms_a / max(n_a, 1)
         achieved = bytes_a / (mean_a * 1e-3) / 1e9
         out["roofline"] = {
@@ -176,9 +220,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": bytes_a,
             "mean_launch_ms": mean_a,
+            "traffic_source": PROFILE if traffic is not None else None,
         }
         out["kernels_ms"] = {"k_day_demography": ms_a, "k_day_events": ms_b, "launches": n_a}
     eng.close()
@@ -186,7 +231,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] CPU oracle baseline on {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(specs[:args.cpu_sims], args.cpu_days, threads)
+        out["cpu_baseline"] = cpu_baseline(specs[:args.cpu_sims], args.cpu_days, threads, args.cpu_serial_days)
     print(json.dumps(out), flush=True)
 
 

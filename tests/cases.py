@@ -1,7 +1,7 @@
 """Parity cases shared by the CPU oracle tests and the GPU parity tests.
 
 Each case is a list of simulations and a number of days.  Together they cover
-the configured workload (config 0 in full), the reference's own parameter set
+the 10x10-lattice workload (1e5 humans, 365 days, in full), the reference's own parameter set
 (von Neumann lattice, unit weights, one serotype, bi-Weibull humans,
 Weibull(4, 23 d) mosquitoes, external introductions), isolated and ragged
 communities (1 human; the maximum of 8,192 humans and slots; a community with
@@ -16,8 +16,8 @@ from arbo_gpu_fit_amd import configs
 from arbo_gpu_fit_amd.engine import SimSpec
 
 
-def case_config0():
-    specs, T = configs.config0()
+def case_lattice10():
+    specs, T = configs.lattice10()
     return specs, T
 
 
@@ -50,20 +50,20 @@ def case_high_lambda():
     return [SimSpec(3, 3, np.full(9, 2000, np.uint32), p, 15)], 60
 
 
-def case_config2_pair():
-    sims = [configs.config2_sim(i) for i in range(24)]
+def case_batch_pair():
+    sims = [configs.batch_sim(i) for i in range(24)]
     sims.sort(key=lambda s: int(s.humans.sum()))
     return sims[:2], 40
 
 
 CASES = {
-    "config0": case_config0,
+    "lattice10": case_lattice10,
     "tiny_isolated": case_tiny_isolated,
     "reference_params": case_reference_params,
     "ragged": case_ragged,
     "beta25": case_beta25,
     "high_lambda": case_high_lambda,
-    "config2_pair": case_config2_pair,
+    "batch_pair": case_batch_pair,
 }
 
 # days at which the full census is compared (plus the last day)

@@ -15,8 +15,8 @@ def test_square_lattice_rule():
         assert configs.square_lattice(float(meta)) == want, meta
 
 
-def test_config0_values():
-    specs, T = configs.config0()
+def test_lattice10_values():
+    specs, T = configs.lattice10()
     (s,) = specs
     p = s.params
     assert T == 365 and (s.grid_w, s.grid_h) == (10, 10) and int(s.humans.sum()) == 100_000
@@ -33,15 +33,15 @@ def test_config0_values():
     assert math.isclose(p.seed_frac_h, 1e-3, rel_tol=1e-6) and math.isclose(p.seed_frac_m, 1e-3, rel_tol=1e-6)
 
 
-def test_config1_values():
-    specs, T = configs.config1(grid=100)
+def test_lattice100_values():
+    specs, T = configs.lattice100(grid=100)
     (s,) = specs
     assert T == 3650 and s.communities == 10_000 and int(s.humans.sum()) == 10_000_000
     assert s.params.immunity_mode == 1
 
 
-def test_config2_batch_ranges():
-    specs, T = configs.config2(64)
+def test_batch256_ranges():
+    specs, T = configs.batch256(64)
     assert T == 1825 and len(specs) == 64
     for s in specs:
         C = s.communities
@@ -53,11 +53,11 @@ def test_config2_batch_ranges():
         assert 1e-3 * (1 - 1e-6) <= p.long_distance <= 1e-1 * (1 + 1e-6)
         assert 0 <= p.season_amp <= 0.9 and 0 <= p.season_phase < 2 * math.pi + 1e-6
     # log-uniform community counts: median near sqrt(400 * 10000) = 2000
-    cs = np.array([s.communities for s in configs.config2(256)[0]])
+    cs = np.array([s.communities for s in configs.batch256(256)[0]])
     assert 1000 < np.median(cs) < 4000
     # distinct seeds, reproducible generation
     assert len({s.seed for s in specs}) == len(specs)
-    again = configs.config2_sim(5)
+    again = configs.batch_sim(5)
     assert np.array_equal(again.humans, specs[5].humans) and again.seed == specs[5].seed
 
 

@@ -59,8 +59,8 @@ def test_parity(name, hip_lib, oracle_lib):
 def test_graph_replay_matches_direct(hip_lib, oracle_lib):
     """hipGraph replay of the day loop gives the oracle's census too."""
     from arbo_gpu_fit_amd.abi import F_GRAPH
-    specs, T = CASES["config0"]()
-    print("[parity] graph replay config0", flush=True)
+    specs, T = CASES["lattice10"]()
+    print("[parity] graph replay lattice10", flush=True)
     gpu, orc = _run_pair(specs, T, hip_lib, oracle_lib, flags=F_GRAPH)
     gpu.close()
     orc.close()
@@ -70,7 +70,7 @@ def test_batch_invariance(hip_lib):
     """A simulation's results do not depend on the batch it runs in."""
     a, Ta = CASES["reference_params"]()
     b, Tb = CASES["ragged"]()
-    c, Tc = CASES["config2_pair"]()
+    c, Tc = CASES["batch_pair"]()
     T = 30
     print("[parity] batch invariance", flush=True)
     batch = Engine(hip_lib)

@@ -1,6 +1,6 @@
 """GPU engine at the configured sizes.
 
-* config 1 (100x100 lattice, 1e7 humans): bit-exact against the oracle for
+* the 100x100 lattice (1e7 humans, 3,650 days configured): bit-exact against the oracle for
   the first 30 days;
 * the full 256-simulation batch for all 1,825 days, checked through properties
   that do not need the oracle at that size: constant human counts, mosquito
@@ -25,9 +25,9 @@ pytestmark = pytest.mark.gpu
 
 
 def test_config1_first_month_bit_exact(hip_lib, oracle_lib):
-    specs, _ = configs.config1()
+    specs, _ = configs.lattice100()
     T = 30
-    print("[scale] config1 30 days", flush=True)
+    print("[scale] lattice100 30 days", flush=True)
     gpu = Engine(hip_lib)
     orc = Engine(oracle_lib, threads=16)
     for e in (gpu, orc):
@@ -35,8 +35,8 @@ def test_config1_first_month_bit_exact(hip_lib, oracle_lib):
     for d in (1, T):
         gpu.step(d - gpu.day)
         orc.step(d - orc.day)
-        _compare_state(gpu.state(0), orc.state(0), f"config1 day {d}")
-    _compare_series(gpu.series(0), orc.series(0), "config1")
+        _compare_state(gpu.state(0), orc.state(0), f"lattice100 day {d}")
+    _compare_series(gpu.series(0), orc.series(0), "lattice100")
     gpu.close()
     orc.close()
 
@@ -50,7 +50,7 @@ def _series_digest(eng, n):
 
 
 def test_full_batch_properties(hip_lib):
-    specs, T = configs.config2()
+    specs, T = configs.batch256()
     print(f"[scale] full batch {len(specs)} sims x {T} days", flush=True)
     eng = Engine(hip_lib, flags=abi.F_GRAPH)
     eng.init(specs, T)
