@@ -55,7 +55,7 @@ def check_close_table(got, ref, exact_frac=0.999):
     assert np.all(np.diff(got.astype(np.int64)) >= 0)
 
 
-def test_lifespan_tables_config(oracle_lib):
+def test_lifespan_tables_weibull_exponential(oracle_lib):
     spec = SimSpec(2, 2, np.full(4, 10, np.uint32), configs.base_params(), 1)
     eng = make(oracle_lib, spec)
     p = spec.params
