@@ -34,10 +34,13 @@ namespace {
 
 constexpr uint32_t kGraphDays = 8;
 
+// A device allocation that a later batch reuses when it is large enough, so
+// an MCMC driver re-initialising the engine every step does not reallocate.
 template <class T>
 struct DevBuf {
     T* p = nullptr;
-    size_t n = 0;
+    size_t n = 0;     // elements in use by the current batch
+    size_t cap = 0;   // elements allocated
 };
 
 struct SimHost {
@@ -94,27 +97,38 @@ int fail(ArboEngine* e, int code, const std::string& msg) {
     } while (0)
 
 template <class T>
+void dev_free(DevBuf<T>& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+    b.cap = 0;
+}
+
+template <class T>
 int dev_alloc(ArboEngine* e, DevBuf<T>& b, size_t n) {
-    b.n = n;
-    if (n == 0) { b.p = nullptr; return ARBO_OK; }
+    if (b.p && n <= b.cap) {
+        b.n = n;
+        return ARBO_OK;
+    }
+    dev_free(b);
+    if (n == 0) return ARBO_OK;
     hipError_t st = hipMalloc((void**)&b.p, n * sizeof(T));
     if (st != hipSuccess) {
         b.p = nullptr;
         return fail(e, ARBO_E_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(st));
     }
+    b.n = n;
+    b.cap = n;
     return ARBO_OK;
 }
 
-template <class T>
-void dev_free(DevBuf<T>& b) {
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.n = 0;
+void release_graph(ArboEngine* e) {
+    if (e->graph_exec) { (void)hipGraphExecDestroy(e->graph_exec); e->graph_exec = nullptr; }
+    if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
 }
 
 void free_batch(ArboEngine* e) {
-    if (e->graph_exec) { (void)hipGraphExecDestroy(e->graph_exec); e->graph_exec = nullptr; }
-    if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+    release_graph(e);
     dev_free(e->comm); dev_free(e->simd);
     dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->series); dev_free(e->csr_thr); dev_free(e->csr_inbox);
     dev_free(e->in_edge); dev_free(e->in_slot); dev_free(e->tables); dev_free(e->day_ctr);
@@ -256,7 +270,8 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     }
     HIP_TRY(e, hipSetDevice(e->device));
     HIP_TRY(e, hipStreamSynchronize(e->stream));
-    free_batch(e);
+    release_graph(e);   // the graph captured the old buffers' addresses
+    e->ready = false;
 
     // host tables, built in parallel over simulations
     std::vector<SimTables> tabs(n_sims);

@@ -106,3 +106,26 @@ def test_checkpoint_resume(hip_lib):
         assert np.array_equal(a[k][T // 2 + 1:], b[k][T // 2 + 1:]), k
     full.close()
     resumed.close()
+
+
+def test_reinit_reuses_engine(hip_lib):
+    """One engine re-initialised with a smaller, then a larger batch (its device
+    buffers reused where large enough, the hipGraph recaptured) gives the
+    results of fresh engines."""
+    from arbo_gpu_fit_amd.abi import F_GRAPH
+    big, _ = CASES["batch_pair"]()
+    small, _ = CASES["high_lambda"]()
+    T = 24
+    print("[parity] re-init", flush=True)
+    reused = Engine(hip_lib, flags=F_GRAPH)
+    for specs in (big, small, big):
+        reused.init(specs, T)
+        reused.step(T)
+        fresh = Engine(hip_lib)
+        fresh.init(specs, T)
+        fresh.step(T)
+        for k in range(len(specs)):
+            _compare_state(reused.state(k), fresh.state(k), f"re-init member {k}")
+            _compare_series(reused.series(k), fresh.series(k), f"re-init member {k}")
+        fresh.close()
+    reused.close()

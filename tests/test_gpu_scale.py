@@ -24,7 +24,7 @@ from tests.test_gpu_parity import _compare_series, _compare_state
 pytestmark = pytest.mark.gpu
 
 
-def test_config1_first_month_bit_exact(hip_lib, oracle_lib):
+def test_lattice100_first_month_bit_exact(hip_lib, oracle_lib):
     specs, _ = configs.lattice100()
     T = 30
     print("[scale] lattice100 30 days", flush=True)
