@@ -51,7 +51,7 @@ struct SimDesc {
     uint32_t immune_off;          // [C*S] thresholds
     uint32_t seed_h, seed_m;
     uint32_t death_m_max;         // largest death age the mosquito table can return
-    uint32_t death_m_guide_off;   // [256] smallest death age of each 2^16-wide PLE bucket
+    uint32_t death_m_guide_off;   // [256] smallest death age of each life_bucket() of draws
     uint64_t series_off;          // this simulation's block in the series array
 };
 

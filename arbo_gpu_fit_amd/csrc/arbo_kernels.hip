@@ -42,6 +42,11 @@ namespace arbo {
 constexpr uint32_t kMaxWords = ARBO_MAX_COMMUNITY_SLOTS / 32;  // alive-bitmap words per community
 constexpr uint32_t kWordsPerLane = kMaxWords / kWave;          // 4
 constexpr uint32_t kUnroll = 4;                                // 16-byte loads in flight per lane
+#ifndef ARBO_EARLY_M
+#define ARBO_EARLY_M 4
+#endif
+constexpr uint32_t kEarlyM = ARBO_EARLY_M;                     // mosquito vectors loaded before the humans
+static_assert(kEarlyM >= 1 && kEarlyM <= kUnroll, "early mosquito batch within the first unrolled batch");
 constexpr uint32_t kChunk = 256;                               // events per claim pass
 constexpr uint32_t kPerLane = kChunk / kWave;                  // 4
 constexpr uint32_t kHash = 512;                                // claim-table entries (2 x chunk)
@@ -50,8 +55,9 @@ constexpr uint32_t kKeyBits = 14;                              // member index <
 constexpr uint32_t kKeyMask = (1u << kKeyBits) - 1u;
 constexpr uint32_t kHumanTable = 65536;
 constexpr uint32_t kMaxRowLen = 9;                             // Moore-8 + self
-constexpr uint32_t kGuide = 256;                               // death-age guide buckets (2^16 PLE values each)
+constexpr uint32_t kGuide = 256;                               // death-age guide entries (life_bucket, padded)
 static_assert(kWordsPerLane == 4, "four bitmap words per lane");
+static_assert(kLifeBuckets <= kGuide, "guide table holds every life bucket");
 static_assert(ARBO_MAX_COMMUNITY_SLOTS <= (1u << kKeyBits) && ARBO_MAX_COMMUNITY_HUMANS <= (1u << kKeyBits),
               "member index must fit the claim key");
 
@@ -86,11 +92,11 @@ __device__ __forceinline__ uint32_t death_age(P K, uint32_t hi, uint32_t k) {
     return lo;
 }
 
-// The same minimum, started from the smallest death age of k's 2^16-wide
-// bucket: guide[b] = min{a >= 1 : K[a] >= b 2^16 + 1} <= result.
+// The same minimum, started from the smallest death age of k's bucket:
+// guide[b] = min{a >= 1 : K[a] >= life_bucket_kmin(b)} <= result.
 __device__ __forceinline__ uint32_t death_age_guided(const uint32_t* K, const uint32_t* guide, uint32_t hi,
                                                      uint32_t k) {
-    uint32_t a = guide[(k - 1) >> 16];
+    uint32_t a = guide[life_bucket(k)];
     while (a < hi && K[a] < k) ++a;
     return a;
 }
@@ -245,13 +251,18 @@ struct HumanCount { uint32_t ih01, ih23, inc01, inc23; };
 __device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& cd, const SimDesc& sd, Key key,
                                           uint32_t t, uint32_t q, uint4 w4, HumanCount& hc) {
     const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+    // death day <= t  <=>  word < (t + 1) 2^8 (t < 2^23, so no overflow)
+    const uint32_t tt = (t + 1u) << 8;
+    const uint32_t left = cd.n - 4 * q;                  // >= 1
+    const uint32_t valid = left >= 4 ? 0xFu : (1u << left) - 1u;
     uint32_t dying = 0, need = 0;
 #pragma unroll
     for (uint32_t e = 0; e < 4; ++e) {
-        if (4 * q + e >= cd.n) break;
-        if (t >= (ws[e] >> 8)) dying |= 1u << e;
-        else if (ws[e] & 3u) need |= 1u << e;
+        dying |= (ws[e] < tt ? 1u : 0u) << e;
+        need |= ((ws[e] & 3u) != 0u ? 1u : 0u) << e;
     }
+    dying &= valid;
+    need &= valid & ~dying;
     if (!(dying | need)) return;
     uint32_t nxs[4] = {0, 0, 0, 0};
     if (need) {
@@ -296,14 +307,14 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     const uint32_t mvec = (cd.cap + 7) >> 3;
     const uint4* w4p = reinterpret_cast<const uint4*>(v.h_word + cd.h_off);
     const uint4* m4p = reinterpret_cast<const uint4*>(v.m_word + cd.m_off);
-    uint4 hb[kUnroll], mb[kUnroll];
+    uint4 hb[kUnroll], mb[kEarlyM];
 #pragma unroll
     for (uint32_t u = 0; u < kUnroll; ++u) {
         const uint32_t q = u * kWave + lane;
         if (q < nvec) hb[u] = w4p[q];
     }
 #pragma unroll
-    for (uint32_t u = 0; u < kUnroll; ++u) {
+    for (uint32_t u = 0; u < kEarlyM; ++u) {
         const uint32_t q = u * kWave + lane;
         if (q < mvec) mb[u] = m4p[q];
     }
@@ -315,27 +326,52 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     __shared__ uint32_t s_dead[kMaxWords];                          // dead-slot bitmap
     __shared__ uint32_t s_died[kMaxWords];                          // died today
     __shared__ uint32_t s_dpre[kMaxWords];                          // dead slots before each word
-    __shared__ uint32_t s_mguide[kGuide];                           // death-age guide of the lifetime table
+    __shared__ uint32_t s_mguide[kLifeBuckets];                     // death-age guide of the lifetime table
     __shared__ uint32_t s_row[kMaxRowLen];                          // alias row of the community
     __shared__ uint32_t s_n[8];
     __shared__ uint32_t s_hist[kCntPerComm];
     uint8_t* s_dead8 = reinterpret_cast<uint8_t*>(s_dead);
     uint8_t* s_died8 = reinterpret_cast<uint8_t*>(s_died);
 
-    // tables for the births, kept in registers until the census is scanned
-    constexpr uint32_t kStage = kGuide / kWave;                     // 4
-    uint32_t gstage[kStage];
-#pragma unroll
-    for (uint32_t k = 0; k < kStage; ++k) gstage[k] = v.tables[sd.death_m_guide_off + k * kWave + lane];
-    const uint32_t rowent = lane < cd.row_len ? v.csr_thr[cd.row_off + lane] : 0u;
-    for (uint32_t i = lane; i < kCntPerComm; i += kWave) s_hist[i] = 0;
-    // emergence target of the day (independent of the census): the next
-    // day's expected population E[t+1] (simulation.cpp:129-132)
-    uint32_t target;
+    // Every small operand of the day goes out now, behind the first census
+    // batches, so its latency is spent while the census streams in: the
+    // mosquito lifetime table and its guide (straight to LDS), the alias row
+    // and the inbox rows of the community's CSR entries, the next day's
+    // seasonal density.
     {
-        const U4 r = draw(key, cd.c, t, P_M_TARGET, 0);
-        target = stoch_round(v.dens[sd.dens_off + t + 1] * cd.base, r.x);
+        const uint32_t* dt = v.tables + sd.death_m_off;
+        const uint32_t* gt = v.tables + sd.death_m_guide_off;
+        const uint32_t nd = sd.death_m_max + 1;
+        constexpr uint32_t kT = 4, kG = (kLifeBuckets + kWave - 1) / kWave;
+        uint32_t tv[kT], gv[kG];
+#pragma unroll
+        for (uint32_t k = 0; k < kT; ++k) {
+            const uint32_t a = k * kWave + lane;
+            tv[k] = a < nd ? dt[a] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kG; ++k) {
+            const uint32_t b = k * kWave + lane;
+            gv[k] = b < kLifeBuckets ? gt[b] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kT; ++k) {
+            const uint32_t a = k * kWave + lane;
+            if (a < nd) s_dtab[a] = tv[k];
+        }
+        for (uint32_t a = kT * kWave + lane; a < nd; a += kWave) s_dtab[a] = dt[a];   // long lifetimes
+#pragma unroll
+        for (uint32_t k = 0; k < kG; ++k) {
+            const uint32_t b = k * kWave + lane;
+            if (b < kLifeBuckets) s_mguide[b] = gv[k];
+        }
     }
+    if (lane < cd.row_len) s_row[lane] = v.csr_thr[cd.row_off + lane];
+    const uint32_t inboxent = lane < cd.row_len ? v.csr_inbox[cd.row_off + lane] : 0u;
+    for (uint32_t i = lane; i < kCntPerComm; i += kWave) s_hist[i] = 0;
+    // emergence target of the day: the next day's expected population E[t+1]
+    // (simulation.cpp:129-132), drawn once the census is counted
+    const float dens_next = v.dens[sd.dens_off + t + 1];
 
     // ---- humans ----
     HumanCount hc{0, 0, 0, 0};
@@ -372,7 +408,11 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
             uint4 raw[kUnroll];
             if (q0 == 0) {
 #pragma unroll
-                for (uint32_t u = 0; u < kUnroll; ++u) raw[u] = mb[u];
+                for (uint32_t u = 0; u < kUnroll; ++u) {
+                    const uint32_t q = u * kWave + lane;
+                    if (u < kEarlyM) raw[u] = mb[u < kEarlyM ? u : 0];
+                    else if (q < mvec) raw[u] = m4p[q];
+                }
             } else {
 #pragma unroll
                 for (uint32_t u = 0; u < kUnroll; ++u) {
@@ -439,13 +479,11 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     // ---- emergence into the lowest-index dead slots (mdemographic.cu:104-142) ----
     uint32_t births;
     {
+        const U4 r = draw(key, cd.c, t, P_M_TARGET, 0);
+        const uint32_t target = stoch_round(dens_next * cd.base, r.x);
         const uint32_t b = target > alive_c ? target - alive_c : 0u;
         births = b < dead_c ? b : dead_c;
     }
-    for (uint32_t a = lane; a <= sd.death_m_max; a += kWave) s_dtab[a] = v.tables[sd.death_m_off + a];
-#pragma unroll
-    for (uint32_t k = 0; k < kStage; ++k) s_mguide[k * kWave + lane] = gstage[k];
-    if (lane < cd.row_len) s_row[lane] = rowent;
     __syncthreads();
     uint32_t dw[kWordsPerLane], kw[kWordsPerLane], wpre[kWordsPerLane];
     {
@@ -556,9 +594,10 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     }
     __syncthreads();
     // push the counts into each destination's inbox: 8 contiguous words per (destination, in-edge)
-    for (uint32_t i = lane; i < 8 * rl; i += kWave) {
-        const uint32_t col = i >> 3, k = i & 7u;
-        v.cnt[(size_t)v.csr_inbox[cd.row_off + col] * 8 + k] = s_hist[k * 9 + col];
+    for (uint32_t i0 = 0; i0 < 8 * rl; i0 += kWave) {   // wave-uniform trip count: every lane shuffles
+        const uint32_t i = i0 + lane, col = i >> 3, k = i & 7u;
+        const uint32_t dst = __shfl(inboxent, col < rl ? col : 0u, kWave);
+        if (i < 8 * rl) v.cnt[(size_t)dst * 8 + k] = s_hist[k * 9 + col];
     }
 
     // ---- series row t+1 (datacollect.cu:79-189) ----

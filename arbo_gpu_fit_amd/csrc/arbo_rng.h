@@ -97,6 +97,25 @@ ARBO_HD uint32_t stoch_round(float x, uint32_t r) {
     return (u24f(r) <= frac) ? (uint32_t)ceilf(x) : (uint32_t)fl;
 }
 
+// Guide buckets of a 24-bit lifetime draw k in [1, 2^24]: the survival
+// c = 2^24 - k on a log scale (its leading bit and the three bits after it),
+// so the draws of one bucket have survivals within a factor 9/8 of each other
+// (c < 8 has a bucket per value).  A table search started at the bucket's
+// smallest death age then walks at most a few entries, also in the tail.
+constexpr uint32_t kLifeBuckets = 176;
+ARBO_HD uint32_t life_bucket(uint32_t k) {
+    const uint32_t c = (1u << 24) - k;
+    if (c < 8u) return c;
+    const uint32_t e = 31u - (uint32_t)__builtin_clz(c);  // 3..23
+    return 8u + (e - 3u) * 8u + ((c >> (e - 3u)) & 7u);
+}
+// The smallest draw of bucket b (its largest survival).
+ARBO_HD uint32_t life_bucket_kmin(uint32_t b) {
+    if (b < 8u) return (1u << 24) - b;
+    const uint32_t e = 3u + (b - 8u) / 8u, m = (b - 8u) % 8u;
+    return (1u << 24) - (((9u + m) << (e - 3u)) - 1u);
+}
+
 // 53-bit uniform in [0, 1) from two words.
 ARBO_HD double u53(uint32_t a, uint32_t b) {
     return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * 0x1p-53;

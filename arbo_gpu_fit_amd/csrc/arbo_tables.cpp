@@ -194,13 +194,13 @@ void build_tables(const ArboShape& sh, const ArboParams& p, uint64_t seed, uint3
     threshold_table(kHumanAgeTable, [&](double a) { return human_cdf(p, a); }, t.death_h);
     threshold_table(kMosqAgeTable, [&](double a) { return mosq_cdf(p, a); }, t.death_m);
     // the largest death age the mosquito table returns, and the smallest
-    // death age of every 2^16-wide bucket of 24-bit PLE values
+    // death age of every life_bucket() of 24-bit draws (padding: the largest)
     t.death_m_max = kMosqAgeTable - 1;
     for (uint32_t a = 1; a < kMosqAgeTable; ++a)
         if (t.death_m[a] >= kOne24) { t.death_m_max = a; break; }
-    t.death_m_guide.resize(kGuideBuckets);
-    for (uint32_t b = 0; b < kGuideBuckets; ++b) {
-        const uint32_t k = b * 65536u + 1u;
+    t.death_m_guide.assign(kGuideBuckets, t.death_m_max);
+    for (uint32_t b = 0; b < kLifeBuckets; ++b) {
+        const uint32_t k = life_bucket_kmin(b);
         uint32_t a = 1;
         while (a < t.death_m_max && t.death_m[a] < k) ++a;
         t.death_m_guide[b] = a;

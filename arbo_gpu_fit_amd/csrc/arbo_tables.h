@@ -18,7 +18,7 @@ constexpr uint32_t kHumanYears = 151;       // C_NMAXINITIALAGE + 1 (constant.h:
 constexpr uint32_t kMosqInitDays = 301;     // C_MMAXINITIALAGE + 1 (constant.h:18)
 constexpr uint32_t kOne24 = 1u << 24;
 constexpr uint32_t kMaxRow = 9;             // Moore-8 + self
-constexpr uint32_t kGuideBuckets = 256;     // 2^24 / 2^16
+constexpr uint32_t kGuideBuckets = 256;     // life_bucket() range (176), padded
 
 struct SimTables {
     uint32_t W = 0, H = 0, C = 0, S = 0;
@@ -27,7 +27,7 @@ struct SimTables {
     std::vector<uint32_t> death_h;         // [kHumanAgeTable]  floor(F_h(a) 2^24)
     std::vector<uint32_t> death_m;         // [kMosqAgeTable]   floor(F_m(a) 2^24)
     uint32_t death_m_max = 0;              // first a >= 1 with death_m[a] = 2^24 (else table end)
-    std::vector<uint32_t> death_m_guide;   // [kGuideBuckets] death age of k = b 2^16 + 1
+    std::vector<uint32_t> death_m_guide;   // [kGuideBuckets] death age of life_bucket_kmin(b)
     std::vector<uint32_t> init_h;          // [kHumanYears]     initial-age year thresholds
     std::vector<uint32_t> init_m;          // [kMosqInitDays]   initial-age day thresholds
     std::vector<float> dens;               // [max_days+1] 1 + A sin(2 pi t/365 + phi)
