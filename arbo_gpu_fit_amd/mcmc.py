@@ -1,0 +1,234 @@
+"""ctypes mirror of include/arbo_mcmc.h and a thin Python driver over it.
+
+`Mcmc` wraps one ArboMcmc handle: chains advanced by the C++ driver inside
+libarbo_hip.so, their likelihoods from engine batches (or, for tests, from a
+Python callback).  The history comes back as numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+from .engine import ArboError, Engine
+
+D = 6
+PARAM_NAMES = ("beta_hm", "beta_mh", "log10_long_distance", "season_amp", "season_phase", "reporting")
+PRIOR_UNIFORM, PRIOR_CIRCULAR, PRIOR_BETA = 0, 1, 2
+
+_u32 = C.c_uint32
+_f64p = C.POINTER(C.c_double)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+
+
+class Prior(C.Structure):
+    _fields_ = [("kind", _u32), ("reserved", _u32), ("a", C.c_double), ("b", C.c_double)]
+
+
+class McmcDesc(C.Structure):
+    _fields_ = [
+        ("abi_version", _u32), ("n_chains", _u32), ("n_particles", _u32), ("n_weeks", _u32),
+        ("skip_weeks", _u32), ("adapt_after", _u32),
+        ("nb_size", C.c_double), ("week_floor", C.c_double), ("mix_beta", C.c_double), ("init_scale", C.c_double),
+        ("seed", C.c_uint64), ("priors", Prior * D),
+    ]
+
+
+class EngineApi(C.Structure):
+    _fields_ = [("batch_init", C.c_void_p), ("batch_step", C.c_void_p), ("read_series", C.c_void_p),
+                ("last_error", C.c_void_p)]
+
+
+LOGLIK_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, _u32, _f64p, _f64p)
+McmcPtr = C.c_void_p
+
+PROTOTYPES = {
+    "arbo_mcmc_create": (C.c_int, [C.POINTER(McmcDesc), C.POINTER(abi.Shape), C.POINTER(abi.Params), _u32p, _f64p,
+                                   C.POINTER(McmcPtr)]),
+    "arbo_mcmc_run": (C.c_int, [McmcPtr, abi.EnginePtr, C.POINTER(EngineApi), _u32]),
+    "arbo_mcmc_run_target": (C.c_int, [McmcPtr, LOGLIK_FN, C.c_void_p, _u32]),
+    "arbo_mcmc_iterations": (C.c_int, [McmcPtr, _u32p]),
+    "arbo_mcmc_read": (C.c_int, [McmcPtr, _u32, _u32, _f64p, _f64p, C.POINTER(C.c_uint8)]),
+    "arbo_mcmc_counters": (C.c_int, [McmcPtr, _u64p, _u64p, _u64p, _u64p, _f64p]),
+    "arbo_mcmc_sim_params": (C.c_int, [_f64p, C.POINTER(abi.Params), C.POINTER(abi.Params)]),
+    "arbo_mcmc_log_prior": (C.c_int, [McmcPtr, _f64p, _f64p]),
+    "arbo_mcmc_destroy": (None, [McmcPtr]),
+    "arbo_mcmc_last_error": (C.c_char_p, [McmcPtr]),
+    "arbo_weekly_incidence": (C.c_int, [_u32p, _u32, _u32, _u32, _u32p]),
+    "arbo_nb_loglik": (C.c_int, [_u32p, _u32p, _u32, _u32, C.c_double, C.c_double, C.c_double, _f64p]),
+}
+
+
+def bind(lib: C.CDLL) -> C.CDLL:
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def weekly_incidence(lib, incidence: np.ndarray, n_weeks: int) -> np.ndarray:
+    """MCMC.cpp:199-209 over all serotypes (incidence [rows, S])."""
+    inc = np.ascontiguousarray(incidence, dtype=np.uint32)
+    rows, S = inc.shape
+    out = np.zeros(n_weeks, np.uint32)
+    rc = lib.arbo_weekly_incidence(inc.ctypes.data_as(_u32p), rows, S, n_weeks, out.ctypes.data_as(_u32p))
+    if rc:
+        raise ArboError(rc, "arbo_weekly_incidence")
+    return out
+
+
+def nb_loglik(lib, sim_weekly, data, skip: int, rho: float, r: float, floor: float = -250.0) -> float:
+    sw = np.ascontiguousarray(sim_weekly, dtype=np.uint32)
+    dt = np.ascontiguousarray(data, dtype=np.uint32)
+    out = C.c_double()
+    rc = lib.arbo_nb_loglik(sw.ctypes.data_as(_u32p), dt.ctypes.data_as(_u32p), sw.size, skip, rho, r, floor,
+                            C.byref(out))
+    if rc:
+        raise ArboError(rc, "arbo_nb_loglik")
+    return out.value
+
+
+def engine_api(lib: C.CDLL) -> EngineApi:
+    """The engine entry points of a library exporting include/arbo_c.h."""
+    return EngineApi(*(C.cast(getattr(lib, n), C.c_void_p).value
+                       for n in ("arbo_batch_init", "arbo_batch_step", "arbo_read_series", "arbo_last_error")))
+
+
+@dataclass
+class McmcConfig:
+    n_chains: int
+    n_weeks: int
+    priors: list            # [(kind, a, b)] * 6
+    n_particles: int = 1
+    skip_weeks: int = 1
+    adapt_after: int = 600
+    nb_size: float = 10.0
+    week_floor: float = -250.0
+    mix_beta: float = 0.05
+    init_scale: float = 0.1
+    seed: int = 1
+
+    def desc(self) -> McmcDesc:
+        d = McmcDesc()
+        d.abi_version = abi.ABI_VERSION
+        d.n_chains = self.n_chains
+        d.n_particles = self.n_particles
+        d.n_weeks = self.n_weeks
+        d.skip_weeks = self.skip_weeks
+        d.adapt_after = self.adapt_after
+        d.nb_size = self.nb_size
+        d.week_floor = self.week_floor
+        d.mix_beta = self.mix_beta
+        d.init_scale = self.init_scale
+        d.seed = self.seed & 0xFFFFFFFFFFFFFFFF
+        for i, (k, a, b) in enumerate(self.priors):
+            d.priors[i] = Prior(k, 0, a, b)
+        return d
+
+
+class Mcmc:
+    """Batched adaptive-Metropolis chains (include/arbo_mcmc.h)."""
+
+    def __init__(self, cfg: McmcConfig, grid_w: int, grid_h: int, humans, base: abi.Params, data,
+                 theta0=None, lib: C.CDLL | None = None):
+        self.lib = bind(lib if lib is not None else abi.load_hip_library())
+        self.cfg = cfg
+        self._humans = np.ascontiguousarray(humans, dtype=np.uint32)
+        self._data = np.ascontiguousarray(data, dtype=np.uint32)
+        assert self._data.size == cfg.n_weeks
+        shape = abi.Shape(grid_w, grid_h, self._humans.ctypes.data_as(_u32p), None)
+        self.base = base
+        t0 = None
+        if theta0 is not None:
+            self._theta0 = _f64(theta0).reshape(cfg.n_chains, D)
+            t0 = self._theta0.ctypes.data_as(_f64p)
+        self._h = McmcPtr()
+        desc = cfg.desc()
+        rc = self.lib.arbo_mcmc_create(C.byref(desc), C.byref(shape), C.byref(base),
+                                       self._data.ctypes.data_as(_u32p), t0, C.byref(self._h))
+        if rc:
+            raise ArboError(rc, "arbo_mcmc_create")
+        self._cb = None
+
+    def _check(self, rc):
+        if rc:
+            msg = self.lib.arbo_mcmc_last_error(self._h)
+            raise ArboError(rc, msg.decode() if msg else "")
+
+    def run(self, engine: Engine, n_iters: int, api: EngineApi | None = None):
+        """n_iters iterations with likelihoods from simulations on `engine`
+        (whose library may differ from the driver's: pass engine_api(lib))."""
+        self._check(self.lib.arbo_mcmc_run(self._h, engine._h, C.byref(api) if api is not None else None, n_iters))
+
+    def run_target(self, loglik, n_iters: int):
+        """n_iters iterations with loglik(theta[n, 6]) -> [n] from Python."""
+        def cb(ctx, n, th, out):
+            try:
+                t = np.ctypeslib.as_array(th, shape=(n * D,)).reshape(n, D).copy()
+                v = np.asarray(loglik(t), dtype=np.float64)
+                for i in range(n):
+                    out[i] = float(v[i])
+                return 0
+            except Exception:
+                return -1
+        self._cb = LOGLIK_FN(cb)
+        self._check(self.lib.arbo_mcmc_run_target(self._h, self._cb, None, n_iters))
+
+    @property
+    def iterations(self) -> int:
+        n = C.c_uint32()
+        self._check(self.lib.arbo_mcmc_iterations(self._h, C.byref(n)))
+        return n.value
+
+    def history(self, first: int = 0, count: int | None = None):
+        """(theta [iters, chains, 6], loglik [iters, chains], accepted [iters, chains])."""
+        n = self.iterations - first if count is None else count
+        C_ = self.cfg.n_chains
+        th = np.zeros((n, C_, D))
+        ll = np.zeros((n, C_))
+        acc = np.zeros((n, C_), np.uint8)
+        self._check(self.lib.arbo_mcmc_read(self._h, first, n, th.ctypes.data_as(_f64p), ll.ctypes.data_as(_f64p),
+                                            acc.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return th, ll, acc
+
+    def counters(self) -> dict:
+        p, s, sims, a = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        sec = C.c_double()
+        self._check(self.lib.arbo_mcmc_counters(self._h, C.byref(p), C.byref(s), C.byref(sims), C.byref(a),
+                                                C.byref(sec)))
+        return {"proposals": p.value, "in_support": s.value, "simulations": sims.value, "accepted": a.value,
+                "eval_seconds": sec.value}
+
+    def log_prior(self, theta) -> float:
+        t = _f64(theta)
+        out = C.c_double()
+        rc = self.lib.arbo_mcmc_log_prior(self._h, t.ctypes.data_as(_f64p), C.byref(out))
+        return out.value if rc == 0 else -np.inf
+
+    def close(self):
+        if self._h:
+            self.lib.arbo_mcmc_destroy(self._h)
+            self._h = McmcPtr()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sim_params(lib, theta, base: abi.Params) -> abi.Params:
+    out = abi.Params()
+    t = _f64(theta)
+    rc = bind(lib).arbo_mcmc_sim_params(t.ctypes.data_as(_f64p), C.byref(base), C.byref(out))
+    if rc:
+        raise ArboError(rc, "arbo_mcmc_sim_params")
+    return out

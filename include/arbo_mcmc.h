@@ -1,0 +1,135 @@
+/* arbo_mcmc.h — C ABI of the batched MCMC driver.
+ *
+ * The reference fits its model with one Metropolis chain whose every step
+ * runs C_MAXSIMRUN stochastic simulations of the proposal
+ * (main.cpp:72-349, MCMC.cpp:108-351):
+ *   covariance    MCMC::calcCov         MCMC.cpp:89-105
+ *   propose       MCMC::propose         MCMC.cpp:108-161
+ *   checkParams   MCMC::checkParams     MCMC.cpp:164-186
+ *   storeIncidence (weekly binning)     MCMC.cpp:190-213
+ *   likelihood    negbinomial_pdf, MCMC::calcEachLogLikelihood,
+ *                 MCMC::calcLogLikelihood                MCMC.cpp:216-282
+ *   accept/reject MCMC::nextStep, MCMC::reject           MCMC.cpp:284-380
+ * This driver advances n_chains chains together: the proposals of one
+ * iteration are simulated as one engine batch (include/arbo_c.h), so a step
+ * of all chains costs one pass of the day loop over the batch.
+ *
+ * Parameter vector (ARBO_MCMC_PARAMS = 6), in this order:
+ *   0 beta_hm, 1 beta_mh, 2 log10(long-distance fraction), 3 seasonal
+ *   amplitude, 4 seasonal phase (circular), 5 reporting rate.
+ * Each is mapped onto ArboParams of a base parameter set; the reporting rate
+ * scales the simulated weekly incidence in the likelihood.
+ *
+ * Conventions as include/arbo_c.h: 0 or a negative ARBO_E_* code, message in
+ * arbo_mcmc_last_error(); one driver per thread.
+ */
+#ifndef ARBO_MCMC_H
+#define ARBO_MCMC_H
+#include <stdint.h>
+
+#include "arbo_c.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ARBO_MCMC_PARAMS 6u
+#define ARBO_MCMC_MAX_CHAINS 4096u
+
+enum {
+    ARBO_PRIOR_UNIFORM = 0,   /* U[a, b] */
+    ARBO_PRIOR_CIRCULAR = 1,  /* U[a, b) on a circle: proposals wrap into [a, b) */
+    ARBO_PRIOR_BETA = 2       /* Beta(a, b) on (0, 1) */
+};
+
+typedef struct ArboPrior {
+    uint32_t kind;
+    uint32_t reserved;
+    double a, b;
+} ArboPrior;
+
+typedef struct ArboMcmcDesc {
+    uint32_t abi_version;     /* ARBO_ABI_VERSION */
+    uint32_t n_chains;        /* 1..ARBO_MCMC_MAX_CHAINS */
+    uint32_t n_particles;     /* simulations per proposal, averaged as in MCMC.cpp:252-282 (C_MAXSIMRUN) */
+    uint32_t n_weeks;         /* weekly counts; each simulation runs 7 n_weeks - 1 days (main.cpp:34) */
+    uint32_t skip_weeks;      /* leading weeks left out of the likelihood (MCMC.cpp:238 skips one) */
+    uint32_t adapt_after;     /* pooled chain samples before the adapted covariance is used */
+    double nb_size;           /* negative-binomial size r: variance mu + mu^2 / r */
+    double week_floor;        /* lower clamp of each week's log-likelihood (MCMC.cpp:243: -250) */
+    double mix_beta;          /* probability of the fixed proposal (MCMC.cpp:127: 0.05) */
+    double init_scale;        /* fixed proposal sd per parameter, as a fraction of its prior sd, over sqrt(d) */
+    uint64_t seed;            /* proposals, acceptances and simulation seeds */
+    ArboPrior priors[ARBO_MCMC_PARAMS];
+} ArboMcmcDesc;
+
+/* The engine entry points the driver calls (a caller may point them at any
+ * library exporting include/arbo_c.h). */
+typedef struct ArboEngineApi {
+    int (*batch_init)(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, const ArboParams* params,
+                      const uint64_t* seeds, uint32_t max_days);
+    int (*batch_step)(ArboEngine* e, uint32_t n_days);
+    int (*read_series)(ArboEngine* e, uint32_t sim, ArboSeries* out);
+    const char* (*last_error)(const ArboEngine* e);
+} ArboEngineApi;
+
+/* A caller-supplied log-likelihood of n parameter vectors (theta[n * 6]);
+ * returns 0 or a negative code. */
+typedef int (*ArboLogLikFn)(void* ctx, uint32_t n, const double* theta, double* loglik);
+
+typedef struct ArboMcmc ArboMcmc;
+
+/* Create the driver.  shape: the model's lattice and humans per community
+ * (copied); base: every simulation parameter not in the vector; data:
+ * n_weeks observed weekly counts; theta0: n_chains * 6 starting values, or
+ * NULL for the prior means. */
+ARBO_API int arbo_mcmc_create(const ArboMcmcDesc* desc, const ArboShape* shape, const ArboParams* base,
+                              const uint32_t* data, const double* theta0, ArboMcmc** out);
+
+/* n_iters iterations of every chain, the likelihood from simulations on the
+ * engine (api NULL: this library's engine).  The first call also evaluates
+ * the starting values. */
+ARBO_API int arbo_mcmc_run(ArboMcmc* m, ArboEngine* engine, const ArboEngineApi* api, uint32_t n_iters);
+
+/* The same iterations with a caller-supplied log-likelihood. */
+ARBO_API int arbo_mcmc_run_target(ArboMcmc* m, ArboLogLikFn fn, void* ctx, uint32_t n_iters);
+
+/* Iterations recorded so far (not counting the starting values). */
+ARBO_API int arbo_mcmc_iterations(const ArboMcmc* m, uint32_t* out);
+
+/* History of iterations [first, first + count): theta[count * n_chains * 6]
+ * (state after each iteration), loglik[count * n_chains] (its likelihood
+ * estimate), accepted[count * n_chains]; any output may be NULL. */
+ARBO_API int arbo_mcmc_read(const ArboMcmc* m, uint32_t first, uint32_t count, double* theta, double* loglik,
+                            uint8_t* accepted);
+
+/* Counters: proposals made, proposals inside the prior support, simulations
+ * run, proposals accepted, seconds spent in likelihood evaluation. */
+ARBO_API int arbo_mcmc_counters(const ArboMcmc* m, uint64_t* proposals, uint64_t* in_support,
+                                uint64_t* simulations, uint64_t* accepted, double* eval_seconds);
+
+/* Simulation parameters of one vector: base with the vector's fields set. */
+ARBO_API int arbo_mcmc_sim_params(const double* theta, const ArboParams* base, ArboParams* out);
+
+/* log prior density of one vector (-inf outside the support is reported as
+ * ARBO_E_INVALID). */
+ARBO_API int arbo_mcmc_log_prior(const ArboMcmc* m, const double* theta, double* out);
+
+ARBO_API void arbo_mcmc_destroy(ArboMcmc* m);
+ARBO_API const char* arbo_mcmc_last_error(const ArboMcmc* m);
+
+/* Weekly incidence from a series (MCMC.cpp:199-209): week w sums rows
+ * 7w .. 7w+6 (rows <= rows-1) over all serotypes; incidence is [rows * S]. */
+ARBO_API int arbo_weekly_incidence(const uint32_t* incidence, uint32_t rows, uint32_t serotypes, uint32_t n_weeks,
+                                   uint32_t* out);
+
+/* Negative-binomial log-likelihood of observed weekly counts given simulated
+ * weekly incidence (MCMC.cpp:216-248): weeks skip..n_weeks-1, mean
+ * rho * simulated, size r, each week clamped below at floor (MCMC.cpp:243). */
+ARBO_API int arbo_nb_loglik(const uint32_t* sim_weekly, const uint32_t* data, uint32_t n_weeks, uint32_t skip,
+                            double rho, double r, double floor, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARBO_MCMC_H */

@@ -1,7 +1,7 @@
 """The C ABI boundary (CPU): libarbo_hip.so loads and exports every entry point
-include/arbo_c.h declares (no compute is run), the oracle exports the same
-ABI, and the ctypes mirror matches the C struct layout (sizes and offsets
-printed by a program compiled from the header)."""
+include/*.h declares (no compute is run), the oracle exports the engine ABI
+(include/arbo_c.h), and the ctypes mirrors match the C struct layouts (sizes
+and offsets printed by a program compiled from the headers)."""
 import ctypes as C
 import os
 import re
@@ -9,20 +9,25 @@ import subprocess
 
 import pytest
 
-from arbo_gpu_fit_amd import abi
+from arbo_gpu_fit_amd import abi, mcmc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "arbo_c.h")
+HEADERS = {"arbo_c.h": abi.PROTOTYPES, "arbo_mcmc.h": mcmc.PROTOTYPES}
 
 
-def declared_functions():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"ARBO_API\s+[\w\s\*]+?\b(arbo_\w+)\s*\(", text)))
+def declared_functions(header=None):
+    names = set()
+    for h in ([header] if header else sorted(os.listdir(os.path.join(ROOT, "include")))):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        names |= set(re.findall(r"ARBO_API\s+[\w\s\*]+?\b(arbo_\w+)\s*\(", text))
+    return sorted(names)
 
 
-def test_header_declares_the_abi():
-    names = declared_functions()
-    assert set(names) == set(abi.PROTOTYPES), (set(names) ^ set(abi.PROTOTYPES))
+def test_headers_declare_the_abi():
+    assert sorted(os.listdir(os.path.join(ROOT, "include"))) == sorted(HEADERS)
+    for h, protos in HEADERS.items():
+        names = declared_functions(h)
+        assert set(names) == set(protos), (h, set(names) ^ set(protos))
 
 
 def test_hip_library_exports_every_symbol():
@@ -32,13 +37,14 @@ def test_hip_library_exports_every_symbol():
     for name in declared_functions():
         assert hasattr(lib, name), name
     abi.bind(lib)
+    mcmc.bind(lib)
     out = subprocess.run(["nm", "-D", "--defined-only", abi.HIP_LIB_PATH], capture_output=True, text=True).stdout
     exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
     assert {n for n in exported if n.startswith("arbo_")} == set(declared_functions())
 
 
-def test_oracle_exports_the_same_abi(oracle_lib):
-    for name in declared_functions():
+def test_oracle_exports_the_engine_abi(oracle_lib):
+    for name in declared_functions("arbo_c.h"):
         assert hasattr(oracle_lib, name), name
 
 
@@ -53,10 +59,13 @@ LAYOUT_PROGRAM = r"""
 #include <stddef.h>
 #include <stdio.h>
 #include "arbo_c.h"
+#include "arbo_mcmc.h"
 #define F(T, f) printf("%s.%s %zu\n", #T, #f, offsetof(T, f));
 int main(void) {
     printf("ArboEngineDesc %zu\nArboParams %zu\nArboShape %zu\nArboSeries %zu\nArboState %zu\n",
            sizeof(ArboEngineDesc), sizeof(ArboParams), sizeof(ArboShape), sizeof(ArboSeries), sizeof(ArboState));
+    printf("ArboPrior %zu\nArboMcmcDesc %zu\nArboEngineApi %zu\n", sizeof(ArboPrior), sizeof(ArboMcmcDesc),
+           sizeof(ArboEngineApi));
     FIELDS
     return 0;
 }
@@ -65,7 +74,8 @@ int main(void) {
 
 def test_ctypes_layout_matches_header(tmp_path):
     structs = {"ArboEngineDesc": abi.EngineDesc, "ArboParams": abi.Params, "ArboShape": abi.Shape,
-               "ArboSeries": abi.Series, "ArboState": abi.State}
+               "ArboSeries": abi.Series, "ArboState": abi.State, "ArboPrior": mcmc.Prior,
+               "ArboMcmcDesc": mcmc.McmcDesc, "ArboEngineApi": mcmc.EngineApi}
     fields = "".join(f"F({cname}, {f})" for cname, cls in structs.items() for f, _ in cls._fields_)
     src = tmp_path / "layout.c"
     src.write_text(LAYOUT_PROGRAM.replace("FIELDS", fields))

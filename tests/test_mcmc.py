@@ -1,0 +1,175 @@
+"""The batched MCMC driver (CPU): its negative-binomial likelihood against
+scipy, its weekly binning against the reference's rule, the adaptive
+Metropolis core on targets with known answers (a correlated Gaussian; the
+prior itself under a flat likelihood, which pins the circular wrap and the
+Beta density), and the full driver over the CPU oracle's engine on a tiny
+model (determinism, counters, support)."""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from arbo_gpu_fit_amd import abi, configs, mcmc
+from arbo_gpu_fit_amd.diagnostics import ess, rhat
+from arbo_gpu_fit_amd.engine import ArboError, Engine, SimSpec
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return mcmc.bind(abi.load_hip_library())
+
+
+def test_nb_loglik_matches_scipy(lib):
+    rng = np.random.default_rng(5)
+    r = 10.0
+    sim = rng.integers(0, 500, 40).astype(np.uint32)
+    data = rng.integers(0, 80, 40).astype(np.uint32)
+    rho = 0.13
+    got = mcmc.nb_loglik(lib, sim, data, 1, rho, r, floor=-1e300)
+    mu = rho * sim[1:].astype(float)
+    ref = np.where(mu > 0, stats.nbinom.logpmf(data[1:], r, r / (r + np.where(mu > 0, mu, 1.0))),
+                   np.where(data[1:] == 0, 0.0, -np.inf))
+    ref = np.maximum(ref, -1e300)
+    assert np.isclose(got, ref.sum(), rtol=1e-12, atol=1e-9)
+    # the reference's parameterisation: dispersion 1/r, p = disp mu / (1 + disp mu) (MCMC.cpp:216-243)
+    k, m = 7, 4.5
+    disp = 1.0 / r
+    p = disp * m / (1 + disp * m)
+    ref_gsl = math.lgamma(k + r) - math.lgamma(k + 1) - math.lgamma(r) + k * math.log(p) + r * math.log(1 - p)
+    one = mcmc.nb_loglik(lib, np.array([0, 45], np.uint32), np.array([0, k], np.uint32), 1, 0.1, r)
+    assert np.isclose(one, ref_gsl, rtol=1e-13)
+    # clamp at -250 per week, zero mean with a positive count is clamped, zero with zero is 0
+    z = mcmc.nb_loglik(lib, np.array([9, 0, 0, 100000], np.uint32), np.array([9, 0, 5, 0], np.uint32), 1, 0.5, r)
+    assert np.isclose(z, 0.0 - 250.0 + max(-250.0, stats.nbinom.logpmf(0, r, r / (r + 50000.0))))
+
+
+def test_weekly_binning_reference_rule(lib):
+    rng = np.random.default_rng(3)
+    W = 9
+    rows = 7 * W            # days 0..7W-1
+    inc1 = rng.integers(0, 50, (rows, 1)).astype(np.uint32)
+    got = mcmc.weekly_incidence(lib, inc1, W)
+    max_time = rows - 1
+    ref = []
+    for w in range(W):                      # MCMC.cpp:199-209 with C_STRAINS = 1
+        s = int(inc1[7 * w, 0])
+        for t in range(1, 7):
+            if w * 7 + t <= max_time:
+                s += int(inc1[7 * w + t, 0])
+        ref.append(s)
+    assert list(got) == ref
+    inc4 = rng.integers(0, 50, (rows - 3, 4)).astype(np.uint32)    # short last week
+    got4 = mcmc.weekly_incidence(lib, inc4, W)
+    ref4 = [int(inc4[7 * w:min(7 * w + 7, rows - 3)].sum()) for w in range(W)]
+    assert list(got4) == ref4
+
+
+def _gauss_cfg(n_chains=64, seed=11, **kw):
+    priors = [(0, -20.0, 20.0)] * 6
+    return mcmc.McmcConfig(n_chains=n_chains, n_weeks=3, priors=priors, adapt_after=200, seed=seed, **kw)
+
+
+def _dummy_model(cfg, lib, theta0=None):
+    return mcmc.Mcmc(cfg, 1, 1, [10], configs.base_params(), np.zeros(cfg.n_weeks, np.uint32), theta0=theta0,
+                     lib=lib)
+
+
+def test_gaussian_target_recovery(lib):
+    rng = np.random.default_rng(1)
+    A = rng.normal(size=(6, 6))
+    cov = A @ A.T / 6 + 0.2 * np.eye(6)
+    mean = rng.normal(size=6)
+    prec = np.linalg.inv(cov)
+
+    def loglik(th):
+        d = th - mean
+        return -0.5 * np.einsum("ni,ij,nj->n", d, prec, d)
+
+    cfg = _gauss_cfg()
+    m = _dummy_model(cfg, lib, theta0=np.tile(mean + 1.0, (cfg.n_chains, 1)))
+    m.run_target(loglik, 1500)
+    th, _, acc = m.history()
+    post = th[500:].reshape(-1, 6)
+    se = np.sqrt(np.diag(cov) / np.array([ess(th[500:, :, i]) for i in range(6)]))
+    assert np.all(np.abs(post.mean(axis=0) - mean) < 5 * se)
+    assert np.allclose(np.cov(post.T), cov, atol=0.12 * np.abs(cov).max())
+    assert 0.1 < acc[500:].mean() < 0.5                  # adapted random walk in 6 d (~0.23 optimal)
+    assert all(rhat(th[500:, :, i]) < 1.05 for i in range(6))     # ~50 effective samples per chain
+    c = m.counters()
+    assert c["proposals"] == 1500 * 64 and c["accepted"] == int(acc.sum())
+    m.close()
+
+
+def test_prior_is_sampled_under_flat_likelihood(lib):
+    priors = configs.mcmc_priors()
+    cfg = mcmc.McmcConfig(n_chains=64, n_weeks=3, priors=priors, adapt_after=300, seed=4)
+    m = _dummy_model(cfg, lib)
+    m.run_target(lambda th: np.zeros(len(th)), 1500)
+    th, _, _ = m.history()
+    s = th[300::5].reshape(-1, 6)
+    # every state inside the support; the phase wraps into [0, 2 pi)
+    assert np.all((s[:, 4] >= 0) & (s[:, 4] < 2 * math.pi))
+    assert stats.kstest(s[:, 4], stats.uniform(0, 2 * math.pi).cdf).pvalue > 1e-3
+    assert stats.kstest(s[:, 5], stats.beta(10, 90).cdf).pvalue > 1e-3
+    assert stats.kstest(s[:, 2], stats.uniform(-3, 2).cdf).pvalue > 1e-3
+    for i, (k, a, b) in enumerate(priors):
+        if k == 0:
+            assert s[:, i].min() >= a and s[:, i].max() <= b
+    # log prior of the reference's Beta(10, 90) reporting prior and the uniform box
+    lp = m.log_prior(np.array([0.3, 0.3, -2.0, 0.5, 1.0, 0.1]))
+    ref = (-math.log(0.55) * 2 - math.log(2.0) - math.log(0.9) - math.log(2 * math.pi)
+           + stats.beta(10, 90).logpdf(0.1))
+    assert np.isclose(lp, ref, rtol=1e-12)
+    assert m.log_prior(np.array([0.7, 0.3, -2.0, 0.5, 1.0, 0.1])) == -np.inf
+    m.close()
+
+
+def test_invalid_descriptors(lib):
+    cfg = _gauss_cfg(n_chains=0)
+    with pytest.raises(ArboError):
+        _dummy_model(cfg, lib)
+    bad = _gauss_cfg()
+    bad.priors = list(bad.priors)
+    bad.priors[5] = (2, -1.0, 3.0)            # Beta with a negative shape
+    with pytest.raises(ArboError):
+        _dummy_model(bad, lib)
+    with pytest.raises(ArboError):             # start outside the support
+        _dummy_model(_gauss_cfg(), lib, theta0=np.full((64, 6), 30.0))
+
+
+def _tiny_driver(lib, oracle_lib, seed=9, n_chains=6, n_particles=2):
+    W = 6
+    priors = configs.mcmc_priors()
+    humans = np.full(4, 150, np.uint32)
+    orc = Engine(oracle_lib, threads=2)
+    p = mcmc.sim_params(lib, configs.MCMC_TRUTH, configs.base_params(seed_frac_h=0.02, seed_frac_m=0.02))
+    orc.init([SimSpec(2, 2, humans, p, 5)], 7 * W - 1)
+    orc.step(7 * W - 1)
+    data = mcmc.weekly_incidence(lib, orc.series(0)["n_incidence"], W)
+    cfg = mcmc.McmcConfig(n_chains=n_chains, n_weeks=W, priors=priors, n_particles=n_particles, adapt_after=50,
+                          seed=seed)
+    m = mcmc.Mcmc(cfg, 2, 2, humans, configs.base_params(seed_frac_h=0.02, seed_frac_m=0.02), data, lib=lib)
+    return m, orc
+
+
+def test_driver_over_oracle_engine(lib, oracle_lib):
+    api = mcmc.engine_api(oracle_lib)
+    runs = []
+    for _ in range(2):
+        m, orc = _tiny_driver(lib, oracle_lib)
+        m.run(orc, 3, api=api)
+        runs.append((m.history(), m.counters()))
+        th, ll, acc = runs[-1][0]
+        assert th.shape == (3, 6, 6) and np.all(np.isfinite(ll))
+        for c in range(6):
+            for it in range(3):
+                assert np.isfinite(m.log_prior(th[it, c]))
+        cnt = runs[-1][1]
+        assert cnt["simulations"] == 2 * (6 + cnt["in_support"])      # starting values + proposals, 2 particles
+        assert cnt["proposals"] == 18
+        m.close()
+        orc.close()
+    (a, b) = runs
+    for x, y in zip(a[0], b[0]):
+        assert np.array_equal(x, y)
