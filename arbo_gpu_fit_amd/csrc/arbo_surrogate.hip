@@ -1,0 +1,628 @@
+// arbo_surrogate.hip — the surrogate MLP on gfx950: bfloat16 MFMA matrix
+// products with float32 accumulation, fused bias/GELU epilogues, the Poisson
+// loss, and Adam.
+//
+// One kernel does every matrix product of the forward and backward passes:
+// C[i, j] = sum_k A(i, k) B(j, k) with element strides for both operands, so
+// X W^T (forward), dZ W (input gradients, through a transposed copy of W)
+// and dZ^T X (weight gradients, reducing over the batch) are the same code.
+// A wave computes one 32 x 32 tile with v_mfma_f32_32x32x8_bf16 (8-deep
+// k-steps, float32 accumulators in 16 registers per lane).  The networks
+// here are small (<= 4096 wide, batches of a few thousand rows), so the
+// operands are gathered straight from global memory (L2 resident) and the
+// launch count, not the arithmetic, sets the time.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "arbo_c.h"
+#include "arbo_rng.h"
+#include "arbo_surrogate.h"
+
+namespace arbo {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr uint32_t kTile = 32;
+
+// float32 -> bfloat16, round to nearest even (NaN kept quiet)
+__host__ __device__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float z) {
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+enum : uint32_t {
+    EPI_STORE = 0,       // outf = acc
+    EPI_BIAS = 1,        // outf = acc + bias
+    EPI_BIAS_GELU = 2,   // outf = z = acc + bias; outh = bf16(gelu(z))
+    EPI_GELU_GRAD = 3    // outf = acc * gelu'(zprev); outh = bf16(outf)
+};
+
+struct GemmArgs {
+    uint32_t M, N, K, epi;
+    const uint16_t* A;
+    uint64_t sAi, sAk;
+    const uint16_t* B;
+    uint64_t sBj, sBk;
+    const float* bias;
+    const float* zprev;
+    uint32_t ldz;
+    float* outf;
+    uint32_t ldf;
+    uint16_t* outh;
+    uint32_t ldh;
+};
+
+// Operand layout of v_mfma_f32_32x32x8_bf16: lane l holds A(l % 32, 4 (l / 32) + v)
+// and B(4 (l / 32) + v, l % 32) for v = 0..3; accumulator r of lane l is
+// D(8 (r / 4) + 4 (l / 32) + r % 4, l % 32).
+__global__ __launch_bounds__(64) void k_gemm(GemmArgs g) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i0 = blockIdx.y * kTile, j0 = blockIdx.x * kTile;
+    const uint32_t ia = i0 + (lane & 31u), jb = j0 + (lane & 31u), kq = 4u * (lane >> 5);
+    f32x16 acc = {};
+    for (uint32_t k0 = 0; k0 < g.K; k0 += 8) {
+        bf16x4 a, b;
+#pragma unroll
+        for (uint32_t v = 0; v < 4; ++v) {
+            const uint32_t k = k0 + kq + v;
+            a[v] = (ia < g.M && k < g.K) ? (short)g.A[ia * g.sAi + k * g.sAk] : (short)0;
+            b[v] = (jb < g.N && k < g.K) ? (short)g.B[jb * g.sBj + k * g.sBk] : (short)0;
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    const uint32_t j = jb;
+    if (j >= g.N) return;
+#pragma unroll
+    for (uint32_t r = 0; r < 16; ++r) {
+        const uint32_t i = i0 + 8u * (r >> 2) + 4u * (lane >> 5) + (r & 3u);
+        if (i >= g.M) continue;
+        float v = acc[r];
+        switch (g.epi) {
+            case EPI_STORE:
+                g.outf[(size_t)i * g.ldf + j] = v;
+                break;
+            case EPI_BIAS:
+                g.outf[(size_t)i * g.ldf + j] = v + g.bias[j];
+                break;
+            case EPI_BIAS_GELU: {
+                const float z = v + g.bias[j];
+                g.outf[(size_t)i * g.ldf + j] = z;
+                g.outh[(size_t)i * g.ldh + j] = f2bf(gelu(z));
+                break;
+            }
+            default: {
+                const float d = v * gelu_grad(g.zprev[(size_t)i * g.ldz + j]);
+                g.outf[(size_t)i * g.ldf + j] = d;
+                g.outh[(size_t)i * g.ldh + j] = f2bf(d);
+                break;
+            }
+        }
+    }
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f2bf(x[i]);
+}
+
+// rows idx[r] of x [*, w] to out [r, w]
+__global__ void k_gather(const float* __restrict__ x, const uint32_t* __restrict__ idx, float* __restrict__ out,
+                         uint32_t rows, uint32_t w) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)rows * w) return;
+    const uint32_t r = (uint32_t)(t / w), c = (uint32_t)(t % w);
+    out[t] = x[(uint64_t)idx[r] * w + c];
+}
+
+// Poisson loss terms of the output layer: dF = (exp(f) - y) * scale (float32
+// and bfloat16), loss partial sums per workgroup into loss[blockIdx.x]
+__global__ __launch_bounds__(256) void k_poisson(const float* __restrict__ f, const float* __restrict__ y,
+                                                 uint64_t n, float scale, float* __restrict__ df,
+                                                 uint16_t* __restrict__ dfh, double* __restrict__ part) {
+    __shared__ double s[256];
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (i < n) {
+        const float e = expf(f[i]);
+        const float d = (e - y[i]) * scale;
+        df[i] = d;
+        dfh[i] = f2bf(d);
+        l = (double)e - (double)y[i] * (double)f[i];
+    }
+    s[threadIdx.x] = l;
+    __syncthreads();
+    for (uint32_t o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// db[j] = sum_i dz[i, j]
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ dz, uint32_t rows, uint32_t cols,
+                                                float* __restrict__ db) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cols) return;
+    float s = 0.0f;
+    for (uint32_t i = 0; i < rows; ++i) s += dz[(size_t)i * cols + j];
+    db[j] = s;
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, uint64_t n, float lr, float c1, float c2) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+// bfloat16 copies of W [N, K]: w [N, K] and wt [K, N]
+__global__ void k_pack(const float* __restrict__ W, uint32_t N, uint32_t K, uint16_t* __restrict__ w,
+                       uint16_t* __restrict__ wt) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)N * K) return;
+    const uint32_t n = (uint32_t)(t / K), k = (uint32_t)(t % K);
+    const uint16_t h = f2bf(W[t]);
+    w[t] = h;
+    wt[(size_t)k * N + n] = h;
+}
+
+template <class T>
+struct Buf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+}  // namespace arbo
+
+using namespace arbo;
+
+struct ArboSurrogate {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<uint32_t> w;        // widths
+    uint32_t L = 0, max_batch = 0;
+    std::vector<size_t> woff, boff; // offsets of W_l and b_l in the flat parameter array
+    size_t n_params = 0;
+    Buf<float> P, G, Mo, Vo;        // parameters, gradients, Adam moments
+    std::vector<Buf<uint16_t>> Wh, Wt;
+    std::vector<Buf<float>> Z;      // pre-activations [B, w_{l+1}]
+    std::vector<Buf<uint16_t>> Xh;  // layer inputs (bf16) [B, w_l]
+    std::vector<Buf<float>> dZ;     // float32 gradients [B, w_{l+1}]
+    std::vector<Buf<uint16_t>> dZh; // bfloat16 gradients
+    Buf<float> xin, yin, dxin;
+    Buf<double> part;
+    Buf<float> data_x, data_y, by;
+    Buf<uint32_t> idx;
+    uint64_t adam_t = 0;
+    std::string err;
+};
+
+namespace {
+
+int fail(ArboSurrogate* s, int code, const std::string& msg) {
+    if (s) s->err = msg;
+    return code;
+}
+
+#define SUR_TRY(s, call)                                                                    \
+    do {                                                                                    \
+        hipError_t _st = (call);                                                            \
+        if (_st != hipSuccess)                                                              \
+            return fail((s), ARBO_E_DEVICE, std::string(#call ": ") + hipGetErrorString(_st)); \
+    } while (0)
+
+template <class T>
+int alloc(ArboSurrogate* s, Buf<T>& b, size_t n) {
+    if (b.p && b.n >= n) return ARBO_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+    if (n == 0) return ARBO_OK;
+    if (hipMalloc((void**)&b.p, n * sizeof(T)) != hipSuccess) {
+        b.p = nullptr;
+        return fail(s, ARBO_E_ALLOC, "hipMalloc failed");
+    }
+    b.n = n;
+    return ARBO_OK;
+}
+
+template <class T>
+void release(Buf<T>& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+}
+
+uint32_t blocks(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
+
+hipError_t gemm(ArboSurrogate* s, const GemmArgs& g) {
+    dim3 grid(blocks(g.N, kTile), blocks(g.M, kTile));
+    hipLaunchKernelGGL(k_gemm, grid, dim3(64), 0, s->stream, g);
+    return hipGetLastError();
+}
+
+GemmArgs args(uint32_t M, uint32_t N, uint32_t K, uint32_t epi) {
+    GemmArgs g{};
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.epi = epi;
+    return g;
+}
+
+// Forward pass of B rows already in Xh[0]; the output layer's z lands in Z[L-1].
+int forward(ArboSurrogate* s, uint32_t B) {
+    for (uint32_t l = 0; l < s->L; ++l) {
+        const uint32_t K = s->w[l], N = s->w[l + 1];
+        const bool last = l + 1 == s->L;
+        GemmArgs g = args(B, N, K, last ? EPI_BIAS : EPI_BIAS_GELU);
+        g.A = s->Xh[l].p;
+        g.sAi = K;
+        g.sAk = 1;
+        g.B = s->Wh[l].p;
+        g.sBj = K;
+        g.sBk = 1;
+        g.bias = s->P.p + s->boff[l];
+        g.outf = s->Z[l].p;
+        g.ldf = N;
+        if (!last) {
+            g.outh = s->Xh[l + 1].p;
+            g.ldh = N;
+        }
+        SUR_TRY(s, gemm(s, g));
+    }
+    return ARBO_OK;
+}
+
+// Backward pass from dZ[L-1] / dZh[L-1]: weight gradients into G when
+// want_w, input gradients into dxin when want_x.
+int backward(ArboSurrogate* s, uint32_t B, bool want_w, bool want_x) {
+    for (int l = (int)s->L - 1; l >= 0; --l) {
+        const uint32_t K = s->w[l], N = s->w[l + 1];
+        if (want_w) {
+            GemmArgs g = args(N, K, B, EPI_STORE);    // dW = dZ^T X
+            g.A = s->dZh[l].p;
+            g.sAi = 1;
+            g.sAk = N;
+            g.B = s->Xh[l].p;
+            g.sBj = 1;
+            g.sBk = K;
+            g.outf = s->G.p + s->woff[l];
+            g.ldf = K;
+            SUR_TRY(s, gemm(s, g));
+            hipLaunchKernelGGL(k_colsum, dim3(blocks(N, 256)), dim3(256), 0, s->stream, s->dZ[l].p, B, N,
+                               s->G.p + s->boff[l]);
+            SUR_TRY(s, hipGetLastError());
+        }
+        if (l == 0 && !want_x) break;
+        GemmArgs g = args(B, K, N, l > 0 ? EPI_GELU_GRAD : EPI_STORE);   // dX = dZ W
+        g.A = s->dZh[l].p;
+        g.sAi = N;
+        g.sAk = 1;
+        g.B = s->Wt[l].p;
+        g.sBj = N;
+        g.sBk = 1;
+        if (l > 0) {
+            g.zprev = s->Z[l - 1].p;
+            g.ldz = K;
+            g.outf = s->dZ[l - 1].p;
+            g.ldf = K;
+            g.outh = s->dZh[l - 1].p;
+            g.ldh = K;
+        } else {
+            g.outf = s->dxin.p;
+            g.ldf = K;
+        }
+        SUR_TRY(s, gemm(s, g));
+    }
+    return ARBO_OK;
+}
+
+int pack_weights(ArboSurrogate* s) {
+    for (uint32_t l = 0; l < s->L; ++l) {
+        const uint64_t n = (uint64_t)s->w[l + 1] * s->w[l];
+        hipLaunchKernelGGL(k_pack, dim3(blocks(n, 256)), dim3(256), 0, s->stream, s->P.p + s->woff[l], s->w[l + 1],
+                           s->w[l], s->Wh[l].p, s->Wt[l].p);
+        SUR_TRY(s, hipGetLastError());
+    }
+    return ARBO_OK;
+}
+
+int upload_x(ArboSurrogate* s, uint32_t n, const float* x) {
+    const uint64_t cnt = (uint64_t)n * s->w[0];
+    SUR_TRY(s, hipMemcpyAsync(s->xin.p, x, cnt * 4, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(cnt, 256)), dim3(256), 0, s->stream, s->xin.p, s->Xh[0].p, cnt);
+    SUR_TRY(s, hipGetLastError());
+    return ARBO_OK;
+}
+
+// Poisson loss and output gradient for B rows with targets in `y` (device);
+// the loss sum is returned through *loss (synchronises).
+int poisson(ArboSurrogate* s, uint32_t B, const float* y, float scale, double* loss) {
+    const uint32_t out = s->w[s->L];
+    const uint64_t n = (uint64_t)B * out;
+    const uint32_t nb = blocks(n, 256);
+    if (int rc = alloc(s, s->part, nb)) return rc;
+    hipLaunchKernelGGL(k_poisson, dim3(nb), dim3(256), 0, s->stream, s->Z[s->L - 1].p, y, n, scale,
+                       s->dZ[s->L - 1].p, s->dZh[s->L - 1].p, s->part.p);
+    SUR_TRY(s, hipGetLastError());
+    if (loss) {
+        std::vector<double> h(nb);
+        SUR_TRY(s, hipMemcpyAsync(h.data(), s->part.p, nb * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        SUR_TRY(s, hipStreamSynchronize(s->stream));
+        double t = 0.0;
+        for (double v : h) t += v;
+        *loss = t;
+    }
+    return ARBO_OK;
+}
+
+int adam_step(ArboSurrogate* s, float lr) {
+    s->adam_t++;
+    const float c1 = 1.0f - std::pow(0.9f, (float)s->adam_t), c2 = 1.0f - std::pow(0.999f, (float)s->adam_t);
+    hipLaunchKernelGGL(k_adam, dim3(blocks(s->n_params, 256)), dim3(256), 0, s->stream, s->P.p, s->G.p, s->Mo.p,
+                       s->Vo.p, (uint64_t)s->n_params, lr, c1, c2);
+    SUR_TRY(s, hipGetLastError());
+    return pack_weights(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+ARBO_API int arbo_sur_create(int32_t device, uint32_t n_widths, const uint32_t* widths, uint32_t max_batch,
+                             uint64_t seed, ArboSurrogate** out) {
+    if (!out) return ARBO_E_INVALID;
+    *out = nullptr;
+    if (!widths || n_widths < 2 || n_widths > ARBO_SUR_MAX_LAYERS + 1 || max_batch == 0 || max_batch > (1u << 20))
+        return ARBO_E_INVALID;
+    for (uint32_t i = 0; i < n_widths; ++i)
+        if (widths[i] == 0 || widths[i] > ARBO_SUR_MAX_WIDTH) return ARBO_E_INVALID;
+    ArboSurrogate* s = new (std::nothrow) ArboSurrogate();
+    if (!s) return ARBO_E_ALLOC;
+    s->device = device;
+    s->w.assign(widths, widths + n_widths);
+    s->L = n_widths - 1;
+    s->max_batch = max_batch;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return ARBO_E_DEVICE;
+    }
+    size_t off = 0;
+    for (uint32_t l = 0; l < s->L; ++l) {
+        s->woff.push_back(off);
+        off += (size_t)s->w[l + 1] * s->w[l];
+        s->boff.push_back(off);
+        off += s->w[l + 1];
+    }
+    s->n_params = off;
+    s->Wh.resize(s->L);
+    s->Wt.resize(s->L);
+    s->Z.resize(s->L);
+    s->Xh.resize(s->L);
+    s->dZ.resize(s->L);
+    s->dZh.resize(s->L);
+    int rc = alloc(s, s->P, off);
+    if (!rc) rc = alloc(s, s->G, off);
+    if (!rc) rc = alloc(s, s->Mo, off);
+    if (!rc) rc = alloc(s, s->Vo, off);
+    for (uint32_t l = 0; l < s->L && !rc; ++l) {
+        const size_t wn = (size_t)s->w[l + 1] * s->w[l];
+        rc = alloc(s, s->Wh[l], wn);
+        if (!rc) rc = alloc(s, s->Wt[l], wn);
+        if (!rc) rc = alloc(s, s->Z[l], (size_t)max_batch * s->w[l + 1]);
+        if (!rc) rc = alloc(s, s->Xh[l], (size_t)max_batch * s->w[l]);
+        if (!rc) rc = alloc(s, s->dZ[l], (size_t)max_batch * s->w[l + 1]);
+        if (!rc) rc = alloc(s, s->dZh[l], (size_t)max_batch * s->w[l + 1]);
+    }
+    if (!rc) rc = alloc(s, s->xin, (size_t)max_batch * s->w[0]);
+    if (!rc) rc = alloc(s, s->dxin, (size_t)max_batch * s->w[0]);
+    if (!rc) rc = alloc(s, s->yin, (size_t)max_batch * s->w[s->L]);
+    if (rc) {
+        arbo_sur_destroy(s);
+        return rc;
+    }
+    // He-normal weights from Philox (Box-Muller), zero biases
+    std::vector<float> h(off, 0.0f);
+    const Key key{(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (uint32_t l = 0; l < s->L; ++l) {
+        const double sd = std::sqrt(2.0 / s->w[l]);
+        const size_t n = (size_t)s->w[l + 1] * s->w[l];
+        for (size_t i = 0; i < n; i += 2) {
+            const U4 r = draw(key, l, (uint32_t)(i >> 1), (uint32_t)((uint64_t)i >> 33), 0x5u);
+            const double u1 = 1.0 - u53(r.x, r.y), u2 = u53(r.z, r.w);
+            const double rad = std::sqrt(-2.0 * std::log(u1));
+            h[s->woff[l] + i] = (float)(sd * rad * std::cos(6.283185307179586 * u2));
+            if (i + 1 < n) h[s->woff[l] + i + 1] = (float)(sd * rad * std::sin(6.283185307179586 * u2));
+        }
+    }
+    rc = arbo_sur_set_params(s, h.data());
+    if (rc) {
+        arbo_sur_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_sur_n_params(const ArboSurrogate* s, uint64_t* out) {
+    if (!s || !out) return ARBO_E_INVALID;
+    *out = s->n_params;
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_sur_get_params(ArboSurrogate* s, float* out) {
+    if (!s || !out) return ARBO_E_INVALID;
+    SUR_TRY(s, hipSetDevice(s->device));
+    SUR_TRY(s, hipMemcpyAsync(out, s->P.p, s->n_params * 4, hipMemcpyDeviceToHost, s->stream));
+    SUR_TRY(s, hipStreamSynchronize(s->stream));
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_sur_set_params(ArboSurrogate* s, const float* in) {
+    if (!s || !in) return ARBO_E_INVALID;
+    SUR_TRY(s, hipSetDevice(s->device));
+    SUR_TRY(s, hipMemcpyAsync(s->P.p, in, s->n_params * 4, hipMemcpyHostToDevice, s->stream));
+    SUR_TRY(s, hipMemsetAsync(s->Mo.p, 0, s->n_params * 4, s->stream));
+    SUR_TRY(s, hipMemsetAsync(s->Vo.p, 0, s->n_params * 4, s->stream));
+    s->adam_t = 0;
+    if (int rc = pack_weights(s)) return rc;
+    SUR_TRY(s, hipStreamSynchronize(s->stream));
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_sur_forward(ArboSurrogate* s, uint32_t n, const float* x, float* out) {
+    if (!s || !x || !out || n == 0) return ARBO_E_INVALID;
+    if (n > s->max_batch) return fail(s, ARBO_E_CAPACITY, "more rows than max_batch");
+    SUR_TRY(s, hipSetDevice(s->device));
+    if (int rc = upload_x(s, n, x)) return rc;
+    if (int rc = forward(s, n)) return rc;
+    SUR_TRY(s, hipMemcpyAsync(out, s->Z[s->L - 1].p, (size_t)n * s->w[s->L] * 4, hipMemcpyDeviceToHost, s->stream));
+    SUR_TRY(s, hipStreamSynchronize(s->stream));
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_sur_input_grad(ArboSurrogate* s, uint32_t n, const float* x, const float* dout, float* out,
+                                 float* dx) {
+    if (!s || !x || !dout || !dx || n == 0) return ARBO_E_INVALID;
+    if (n > s->max_batch) return fail(s, ARBO_E_CAPACITY, "more rows than max_batch");
+    SUR_TRY(s, hipSetDevice(s->device));
+    if (int rc = upload_x(s, n, x)) return rc;
+    if (int rc = forward(s, n)) return rc;
+    const uint64_t no = (uint64_t)n * s->w[s->L];
+    SUR_TRY(s, hipMemcpyAsync(s->dZ[s->L - 1].p, dout, no * 4, hipMemcpyHostToDevice, s->stream));
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(no, 256)), dim3(256), 0, s->stream, s->dZ[s->L - 1].p,
+                       s->dZh[s->L - 1].p, no);
+    SUR_TRY(s, hipGetLastError());
+    if (int rc = backward(s, n, false, true)) return rc;
+    if (out) SUR_TRY(s, hipMemcpyAsync(out, s->Z[s->L - 1].p, no * 4, hipMemcpyDeviceToHost, s->stream));
+    SUR_TRY(s, hipMemcpyAsync(dx, s->dxin.p, (size_t)n * s->w[0] * 4, hipMemcpyDeviceToHost, s->stream));
+    SUR_TRY(s, hipStreamSynchronize(s->stream));
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_sur_loss_grad(ArboSurrogate* s, uint32_t n, const float* x, const float* y, double* loss,
+                                float* grad) {
+    if (!s || !x || !y || n == 0) return ARBO_E_INVALID;
+    if (n > s->max_batch) return fail(s, ARBO_E_CAPACITY, "more rows than max_batch");
+    SUR_TRY(s, hipSetDevice(s->device));
+    if (int rc = upload_x(s, n, x)) return rc;
+    const uint64_t no = (uint64_t)n * s->w[s->L];
+    SUR_TRY(s, hipMemcpyAsync(s->yin.p, y, no * 4, hipMemcpyHostToDevice, s->stream));
+    if (int rc = forward(s, n)) return rc;
+    double l = 0.0;
+    if (int rc = poisson(s, n, s->yin.p, 1.0f / (float)no, &l)) return rc;
+    if (loss) *loss = l / (double)no;
+    if (int rc = backward(s, n, true, false)) return rc;
+    if (grad) SUR_TRY(s, hipMemcpyAsync(grad, s->G.p, s->n_params * 4, hipMemcpyDeviceToHost, s->stream));
+    SUR_TRY(s, hipStreamSynchronize(s->stream));
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_sur_train(ArboSurrogate* s, uint32_t n, const float* x, const float* y, uint32_t epochs,
+                            uint32_t batch, float lr, uint64_t seed, double* epoch_loss) {
+    if (!s || !x || !y || n == 0 || batch == 0 || !(lr > 0.0f)) return ARBO_E_INVALID;
+    if (batch > s->max_batch) return fail(s, ARBO_E_CAPACITY, "batch larger than max_batch");
+    SUR_TRY(s, hipSetDevice(s->device));
+    const uint32_t in = s->w[0], out = s->w[s->L];
+    int rc = alloc(s, s->data_x, (size_t)n * in);
+    if (!rc) rc = alloc(s, s->data_y, (size_t)n * out);
+    if (!rc) rc = alloc(s, s->by, (size_t)batch * out);
+    if (!rc) rc = alloc(s, s->idx, n);
+    if (rc) return rc;
+    SUR_TRY(s, hipMemcpyAsync(s->data_x.p, x, (size_t)n * in * 4, hipMemcpyHostToDevice, s->stream));
+    SUR_TRY(s, hipMemcpyAsync(s->data_y.p, y, (size_t)n * out * 4, hipMemcpyHostToDevice, s->stream));
+    const Key key{(uint32_t)seed, (uint32_t)(seed >> 32)};
+    std::vector<uint32_t> perm(n);
+    for (uint32_t e = 0; e < epochs; ++e) {
+        for (uint32_t i = 0; i < n; ++i) perm[i] = i;
+        for (uint32_t i = n - 1; i > 0; --i) {   // Fisher-Yates on Philox draws
+            const U4 r = draw(key, i, e, 0x7u, 0);
+            const uint32_t j = mulhi(r.x, i + 1);
+            const uint32_t t = perm[i];
+            perm[i] = perm[j];
+            perm[j] = t;
+        }
+        SUR_TRY(s, hipStreamSynchronize(s->stream));   // perm is reused below
+        SUR_TRY(s, hipMemcpyAsync(s->idx.p, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s->stream));
+        double tot = 0.0;
+        for (uint32_t b0 = 0; b0 < n; b0 += batch) {
+            const uint32_t B = std::min(batch, n - b0);
+            hipLaunchKernelGGL(k_gather, dim3(blocks((uint64_t)B * in, 256)), dim3(256), 0, s->stream, s->data_x.p,
+                               s->idx.p + b0, s->xin.p, B, in);
+            hipLaunchKernelGGL(k_gather, dim3(blocks((uint64_t)B * out, 256)), dim3(256), 0, s->stream, s->data_y.p,
+                               s->idx.p + b0, s->by.p, B, out);
+            hipLaunchKernelGGL(k_to_bf16, dim3(blocks((uint64_t)B * in, 256)), dim3(256), 0, s->stream, s->xin.p,
+                               s->Xh[0].p, (uint64_t)B * in);
+            SUR_TRY(s, hipGetLastError());
+            if ((rc = forward(s, B))) return rc;
+            double l = 0.0;
+            const bool want = epoch_loss != nullptr && b0 + batch >= n;   // one sync per epoch
+            if ((rc = poisson(s, B, s->by.p, 1.0f / ((float)B * out), want ? &l : nullptr))) return rc;
+            if (want) tot = l / ((double)B * out);
+            if ((rc = backward(s, B, true, false))) return rc;
+            if ((rc = adam_step(s, lr))) return rc;
+        }
+        if (epoch_loss) epoch_loss[e] = tot;
+    }
+    SUR_TRY(s, hipStreamSynchronize(s->stream));
+    return ARBO_OK;
+}
+
+ARBO_API void arbo_sur_destroy(ArboSurrogate* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    release(s->P);
+    release(s->G);
+    release(s->Mo);
+    release(s->Vo);
+    for (auto& b : s->Wh) release(b);
+    for (auto& b : s->Wt) release(b);
+    for (auto& b : s->Z) release(b);
+    for (auto& b : s->Xh) release(b);
+    for (auto& b : s->dZ) release(b);
+    for (auto& b : s->dZh) release(b);
+    release(s->xin);
+    release(s->yin);
+    release(s->dxin);
+    release(s->part);
+    release(s->data_x);
+    release(s->data_y);
+    release(s->by);
+    release(s->idx);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+ARBO_API const char* arbo_sur_last_error(const ArboSurrogate* s) { return s ? s->err.c_str() : "null surrogate"; }
+
+}  // extern "C"
