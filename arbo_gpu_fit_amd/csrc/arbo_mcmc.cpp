@@ -118,6 +118,10 @@ struct ArboMcmc {
     // counters
     uint64_t n_prop = 0, n_support = 0, n_sims = 0, n_acc = 0;
     double eval_s = 0.0;
+    // delayed acceptance
+    uint64_t da_steps = 0, da_acc = 0;
+    double sur_s = 0.0, da_log_eps = 0.0, da_t = 0.0;
+    bool da_init = false;
     std::string err;
 };
 
@@ -466,6 +470,372 @@ ARBO_API int arbo_nb_loglik(const uint32_t* sim_weekly, const uint32_t* data, ui
                             double rho, double r, double floor_v, double* out) {
     if (!sim_weekly || !data || !out || !(r > 0.0) || !(rho >= 0.0)) return ARBO_E_INVALID;
     *out = nb_loglik(sim_weekly, data, n_weeks, skip, rho, r, floor_v);
+    return ARBO_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Delayed acceptance with the surrogate (see include/arbo_mcmc.h).
+
+namespace {
+
+enum : uint32_t { Q_MALA_NORMAL = 5, Q_MALA_ACCEPT = 6 };
+
+void features_of(const ArboMcmc* m, const double* th, float* x) {
+    for (uint32_t i = 0; i < 4; ++i) {
+        const ArboPrior& p = m->desc.priors[i];
+        x[i] = (float)((th[i] - 0.5 * (p.a + p.b)) / (0.5 * (p.b - p.a)));
+    }
+    x[4] = (float)std::cos(th[4]);
+    x[5] = (float)std::sin(th[4]);
+}
+
+// log prior gradient (zero for the uniform and circular priors)
+double prior_grad(const ArboPrior& p, double x) {
+    if (p.kind == ARBO_PRIOR_BETA) return (p.a - 1.0) / x - (p.b - 1.0) / (1.0 - x);
+    return 0.0;
+}
+
+// Surrogate log posterior and gradient of n vectors.
+int sur_logpost(ArboMcmc* m, ArboSurrogate* sur, uint32_t n, const double* th, double* lp, double* grad) {
+    if (n == 0) return ARBO_OK;
+    const uint32_t W = m->desc.n_weeks;
+    const double r = m->desc.nb_size;
+    std::vector<float> x((size_t)n * D), f((size_t)n * W), dout((size_t)n * W, 0.0f), dx((size_t)n * D);
+    std::vector<double> drho(n, 0.0);
+    for (uint32_t i = 0; i < n; ++i) features_of(m, th + (size_t)i * D, &x[(size_t)i * D]);
+    int rc = arbo_sur_forward(sur, n, x.data(), f.data());
+    if (rc != ARBO_OK) return fail(m, rc, std::string("surrogate: ") + arbo_sur_last_error(sur));
+    for (uint32_t i = 0; i < n; ++i) {
+        const double rho = th[(size_t)i * D + 5];
+        double ll = 0.0;
+        for (uint32_t w = m->desc.skip_weeks; w < W; ++w) {
+            const double mu = rho * std::exp((double)f[(size_t)i * W + w]);
+            const uint32_t k = m->data[w];
+            const double v = nb_week(k, mu, r);
+            if (v > m->desc.week_floor) {
+                ll += v;
+                const double d = (double)k - ((double)k + r) * mu / (r + mu);   // d ll / d log mu
+                dout[(size_t)i * W + w] = (float)d;
+                drho[i] += d / rho;
+            } else {
+                ll += m->desc.week_floor;
+            }
+        }
+        lp[i] = ll + log_prior(m, th + (size_t)i * D);
+    }
+    if (grad) {
+        rc = arbo_sur_input_grad(sur, n, x.data(), dout.data(), nullptr, dx.data());
+        if (rc != ARBO_OK) return fail(m, rc, std::string("surrogate: ") + arbo_sur_last_error(sur));
+        for (uint32_t i = 0; i < n; ++i) {
+            const double* t = th + (size_t)i * D;
+            double* g = grad + (size_t)i * D;
+            const float* d = &dx[(size_t)i * D];
+            for (uint32_t k = 0; k < 4; ++k) {
+                const ArboPrior& p = m->desc.priors[k];
+                g[k] = (double)d[k] / (0.5 * (p.b - p.a)) + prior_grad(p, t[k]);
+            }
+            g[4] = -std::sin(t[4]) * (double)d[4] + std::cos(t[4]) * (double)d[5] + prior_grad(m->desc.priors[4], t[4]);
+            g[5] = drho[i] + prior_grad(m->desc.priors[5], t[5]);
+        }
+    }
+    return ARBO_OK;
+}
+
+// solves L y = v (lower triangular)
+void lower_solve(const double* L, const double* v, double* y) {
+    for (uint32_t i = 0; i < D; ++i) {
+        double s = v[i];
+        for (uint32_t k = 0; k < i; ++k) s -= L[i * D + k] * y[k];
+        y[i] = s / L[i * D + i];
+    }
+}
+
+// -|L^-1 (a - mean)|^2 / (2 eps^2), the circular coordinate by its shortest arc
+double log_q(const ArboMcmc* m, const double* L, double eps, const double* a, const double* mean) {
+    double d[D], y[D];
+    for (uint32_t i = 0; i < D; ++i) {
+        d[i] = a[i] - mean[i];
+        const ArboPrior& p = m->desc.priors[i];
+        if (p.kind == ARBO_PRIOR_CIRCULAR) {
+            const double w = p.b - p.a;
+            d[i] -= w * std::nearbyint(d[i] / w);
+        }
+    }
+    lower_solve(L, d, y);
+    double s = 0.0;
+    for (uint32_t i = 0; i < D; ++i) s += y[i] * y[i];
+    return -s / (2.0 * eps * eps);
+}
+
+void wrap(const ArboMcmc* m, double* th) {
+    for (uint32_t i = 0; i < D; ++i) {
+        const ArboPrior& p = m->desc.priors[i];
+        if (p.kind != ARBO_PRIOR_CIRCULAR) continue;
+        const double w = p.b - p.a;
+        double x = std::fmod(th[i] - p.a, w);
+        if (x < 0.0) x += w;
+        th[i] = p.a + x;
+        if (th[i] >= p.b) th[i] = p.a;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+ARBO_API int arbo_mcmc_features(const ArboMcmc* m, uint32_t n, const double* theta, float* x) {
+    if (!m || !theta || !x) return ARBO_E_INVALID;
+    for (uint32_t i = 0; i < n; ++i) features_of(m, theta + (size_t)i * D, x + (size_t)i * D);
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_mcmc_surrogate_logpost(ArboMcmc* m, ArboSurrogate* sur, uint32_t n, const double* theta,
+                                         double* out, double* grad) {
+    if (!m || !sur || !theta || !out) return ARBO_E_INVALID;
+    uint32_t in = 0, outw = 0;
+    if (arbo_sur_shape(sur, &in, &outw) != ARBO_OK || in != D || outw != m->desc.n_weeks)
+        return fail(m, ARBO_E_INVALID, "surrogate shape does not match (6 inputs, n_weeks outputs)");
+    return sur_logpost(m, sur, n, theta, out, grad);
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_da(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da) {
+    if (!sur || !da) return fail(m, ARBO_E_INVALID, "surrogate and descriptor are required");
+    if (da->sub_steps == 0 || da->sub_steps > 1000 || !(da->step0 > 0.0) ||
+        !(da->target_accept > 0.0 && da->target_accept < 1.0))
+        return fail(m, ARBO_E_INVALID, "bad delayed-acceptance descriptor");
+    uint32_t in = 0, outw = 0;
+    if (arbo_sur_shape(sur, &in, &outw) != ARBO_OK || in != D || outw != m->desc.n_weeks)
+        return fail(m, ARBO_E_INVALID, "surrogate shape does not match (6 inputs, n_weeks outputs)");
+    return ARBO_OK;
+}
+
+// Delayed-acceptance iterations; `eval` gives the second stage's
+// log-likelihoods (n vectors, their chains, the iteration).
+template <class Eval>
+int run_da_impl(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da, uint32_t n_iters, Eval eval) {
+    using clk = std::chrono::steady_clock;
+    const uint32_t C = m->C;
+    auto ibm = [&](uint32_t n, const double* th, const uint32_t* chains, uint32_t it, double* ll) {
+        if (n == 0) return (int)ARBO_OK;
+        const auto t0 = clk::now();
+        const int rc = eval(n, th, chains, it, ll);
+        m->eval_s += std::chrono::duration<double>(clk::now() - t0).count();
+        return rc;
+    };
+    std::vector<uint32_t> all(C);
+    for (uint32_t c = 0; c < C; ++c) all[c] = c;
+    if (!m->started) {
+        int rc = ibm(C, m->theta.data(), all.data(), kInitIter, m->loglik.data());
+        if (rc != ARBO_OK) return rc;
+        for (uint32_t c = 0; c < C; ++c) {
+            m->logprior[c] = log_prior(m, &m->theta[(size_t)c * D]);
+            pool_add(m, &m->theta[(size_t)c * D]);
+        }
+        m->started = true;
+    }
+    if (!m->da_init) {
+        m->da_log_eps = std::log(da->step0);
+        m->da_init = true;
+    }
+    // surrogate log posterior and gradient at the chains' states
+    std::vector<double> s_lp(C), s_g((size_t)C * D);
+    {
+        const auto t0 = clk::now();
+        int rc = sur_logpost(m, sur, C, m->theta.data(), s_lp.data(), s_g.data());
+        m->sur_s += std::chrono::duration<double>(clk::now() - t0).count();
+        if (rc != ARBO_OK) return rc;
+    }
+    std::vector<double> cur((size_t)C * D), cur_lp(C), cur_g((size_t)C * D), prop((size_t)C * D), p_lp(C),
+        p_g((size_t)C * D), sub((size_t)C * D), sub_ll(C);
+    std::vector<uint32_t> idx(C), sub_chain(C);
+    std::vector<uint8_t> moved(C), acc(C);
+    for (uint32_t step = 0; step < n_iters; ++step) {
+        const uint32_t it = m->iter;
+        // preconditioner: the pooled covariance once adapted, else the fixed proposal's
+        double L[D * D];
+        bool adapted = m->n_pool >= (double)m->desc.adapt_after && m->n_pool > 1.0;
+        if (adapted) {
+            double S[D * D];
+            for (uint32_t i = 0; i < D; ++i)
+                for (uint32_t j = 0; j < D; ++j) {
+                    S[i * D + j] = m->m2[i * D + j] / (m->n_pool - 1.0);
+                    if (i == j) S[i * D + j] += 1e-12 + 1e-10 * m->sd0[i] * m->sd0[i];
+                }
+            adapted = cholesky(S, L);
+        }
+        if (!adapted) {
+            std::memset(L, 0, sizeof L);
+            for (uint32_t i = 0; i < D; ++i) L[i * D + i] = m->sd0[i] * std::sqrt((double)D);
+        }
+        cur = m->theta;
+        cur_lp = s_lp;
+        cur_g = s_g;
+        std::fill(moved.begin(), moved.end(), 0);
+        for (uint32_t j = 0; j < da->sub_steps; ++j) {
+            const double eps = std::exp(m->da_log_eps);
+            const uint32_t sj = it * da->sub_steps + j;
+            uint32_t nv = 0;
+            for (uint32_t c = 0; c < C; ++c) {
+                const double* th = &cur[(size_t)c * D];
+                const double* g = &cur_g[(size_t)c * D];
+                double mg[D], z[D], mean[D];
+                // M g = L (L^T g)
+                double ltg[D];
+                for (uint32_t i = 0; i < D; ++i) {
+                    double s = 0.0;
+                    for (uint32_t k = i; k < D; ++k) s += L[k * D + i] * g[k];
+                    ltg[i] = s;
+                }
+                for (uint32_t i = 0; i < D; ++i) {
+                    double s = 0.0;
+                    for (uint32_t k = 0; k <= i; ++k) s += L[i * D + k] * ltg[k];
+                    mg[i] = s;
+                }
+                for (uint32_t k = 0; k < D / 2; ++k) {
+                    const U4 r = draw(m->key, c, sj, Q_MALA_NORMAL, k);
+                    const double u1 = 1.0 - u01(r.x, r.y), u2 = u01(r.z, r.w);
+                    const double rad = std::sqrt(-2.0 * std::log(u1));
+                    z[2 * k] = rad * std::cos(kTwoPi * u2);
+                    z[2 * k + 1] = rad * std::sin(kTwoPi * u2);
+                }
+                double* out = &prop[(size_t)c * D];
+                for (uint32_t i = 0; i < D; ++i) {
+                    mean[i] = th[i] + 0.5 * eps * eps * mg[i];
+                    double s = 0.0;
+                    for (uint32_t k = 0; k <= i; ++k) s += L[i * D + k] * z[k];
+                    out[i] = mean[i] + eps * s;
+                }
+                wrap(m, out);
+                if (std::isfinite(log_prior(m, out))) {
+                    std::memcpy(&sub[(size_t)nv * D], out, sizeof(double) * D);
+                    idx[nv++] = c;
+                }
+            }
+            const auto t0 = clk::now();
+            int rc = sur_logpost(m, sur, nv, sub.data(), sub_ll.data(), p_g.data());
+            m->sur_s += std::chrono::duration<double>(clk::now() - t0).count();
+            if (rc != ARBO_OK) return rc;
+            uint32_t n_acc = 0;
+            for (uint32_t k = 0; k < nv; ++k) {
+                const uint32_t c = idx[k];
+                const double* th = &cur[(size_t)c * D];
+                const double* tp = &sub[(size_t)k * D];
+                const double* gp = &p_g[(size_t)k * D];
+                const double* g = &cur_g[(size_t)c * D];
+                // forward and backward Langevin means
+                double mf[D], mb[D], ltg[D], mg[D];
+                for (int pass = 0; pass < 2; ++pass) {
+                    const double* gg = pass == 0 ? g : gp;
+                    const double* base = pass == 0 ? th : tp;
+                    double* mm = pass == 0 ? mf : mb;
+                    for (uint32_t i = 0; i < D; ++i) {
+                        double s = 0.0;
+                        for (uint32_t q = i; q < D; ++q) s += L[q * D + i] * gg[q];
+                        ltg[i] = s;
+                    }
+                    for (uint32_t i = 0; i < D; ++i) {
+                        double s = 0.0;
+                        for (uint32_t q = 0; q <= i; ++q) s += L[i * D + q] * ltg[q];
+                        mg[i] = s;
+                        mm[i] = base[i] + 0.5 * eps * eps * mg[i];
+                    }
+                }
+                const double log_a = (sub_ll[k] - cur_lp[c]) + log_q(m, L, eps, th, mb) - log_q(m, L, eps, tp, mf);
+                const U4 r = draw(m->key, c, sj, Q_MALA_ACCEPT, 0);
+                if (std::log(1.0 - u01(r.x, r.y)) < log_a) {
+                    std::memcpy(&cur[(size_t)c * D], tp, sizeof(double) * D);
+                    cur_lp[c] = sub_ll[k];
+                    std::memcpy(&cur_g[(size_t)c * D], gp, sizeof(double) * D);
+                    moved[c] = 1;
+                    ++n_acc;
+                }
+            }
+            m->da_steps += C;
+            m->da_acc += n_acc;
+            // step size: diminishing Robbins-Monro on the surrogate acceptance
+            m->da_t += 1.0;
+            m->da_log_eps += std::pow(m->da_t + 10.0, -0.6) * ((double)n_acc / (double)C - da->target_accept);
+        }
+        // second stage: the simulator at the chains that moved
+        uint32_t ns = 0;
+        for (uint32_t c = 0; c < C; ++c) {
+            acc[c] = 0;
+            if (!moved[c]) continue;
+            std::memcpy(&sub[(size_t)ns * D], &cur[(size_t)c * D], sizeof(double) * D);
+            sub_chain[ns++] = c;
+        }
+        m->n_prop += C;
+        m->n_support += ns;
+        int rc = ibm(ns, sub.data(), sub_chain.data(), it, sub_ll.data());
+        if (rc != ARBO_OK) return rc;
+        for (uint32_t k = 0; k < ns; ++k) {
+            const uint32_t c = sub_chain[k];
+            const double* tp = &cur[(size_t)c * D];
+            const double lp_new = log_prior(m, tp);
+            // L(theta') L_s(theta) / (L(theta) L_s(theta')); the surrogate terms carry the prior
+            const double log_a = (sub_ll[k] + lp_new - cur_lp[c]) - (m->loglik[c] + m->logprior[c] - s_lp[c]);
+            const U4 r = draw(m->key, c, it, Q_ACCEPT, 0);
+            if (std::log(1.0 - u01(r.x, r.y)) < log_a) {
+                std::memcpy(&m->theta[(size_t)c * D], tp, sizeof(double) * D);
+                m->loglik[c] = sub_ll[k];
+                m->logprior[c] = lp_new;
+                s_lp[c] = cur_lp[c];
+                std::memcpy(&s_g[(size_t)c * D], &cur_g[(size_t)c * D], sizeof(double) * D);
+                acc[c] = 1;
+                m->n_acc++;
+            }
+        }
+        for (uint32_t c = 0; c < C; ++c) pool_add(m, &m->theta[(size_t)c * D]);
+        m->h_theta.insert(m->h_theta.end(), m->theta.begin(), m->theta.end());
+        m->h_loglik.insert(m->h_loglik.end(), m->loglik.begin(), m->loglik.end());
+        m->h_acc.insert(m->h_acc.end(), acc.begin(), acc.end());
+        m->iter++;
+    }
+    return ARBO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ARBO_API int arbo_mcmc_run_da(ArboMcmc* m, ArboEngine* engine, const ArboEngineApi* api, ArboSurrogate* sur,
+                              const ArboDaDesc* da, uint32_t n_iters) {
+    if (!m) return ARBO_E_INVALID;
+    if (!engine) return fail(m, ARBO_E_INVALID, "no engine");
+    if (int rc = check_da(m, sur, da)) return rc;
+    ArboEngineApi own{arbo_batch_init, arbo_batch_step, arbo_read_series, arbo_last_error};
+    const ArboEngineApi a = api ? *api : own;
+    if (!a.batch_init || !a.batch_step || !a.read_series) return fail(m, ARBO_E_INVALID, "incomplete engine API");
+    return run_da_impl(m, sur, da, n_iters,
+                       [&](uint32_t n, const double* th, const uint32_t* chains, uint32_t it, double* ll) {
+                           return eval_engine(m, engine, a, n, th, chains, it, ll);
+                       });
+}
+
+ARBO_API int arbo_mcmc_run_da_target(ArboMcmc* m, ArboLogLikFn fn, void* ctx, ArboSurrogate* sur,
+                                     const ArboDaDesc* da, uint32_t n_iters) {
+    if (!m) return ARBO_E_INVALID;
+    if (!fn) return fail(m, ARBO_E_INVALID, "no log-likelihood");
+    if (int rc = check_da(m, sur, da)) return rc;
+    return run_da_impl(m, sur, da, n_iters, [&](uint32_t n, const double* th, const uint32_t*, uint32_t, double* ll) {
+        const int rc = fn(ctx, n, th, ll);
+        return rc == ARBO_OK ? (int)ARBO_OK : fail(m, rc, "log-likelihood callback failed");
+    });
+}
+
+ARBO_API int arbo_mcmc_da_counters(const ArboMcmc* m, uint64_t* steps, uint64_t* accepted, double* seconds,
+                                   double* step) {
+    if (!m) return ARBO_E_INVALID;
+    if (steps) *steps = m->da_steps;
+    if (accepted) *accepted = m->da_acc;
+    if (seconds) *seconds = m->sur_s;
+    if (step) *step = std::exp(m->da_log_eps);
     return ARBO_OK;
 }
 

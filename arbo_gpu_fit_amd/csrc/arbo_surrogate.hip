@@ -479,6 +479,13 @@ ARBO_API int arbo_sur_n_params(const ArboSurrogate* s, uint64_t* out) {
     return ARBO_OK;
 }
 
+ARBO_API int arbo_sur_shape(const ArboSurrogate* s, uint32_t* inputs, uint32_t* outputs) {
+    if (!s || !inputs || !outputs) return ARBO_E_INVALID;
+    *inputs = s->w.front();
+    *outputs = s->w.back();
+    return ARBO_OK;
+}
+
 ARBO_API int arbo_sur_get_params(ArboSurrogate* s, float* out) {
     if (!s || !out) return ARBO_E_INVALID;
     SUR_TRY(s, hipSetDevice(s->device));

@@ -1,0 +1,125 @@
+"""The fitting workflow around the engine: training data for the surrogate
+from simulator batches, surrogate training, and the plain and
+delayed-acceptance MCMC fits with their efficiency summaries.
+
+All heavy work runs in the C++/HIP library: simulations on the engine,
+the MLP's matrix products on the MFMA units, the chains in the C++ driver;
+this module sequences them and summarises the output."""
+from __future__ import annotations
+
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import abi, configs, mcmc
+from .diagnostics import circular_unwrap, ess, rhat
+from .engine import Engine, SimSpec
+from .surrogate import Surrogate, features
+
+SURROGATE_WIDTHS = [6, 256, 256, 256, 256, configs.MCMC_WEEKS]
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def prior_sample(n: int, seed: int, priors=None) -> np.ndarray:
+    """n parameter vectors from the prior (numpy PCG64 with `seed`)."""
+    priors = priors or configs.mcmc_priors()
+    rng = np.random.Generator(np.random.PCG64(seed))
+    th = np.zeros((n, mcmc.D))
+    for i, (k, a, b) in enumerate(priors):
+        th[:, i] = rng.beta(a, b, n) if k == mcmc.PRIOR_BETA else rng.uniform(a, b, n)
+    return th
+
+
+def simulate_weekly(engine: Engine, thetas: np.ndarray, seed: int, batch: int = 256,
+                    n_weeks: int = configs.MCMC_WEEKS) -> np.ndarray:
+    """Weekly incidence [n, n_weeks] of one simulation per vector of the MCMC
+    model (1e6 humans, 32 x 32 communities), `batch` simulations per engine
+    batch; simulation i has Philox seed (i << 32) | seed."""
+    lib = mcmc.bind(engine.lib)
+    g, humans = configs.mcmc_shape()
+    base = configs.base_params()
+    days = 7 * n_weeks - 1
+    out = np.zeros((len(thetas), n_weeks), np.uint32)
+    t0 = time.perf_counter()
+    for b0 in range(0, len(thetas), batch):
+        th = thetas[b0:b0 + batch]
+        specs = [SimSpec(g, g, humans, mcmc.sim_params(lib, t, base), ((b0 + i) << 32) | (seed & 0xFFFFFFFF))
+                 for i, t in enumerate(th)]
+        engine.init(specs, days)
+        engine.step(days)
+        for i in range(len(th)):
+            out[b0 + i] = mcmc.weekly_incidence(lib, engine.series(i)["n_incidence"], n_weeks)
+        log(f"[fit] simulated {b0 + len(th)}/{len(thetas)} runs, {time.perf_counter() - t0:.1f} s")
+    return out
+
+
+def train_surrogate(thetas: np.ndarray, weekly: np.ndarray, epochs: int = 200, batch: int = 512,
+                    lr: float = 1e-3, seed: int = 1, lib=None) -> tuple[Surrogate, np.ndarray]:
+    """The MLP (6 -> 256^4 -> n_weeks, GELU) fitted to the weekly incidence
+    by the Poisson loss (Adam), its output bias started at the data's
+    log-mean per week."""
+    widths = list(SURROGATE_WIDTHS[:-1]) + [weekly.shape[1]]
+    sur = Surrogate(widths, max_batch=max(batch, 4096), seed=seed, lib=lib)
+    x = features(thetas, configs.mcmc_priors()).astype(np.float32)
+    y = weekly.astype(np.float32)
+    p = sur.get_params()
+    p[-weekly.shape[1]:] = np.log(y.mean(axis=0) + 1e-2)
+    sur.set_params(p)
+    hist = sur.train(x, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return sur, hist
+
+
+def summarise(m: mcmc.Mcmc, wall: float, burn: int) -> dict:
+    th, ll, acc = m.history()
+    post = th[burn:].copy()
+    post[:, :, 4] = circular_unwrap(post[:, :, 4], 0.0, 2 * math.pi)
+    e = [ess(post[:, :, i]) for i in range(mcmc.D)]
+    r = [rhat(post[:, :, i]) for i in range(mcmc.D)]
+    n = th.shape[0]
+    sampling_wall = wall * (n - burn) / n
+    return {
+        "iterations": n,
+        "burn_in": burn,
+        "wall_s": wall,
+        "s_per_iteration": wall / n,
+        "ess_min": min(e),
+        "ess": dict(zip(mcmc.PARAM_NAMES, e)),
+        "ess_per_hour": min(e) / (sampling_wall / 3600.0),
+        "rhat": dict(zip(mcmc.PARAM_NAMES, r)),
+        "rhat_max": max(r),
+        "acceptance": float(acc[burn:].mean()),
+        "posterior_mean": dict(zip(mcmc.PARAM_NAMES, th[burn:].reshape(-1, mcmc.D).mean(axis=0).tolist())),
+        "posterior_sd": dict(zip(mcmc.PARAM_NAMES, th[burn:].reshape(-1, mcmc.D).std(axis=0).tolist())),
+        "counters": m.counters(),
+    }
+
+
+def run_fit(engine: Engine, data: np.ndarray, iters: int, chains: int = 64, seed: int = 1, sur: Surrogate | None = None,
+            sub_steps: int = 10, theta0=None, burn: int | None = None, chunk: int = 10) -> tuple[mcmc.Mcmc, dict]:
+    """A plain (sur None) or delayed-acceptance fit of `data`; returns the
+    driver and its summary over the iterations after `burn`."""
+    g, humans = configs.mcmc_shape()
+    cfg = mcmc.McmcConfig(n_chains=chains, n_weeks=len(data), priors=configs.mcmc_priors(),
+                          adapt_after=100 * mcmc.D, nb_size=configs.MCMC_NB_SIZE, seed=seed)
+    m = mcmc.Mcmc(cfg, g, g, humans, configs.base_params(), data, theta0=theta0, lib=engine.lib)
+    t0 = time.perf_counter()
+    done = 0
+    while done < iters:
+        n = min(chunk, iters - done)
+        if sur is None:
+            m.run(engine, n)
+        else:
+            m.run_da(engine, sur, n, sub_steps=sub_steps)
+        done += n
+        log(f"[fit] {'delayed-acceptance' if sur is not None else 'plain'} {done}/{iters} iterations, "
+            f"{time.perf_counter() - t0:.1f} s")
+    wall = time.perf_counter() - t0
+    out = summarise(m, wall, iters // 4 if burn is None else burn)
+    if sur is not None:
+        out["delayed_acceptance"] = m.da_counters()
+    return m, out

@@ -37,6 +37,10 @@ class McmcDesc(C.Structure):
     ]
 
 
+class DaDesc(C.Structure):
+    _fields_ = [("sub_steps", _u32), ("reserved", _u32), ("step0", C.c_double), ("target_accept", C.c_double)]
+
+
 class EngineApi(C.Structure):
     _fields_ = [("batch_init", C.c_void_p), ("batch_step", C.c_void_p), ("read_series", C.c_void_p),
                 ("last_error", C.c_void_p)]
@@ -50,6 +54,12 @@ PROTOTYPES = {
                                    C.POINTER(McmcPtr)]),
     "arbo_mcmc_run": (C.c_int, [McmcPtr, abi.EnginePtr, C.POINTER(EngineApi), _u32]),
     "arbo_mcmc_run_target": (C.c_int, [McmcPtr, LOGLIK_FN, C.c_void_p, _u32]),
+    "arbo_mcmc_run_da": (C.c_int, [McmcPtr, abi.EnginePtr, C.POINTER(EngineApi), C.c_void_p, C.POINTER(DaDesc),
+                                   _u32]),
+    "arbo_mcmc_run_da_target": (C.c_int, [McmcPtr, LOGLIK_FN, C.c_void_p, C.c_void_p, C.POINTER(DaDesc), _u32]),
+    "arbo_mcmc_features": (C.c_int, [McmcPtr, _u32, _f64p, C.POINTER(C.c_float)]),
+    "arbo_mcmc_surrogate_logpost": (C.c_int, [McmcPtr, C.c_void_p, _u32, _f64p, _f64p, _f64p]),
+    "arbo_mcmc_da_counters": (C.c_int, [McmcPtr, _u64p, _u64p, _f64p, _f64p]),
     "arbo_mcmc_iterations": (C.c_int, [McmcPtr, _u32p]),
     "arbo_mcmc_read": (C.c_int, [McmcPtr, _u32, _u32, _f64p, _f64p, C.POINTER(C.c_uint8)]),
     "arbo_mcmc_counters": (C.c_int, [McmcPtr, _u64p, _u64p, _u64p, _u64p, _f64p]),
@@ -181,6 +191,52 @@ class Mcmc:
                 return -1
         self._cb = LOGLIK_FN(cb)
         self._check(self.lib.arbo_mcmc_run_target(self._h, self._cb, None, n_iters))
+
+    def run_da(self, engine: Engine, sur, n_iters: int, sub_steps: int = 10, step0: float = 1.0,
+               target_accept: float = 0.574, api: EngineApi | None = None):
+        """n_iters delayed-acceptance iterations: `sub_steps` Langevin steps
+        on the surrogate `sur` (a surrogate.Surrogate), then the simulator."""
+        da = DaDesc(sub_steps, 0, step0, target_accept)
+        self._check(self.lib.arbo_mcmc_run_da(self._h, engine._h, C.byref(api) if api is not None else None, sur._h,
+                                              C.byref(da), n_iters))
+
+    def run_da_target(self, loglik, sur, n_iters: int, sub_steps: int = 10, step0: float = 1.0,
+                      target_accept: float = 0.574):
+        """Delayed acceptance with a Python second-stage log-likelihood."""
+        def cb(ctx, n, th, out):
+            try:
+                t = np.ctypeslib.as_array(th, shape=(n * D,)).reshape(n, D).copy()
+                v = np.asarray(loglik(t), dtype=np.float64)
+                for i in range(n):
+                    out[i] = float(v[i])
+                return 0
+            except Exception:
+                return -1
+        self._cb = LOGLIK_FN(cb)
+        da = DaDesc(sub_steps, 0, step0, target_accept)
+        self._check(self.lib.arbo_mcmc_run_da_target(self._h, self._cb, None, sur._h, C.byref(da), n_iters))
+
+    def da_counters(self) -> dict:
+        st, ac = C.c_uint64(), C.c_uint64()
+        sec, step = C.c_double(), C.c_double()
+        self._check(self.lib.arbo_mcmc_da_counters(self._h, C.byref(st), C.byref(ac), C.byref(sec), C.byref(step)))
+        return {"surrogate_steps": st.value, "surrogate_accepted": ac.value, "surrogate_seconds": sec.value,
+                "step": step.value}
+
+    def features(self, theta) -> np.ndarray:
+        t = _f64(np.atleast_2d(theta))
+        x = np.zeros(t.shape, np.float32)
+        self._check(self.lib.arbo_mcmc_features(self._h, t.shape[0], t.ctypes.data_as(_f64p),
+                                                x.ctypes.data_as(C.POINTER(C.c_float))))
+        return x
+
+    def surrogate_logpost(self, sur, theta):
+        t = _f64(np.atleast_2d(theta))
+        lp = np.zeros(t.shape[0])
+        g = np.zeros_like(t)
+        self._check(self.lib.arbo_mcmc_surrogate_logpost(self._h, sur._h, t.shape[0], t.ctypes.data_as(_f64p),
+                                                         lp.ctypes.data_as(_f64p), g.ctypes.data_as(_f64p)))
+        return lp, g
 
     @property
     def iterations(self) -> int:

@@ -18,6 +18,7 @@ SurPtr = C.c_void_p
 PROTOTYPES = {
     "arbo_sur_create": (C.c_int, [C.c_int32, _u32, C.POINTER(C.c_uint32), _u32, C.c_uint64, C.POINTER(SurPtr)]),
     "arbo_sur_n_params": (C.c_int, [SurPtr, C.POINTER(C.c_uint64)]),
+    "arbo_sur_shape": (C.c_int, [SurPtr, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "arbo_sur_get_params": (C.c_int, [SurPtr, _f32p]),
     "arbo_sur_set_params": (C.c_int, [SurPtr, _f32p]),
     "arbo_sur_forward": (C.c_int, [SurPtr, _u32, _f32p, _f32p]),

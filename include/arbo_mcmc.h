@@ -28,6 +28,7 @@
 #include <stdint.h>
 
 #include "arbo_c.h"
+#include "arbo_surrogate.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -79,6 +80,23 @@ typedef int (*ArboLogLikFn)(void* ctx, uint32_t n, const double* theta, double* 
 
 typedef struct ArboMcmc ArboMcmc;
 
+/* Delayed acceptance with the surrogate (no counterpart in the reference).
+ * Each iteration every chain runs sub_steps Metropolis-adjusted Langevin
+ * steps on the surrogate posterior (prior x negative-binomial likelihood of
+ * the surrogate's mean incidence), preconditioned by the chains' pooled
+ * covariance, then the simulator judges the end point:
+ *   alpha2 = min(1, L(theta') L_s(theta) / (L(theta) L_s(theta')))
+ * (Christen & Fox 2005; the surrogate-transition method of Liu 2001), which
+ * leaves the posterior under the simulator's likelihood exact.  The step
+ * size adapts (diminishing Robbins-Monro) to target_accept of the surrogate
+ * steps. */
+typedef struct ArboDaDesc {
+    uint32_t sub_steps;       /* surrogate steps per simulator evaluation, 1..1000 */
+    uint32_t reserved;
+    double step0;             /* initial step in preconditioner units */
+    double target_accept;     /* surrogate-step acceptance the step size adapts to (0.574 for MALA) */
+} ArboDaDesc;
+
 /* Create the driver.  shape: the model's lattice and humans per community
  * (copied); base: every simulation parameter not in the vector; data:
  * n_weeks observed weekly counts; theta0: n_chains * 6 starting values, or
@@ -93,6 +111,30 @@ ARBO_API int arbo_mcmc_run(ArboMcmc* m, ArboEngine* engine, const ArboEngineApi*
 
 /* The same iterations with a caller-supplied log-likelihood. */
 ARBO_API int arbo_mcmc_run_target(ArboMcmc* m, ArboLogLikFn fn, void* ctx, uint32_t n_iters);
+
+/* n_iters delayed-acceptance iterations: surrogate `sur` (inputs:
+ * arbo_mcmc_features, outputs: n_weeks log-mean weekly incidences) first,
+ * simulations on the engine second. */
+ARBO_API int arbo_mcmc_run_da(ArboMcmc* m, ArboEngine* engine, const ArboEngineApi* api, ArboSurrogate* sur,
+                              const ArboDaDesc* da, uint32_t n_iters);
+
+/* The same with a caller-supplied second-stage log-likelihood. */
+ARBO_API int arbo_mcmc_run_da_target(ArboMcmc* m, ArboLogLikFn fn, void* ctx, ArboSurrogate* sur,
+                                     const ArboDaDesc* da, uint32_t n_iters);
+
+/* Surrogate inputs of n vectors (x[n * 6]): parameters 0-3 scaled from their
+ * prior range to [-1, 1], the phase as (cos, sin). */
+ARBO_API int arbo_mcmc_features(const ArboMcmc* m, uint32_t n, const double* theta, float* x);
+
+/* Surrogate log-posterior and its gradient for n vectors (out[n],
+ * grad[n * 6]). */
+ARBO_API int arbo_mcmc_surrogate_logpost(ArboMcmc* m, ArboSurrogate* sur, uint32_t n, const double* theta,
+                                         double* out, double* grad);
+
+/* Delayed-acceptance counters: surrogate steps proposed and accepted,
+ * seconds in the surrogate, current step size. */
+ARBO_API int arbo_mcmc_da_counters(const ArboMcmc* m, uint64_t* steps, uint64_t* accepted, double* seconds,
+                                   double* step);
 
 /* Iterations recorded so far (not counting the starting values). */
 ARBO_API int arbo_mcmc_iterations(const ArboMcmc* m, uint32_t* out);

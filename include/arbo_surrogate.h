@@ -36,6 +36,8 @@ typedef struct ArboSurrogate ArboSurrogate;
 ARBO_API int arbo_sur_create(int32_t device, uint32_t n_widths, const uint32_t* widths, uint32_t max_batch,
                              uint64_t seed, ArboSurrogate** out);
 ARBO_API int arbo_sur_n_params(const ArboSurrogate* s, uint64_t* out);
+/* Input and output widths. */
+ARBO_API int arbo_sur_shape(const ArboSurrogate* s, uint32_t* inputs, uint32_t* outputs);
 ARBO_API int arbo_sur_get_params(ArboSurrogate* s, float* out);
 ARBO_API int arbo_sur_set_params(ArboSurrogate* s, const float* in);
 

@@ -173,3 +173,12 @@ def test_driver_over_oracle_engine(lib, oracle_lib):
     (a, b) = runs
     for x, y in zip(a[0], b[0]):
         assert np.array_equal(x, y)
+
+
+def test_features_match_python(lib):
+    from arbo_gpu_fit_amd.surrogate import features
+    cfg = mcmc.McmcConfig(n_chains=2, n_weeks=3, priors=configs.mcmc_priors())
+    m = _dummy_model(cfg, lib)
+    th = np.array([[0.3, 0.1, -2.5, 0.8, 5.0, 0.1], [0.05, 0.6, -1.0, 0.0, 0.0, 0.5]])
+    assert np.allclose(m.features(th), features(th, configs.mcmc_priors()), atol=1e-6)
+    m.close()
