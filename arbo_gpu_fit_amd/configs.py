@@ -194,7 +194,9 @@ def mcmc_spec(theta: dict | None = None, seed: int = 3, **kw) -> SimSpec:
 # fraction, seasonal amplitude, seasonal phase, reporting rate) of a model of
 # 1e6 humans in 1,000 requested communities (32 x 32 after squaring), 260
 # weekly counts (7 * 260 - 1 = 1,819 days, main.cpp:34), negative-binomial
-# observations of size 10, synthetic data from the model at MCMC_TRUTH.
+# observations of size r = 10 (variance mu + mu^2 / 10; the reference's
+# dispersion parameter is 1 / r, MCMC.cpp:242-243), synthetic data from the
+# model at MCMC_TRUTH.
 
 MCMC_WEEKS = 260
 MCMC_NB_SIZE = 10.0

@@ -100,6 +100,83 @@ def cpu_baseline(specs, days: int, threads: int, serial_days: int) -> dict:
     }
 
 
+def mcmc_bench(args) -> dict:
+    """MCMC ESS/hour, plain batched adaptive Metropolis vs surrogate delayed
+    acceptance, on the 6-parameter model (1e6 humans, 1,024 communities, 260
+    weekly counts, synthetic data at the true parameters):
+      1. `train_runs` simulations at prior draws -> weekly incidence;
+      2. the surrogate MLP trained on them (Poisson loss, Adam, `epochs`);
+      3. a surrogate-only pilot chain gives both samplers the same starting
+         states near the posterior;
+      4. `iters` iterations of each sampler (64 chains, one simulation per
+         proposal); ESS over the iterations after the first quarter, per hour
+         of that sampler's own wall clock."""
+    import numpy as np
+
+    from arbo_gpu_fit_amd import abi, configs, fit, mcmc
+    from arbo_gpu_fit_amd.engine import Engine
+
+    lib = abi.load_hip_library()
+    eng = Engine(lib, flags=abi.F_GRAPH)
+    data, truth_weekly = configs.mcmc_data(eng)
+    log(f"[mcmc] data: {int(truth_weekly.sum())} cases simulated, {int(data.sum())} observed")
+    t0 = time.perf_counter()
+    if args.load:
+        z = np.load(args.load)
+        thetas, weekly = z["thetas"], z["weekly"]
+    else:
+        thetas = fit.prior_sample(args.train_runs, seed=11)
+        weekly = fit.simulate_weekly(eng, thetas, seed=12)
+    t_data = time.perf_counter() - t0
+    if args.save:
+        np.savez_compressed(args.save, thetas=thetas, weekly=weekly)
+    t0 = time.perf_counter()
+    sur, hist = fit.train_surrogate(thetas, weekly, epochs=args.epochs, lib=lib)
+    t_train = time.perf_counter() - t0
+    log(f"[mcmc] surrogate trained in {t_train:.1f} s: loss {hist[0]:.4g} -> {hist[-1]:.4g}")
+    # surrogate-only pilot: delayed acceptance whose second stage is the
+    # surrogate itself, so every surrogate move is kept
+    g, humans = configs.mcmc_shape()
+    cfg = mcmc.McmcConfig(n_chains=args.chains, n_weeks=len(data), priors=configs.mcmc_priors(),
+                          adapt_after=100 * mcmc.D, nb_size=configs.MCMC_NB_SIZE, seed=99)
+    pilot = mcmc.Mcmc(cfg, g, g, humans, configs.base_params(), data, lib=lib)
+
+    def sur_loglik(th):
+        lp, _ = pilot.surrogate_logpost(sur, th)
+        return lp - np.array([pilot.log_prior(t) for t in th])
+
+    pilot.run_da_target(sur_loglik, sur, 200, sub_steps=5)
+    theta0 = pilot.history()[0][-1]
+    pilot.close()
+    runs = {}
+    for name, s in (("plain", None), ("surrogate", sur)):
+        _, runs[name] = fit.run_fit(eng, data, args.iters, chains=args.chains, seed=args.mcmc_seed, sur=s,
+                                    sub_steps=args.sub_steps, theta0=theta0)
+        log(f"[mcmc] {name}: ESS min {runs[name]['ess_min']:.1f}, {runs[name]['ess_per_hour']:.0f}/h, "
+            f"R-hat max {runs[name]['rhat_max']:.3f}")
+    gain = runs["surrogate"]["ess_per_hour"] / runs["plain"]["ess_per_hour"]
+    with_training = runs["surrogate"]["ess_min"] / ((runs["surrogate"]["wall_s"] * 0.75 + t_data + t_train) / 3600)
+    out = {
+        "metric": "MCMC ESS/hour plain vs surrogate",
+        "value": gain,
+        "unit": "ratio of effective samples per hour (surrogate delayed acceptance / plain adaptive Metropolis)",
+        "n_gpus": 1,
+        "plain": runs["plain"],
+        "surrogate": runs["surrogate"],
+        "surrogate_ess_per_hour_incl_training": with_training,
+        "training": {"runs": int(len(thetas)), "simulate_s": t_data, "train_s": t_train, "epochs": args.epochs,
+                     "loss_first": float(hist[0]), "loss_last": float(hist[-1])},
+        "truth": dict(zip(mcmc.PARAM_NAMES, configs.MCMC_TRUTH.tolist())),
+        "data": "synthetic (the model at the true parameters, NB(10) observations)",
+        "config": {"workload": f"MCMC fit: {args.chains} chains x 1 simulation per proposal, 1e6 humans in "
+                               f"32x32 communities, 1,819 days, 260 weekly counts; {args.iters} iterations each",
+                   "sub_steps": args.sub_steps},
+    }
+    sur.close()
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +188,15 @@ def main():
     ap.add_argument("--cpu-days", type=int, default=365)
     ap.add_argument("--cpu-serial-days", type=int, default=365)
     ap.add_argument("--graph", action="store_true", help="replay the day loop as hipGraphs (no per-kernel timing)")
+    ap.add_argument("--mcmc", action="store_true", help="MCMC ESS/hour, plain vs surrogate, instead")
+    ap.add_argument("--chains", type=int, default=64)
+    ap.add_argument("--iters", type=int, default=160)
+    ap.add_argument("--mcmc-seed", type=int, default=1)
+    ap.add_argument("--train-runs", type=int, default=16384)
+    ap.add_argument("--epochs", type=int, default=200)
+    ap.add_argument("--sub-steps", type=int, default=10)
+    ap.add_argument("--save", default="", help="save the surrogate's training set (npz)")
+    ap.add_argument("--load", default="", help="load the surrogate's training set (npz)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -130,6 +216,10 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         log(f"copied {args.calibrate_copy} MiB x3")
+        return
+
+    if args.mcmc:
+        print(json.dumps(mcmc_bench(args)), flush=True)
         return
 
     from arbo_gpu_fit_amd import abi, configs
