@@ -87,6 +87,8 @@ struct EngineView {
     uint32_t* alive;               // [M/32] alive-slot bitmap of the current day
     uint32_t* cnt;                 // [G][9 in-edges][8 = dir*4 + serotype] local events arriving today
     uint32_t* nld;                 // [G][8] long-distance arrivals of the current day
+    uint32_t* hsum;                // [G][4] human summary: earliest day any member has an event (0 =
+                                   // unknown), infectious counts by serotype (two 16-bit pairs), pad
     unsigned long long* lam;       // [2 parity][n_sims][8]  long-distance rate, 2^-24 fixed point
     uint32_t* series;
     const uint32_t* csr_inbox;     // per CSR entry: inbox row (destination * 9 + in-edge index)

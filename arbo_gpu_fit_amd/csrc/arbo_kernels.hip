@@ -67,6 +67,15 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return x;
 }
 
+__device__ __forceinline__ uint32_t wave_min(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = __shfl_xor(x, o, 64);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
 // Exclusive prefix sum over the wavefront; `total` gets the wave sum.
 __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, uint32_t& total) {
     const uint32_t lane = threadIdx.x;
@@ -243,7 +252,13 @@ __global__ __launch_bounds__(kWave) void k_init(EngineView v) {
 // ---------------------------------------------------------------------------
 // Day kernel A: demography, counters, event counts, series row t+1.
 
-struct HumanCount { uint32_t ih01, ih23, inc01, inc23; };
+// Per-lane human tallies; minev is the earliest day, after today's updates,
+// on which one of the lane's humans dies or changes state.
+struct HumanCount { uint32_t ih01, ih23, inc01, inc23, minev; };
+
+__device__ __forceinline__ uint32_t day_of(uint32_t t, uint32_t nx) {   // absolute day of a 16-bit day near t
+    return (uint32_t)((int32_t)t + (int32_t)(int16_t)(uint16_t)(nx - t));
+}
 
 // Four humans of ndemographic.cu:57-103 (one 16-byte vector of packed words,
 // index 4q..4q+3).  The next-event days of the vector are read with one
@@ -263,6 +278,14 @@ __device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& c
     }
     dying &= valid;
     need &= valid & ~dying;
+    {   // earliest death day of the members that survive today (words order by death day)
+        uint32_t mw = 0xFFFFFFFFu;
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e)
+            if (((valid & ~dying) >> e) & 1u) mw = ws[e] < mw ? ws[e] : mw;
+        const uint32_t md = mw >> 8;
+        hc.minev = md < hc.minev ? md : hc.minev;
+    }
     if (!(dying | need)) return;
     uint32_t nxs[4] = {0, 0, 0, 0};
     if (need) {
@@ -279,22 +302,33 @@ __device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& c
             const uint32_t da = death_age(v.tables + sd.death_h_off, kHumanTable - 1, (r.x >> 8) + 1);
             v.h_word[gi] = (t + da) << 8;
             v.h_birth[gi] = (int32_t)t;
+            hc.minev = t + da < hc.minev ? t + da : hc.minev;
             continue;
         }
         if (!((need >> e) & 1u)) continue;
         const uint32_t st = w & 3u, strain = (w >> 2) & 3u;
         const uint32_t nx = nxs[e];
         const bool due = (int16_t)(uint16_t)(t - nx) > 0;
+        uint32_t ev = 0xFFFFFFFFu;                // next state change: the day after `next`
         if (st == 2) {
-            if (due) v.h_word[gi] = w & ~0xFu;   // recovered
-            else add_pair(hc.ih01, hc.ih23, strain);
+            if (due) {
+                v.h_word[gi] = w & ~0xFu;        // recovered
+            } else {
+                add_pair(hc.ih01, hc.ih23, strain);
+                ev = day_of(t, nx) + 1;
+            }
         } else if (due) {                        // E -> I
             const U4 r = draw(key, cd.h_base + i, t, P_H_REC, 0);
-            v.h_next[gi] = (uint16_t)(nx + stoch_round(sd.recovery, r.x));
+            const uint32_t nn = nx + stoch_round(sd.recovery, r.x);
+            v.h_next[gi] = (uint16_t)nn;
             v.h_word[gi] = (w & ~3u) | 2u;
             add_pair(hc.inc01, hc.inc23, strain);
             add_pair(hc.ih01, hc.ih23, strain);
+            ev = day_of(t, nn) + 1;
+        } else {
+            ev = day_of(t, nx) + 1;
         }
+        hc.minev = ev < hc.minev ? ev : hc.minev;
     }
 }
 
@@ -302,6 +336,11 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     const uint32_t g = blockIdx.x, lane = threadIdx.x;
     const uint32_t t = *v.day;
     const CommDesc cd = v.comm[g];
+    // Humans are scanned only on days when one of them may die or change
+    // state: the summary holds the earliest such day (0 forces a scan) and the
+    // infectious counts, which nothing else changes in between.
+    const uint4 hs = reinterpret_cast<const uint4*>(v.hsum)[g];
+    const bool scan_h = t >= hs.x;
     // the first census batches go out before anything else
     const uint32_t nvec = (cd.n + 3) >> 2;
     const uint32_t mvec = (cd.cap + 7) >> 3;
@@ -311,7 +350,7 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
 #pragma unroll
     for (uint32_t u = 0; u < kUnroll; ++u) {
         const uint32_t q = u * kWave + lane;
-        if (q < nvec) hb[u] = w4p[q];
+        if (scan_h && q < nvec) hb[u] = w4p[q];
     }
 #pragma unroll
     for (uint32_t u = 0; u < kEarlyM; ++u) {
@@ -374,13 +413,17 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     const float dens_next = v.dens[sd.dens_off + t + 1];
 
     // ---- humans ----
-    HumanCount hc{0, 0, 0, 0};
+    HumanCount hc{0, 0, 0, 0, 0xFFFFFFFFu};
+    if (!scan_h && lane == 0) {
+        hc.ih01 = hs.y;
+        hc.ih23 = hs.z;
+    }
 #pragma unroll
     for (uint32_t u = 0; u < kUnroll; ++u) {
         const uint32_t q = u * kWave + lane;
-        if (q < nvec) human_vec(v, cd, sd, key, t, q, hb[u], hc);
+        if (scan_h && q < nvec) human_vec(v, cd, sd, key, t, q, hb[u], hc);
     }
-    for (uint32_t q0 = kUnroll * kWave; q0 < nvec; q0 += kUnroll * kWave) {
+    for (uint32_t q0 = kUnroll * kWave; scan_h && q0 < nvec; q0 += kUnroll * kWave) {
         uint4 w4[kUnroll];
 #pragma unroll
         for (uint32_t u = 0; u < kUnroll; ++u) {
@@ -473,6 +516,10 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     const uint32_t inc01 = wave_sum(hc.inc01), inc23 = wave_sum(hc.inc23);
     im01 = wave_sum(im01);
     im23 = wave_sum(im23);
+    if (scan_h) {
+        const uint32_t minev = wave_min(hc.minev);
+        if (lane == 0) reinterpret_cast<uint4*>(v.hsum)[g] = make_uint4(minev, ih01, ih23, 0u);
+    }
     const uint32_t alive_c = wave_sum(alive);
     const uint32_t dead_c = cd.cap - alive_c;
 
@@ -832,6 +879,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     // ---- onto humans: uniform target, success iff S and never infected by s ----
     if (s_qseg[DIR_MH][nseg] > 0 || sd.intro > 0.0f) {
         const uint32_t N = cd.n;
+        bool infected = false;
         for (uint32_t pos = 0; pos < S; ++pos) {
             const uint32_t s = (s_rot[DIR_MH] + pos) % S;
             const uint32_t qb = s_qseg[DIR_MH][pos * (in_len + 1)];
@@ -877,6 +925,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                     v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
                     const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
                     v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
+                    infected = true;
                 }
                 __syncthreads();
             }
@@ -892,12 +941,18 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                             v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
                             const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
                             v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
+                            infected = true;
                             x--;
                         }
                     }
                 }
                 __syncthreads();
             }
+        }
+        // a human exposed today turns infectious on day t + iip + 1 at the earliest
+        if (__any(infected) && lane == 0) {
+            const uint32_t ev = t + sd.iip_min + 1;
+            if (ev < v.hsum[4 * (size_t)g]) v.hsum[4 * (size_t)g] = ev;
         }
     }
 }
