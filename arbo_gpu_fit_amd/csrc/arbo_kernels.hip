@@ -67,6 +67,15 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
     return x;
 }
 
+__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t y = __shfl_xor(x, o, 64);
+        x = y > x ? y : x;
+    }
+    return x;
+}
+
 __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -344,6 +353,10 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     // the first census batches go out before anything else
     const uint32_t nvec = (cd.n + 3) >> 2;
     const uint32_t mvec = (cd.cap + 7) >> 3;
+    // Mosquito slots from vector mlim on are all empty (births fill the
+    // lowest empty slots, so the alive slots stay packed low): they are not
+    // read.  The summary keeps mlim + 1 (0 = unknown: scan every slot).
+    const uint32_t mlim = (hs.w == 0u || hs.w - 1u > mvec) ? mvec : hs.w - 1u;
     const uint4* w4p = reinterpret_cast<const uint4*>(v.h_word + cd.h_off);
     const uint4* m4p = reinterpret_cast<const uint4*>(v.m_word + cd.m_off);
     uint4 hb[kUnroll], mb[kEarlyM];
@@ -355,7 +368,7 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
 #pragma unroll
     for (uint32_t u = 0; u < kEarlyM; ++u) {
         const uint32_t q = u * kWave + lane;
-        if (q < mvec) mb[u] = m4p[q];
+        if (q < mlim) mb[u] = m4p[q];
     }
     const SimDesc sd = v.sims[cd.sim];
     const Key key{sd.k0, sd.k1};
@@ -447,26 +460,26 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
         // per 16-bit half: t mod 4096 plus a 4096 guard, so (t - death day) mod 4096
         // is the half's low 12 bits with no borrow between halves
         const uint32_t T2 = ((t & 0xFFFu) * 0x00010001u) | 0x10001000u;
-        for (uint32_t q0 = 0; q0 < mvec; q0 += kUnroll * kWave) {
+        for (uint32_t q0 = 0; q0 < mlim; q0 += kUnroll * kWave) {
             uint4 raw[kUnroll];
             if (q0 == 0) {
 #pragma unroll
                 for (uint32_t u = 0; u < kUnroll; ++u) {
                     const uint32_t q = u * kWave + lane;
                     if (u < kEarlyM) raw[u] = mb[u < kEarlyM ? u : 0];
-                    else if (q < mvec) raw[u] = m4p[q];
+                    else if (q < mlim) raw[u] = m4p[q];
                 }
             } else {
 #pragma unroll
                 for (uint32_t u = 0; u < kUnroll; ++u) {
                     const uint32_t q = q0 + u * kWave + lane;
-                    if (q < mvec) raw[u] = m4p[q];
+                    if (q < mlim) raw[u] = m4p[q];
                 }
             }
 #pragma unroll
             for (uint32_t u = 0; u < kUnroll; ++u) {
                 const uint32_t q = q0 + u * kWave + lane;
-                if (q >= mvec) continue;
+                if (q >= mlim) continue;
                 const uint32_t rw[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
                 uint32_t st = 0, sn = 0, dy = 0;
 #pragma unroll
@@ -509,17 +522,19 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
                 s_died8[q] = (uint8_t)((died & 0x55u) | ((died >> 15) & 0xAAu));
             }
         }
-        // the tail bytes of the last bitmap word
-        for (uint32_t q = mvec + lane; q < 4 * nwords; q += kWave) { s_dead8[q] = 0; s_died8[q] = 0; }
+        // the empty slots not read (all dead, none died today) and the bytes
+        // past the capacity in the last bitmap word
+        for (uint32_t q = mlim + lane; q < 4 * nwords; q += kWave) {
+            const uint32_t left = q < mvec ? cd.cap - 8 * q : 0u;
+            s_dead8[q] = (uint8_t)(left >= 8 ? 0xFFu : ((1u << left) - 1u));
+            s_died8[q] = 0;
+        }
     }
     const uint32_t ih01 = wave_sum(hc.ih01), ih23 = wave_sum(hc.ih23);
     const uint32_t inc01 = wave_sum(hc.inc01), inc23 = wave_sum(hc.inc23);
     im01 = wave_sum(im01);
     im23 = wave_sum(im23);
-    if (scan_h) {
-        const uint32_t minev = wave_min(hc.minev);
-        if (lane == 0) reinterpret_cast<uint4*>(v.hsum)[g] = make_uint4(minev, ih01, ih23, 0u);
-    }
+    const uint32_t minev = scan_h ? wave_min(hc.minev) : hs.x;
     const uint32_t alive_c = wave_sum(alive);
     const uint32_t dead_c = cd.cap - alive_c;
 
@@ -572,6 +587,7 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
         }
     }
     // slots that died today and stay empty; the alive bitmap of the day
+    uint32_t top = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kWordsPerLane; ++k) {
         const uint32_t w = kWordsPerLane * lane + k;
@@ -587,8 +603,13 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
             stay &= stay - 1;
             v.m_word[cd.m_off + 32u * w + e] = 3;
         }
-        v.alive[(cd.m_off >> 5) + w] = ~(d & ~reborn) & valid_bits(w, cd.cap);
+        const uint32_t aw = ~(d & ~reborn) & valid_bits(w, cd.cap);
+        v.alive[(cd.m_off >> 5) + w] = aw;
+        if (aw) top = 32u * w + 32u - (uint32_t)__clz((int)aw);   // one past the highest alive slot
     }
+    top = wave_max(top);
+    if (lane == 0)   // the day's summary: human part (unchanged when not scanned) and mosquito limit
+        reinterpret_cast<uint4*>(v.hsum)[g] = make_uint4(minev, ih01, ih23, ((top + 7) >> 3) + 1u);
     const uint32_t Mc = alive_c + births;
 
     // ---- event counts of this community as a source (nepidemic.cu:49-136) ----
