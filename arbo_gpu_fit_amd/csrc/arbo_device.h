@@ -34,8 +34,10 @@ struct CommDesc {
     uint32_t in_off;      // global offset of its in-edges
     uint32_t in_len;
     float base;           // m_ratio * n (f32)
-    uint32_t pad;
+    uint32_t flags;       // kCommIntro: the simulation has external introductions
 };
+
+constexpr uint32_t kCommIntro = 1u;
 
 struct SimDesc {
     uint32_t k0, k1;              // Philox key = 64-bit seed
@@ -87,6 +89,8 @@ struct EngineView {
     uint32_t* alive;               // [M/32] alive-slot bitmap of the current day
     uint32_t* cnt;                 // [G][9 in-edges][8 = dir*4 + serotype] local events arriving today
     uint32_t* nld;                 // [G][8] long-distance arrivals of the current day
+    uint32_t* active;              // [G] 1 when events arrive today (set by the sources, cleared by the
+                                   // destination's event kernel)
     uint32_t* hsum;                // [G][4] human summary: earliest day any member has an event (0 =
                                    // unknown), infectious counts by serotype (two 16-bit pairs), pad
     unsigned long long* lam;       // [2 parity][n_sims][8]  long-distance rate, 2^-24 fixed point

@@ -66,7 +66,7 @@ struct ArboEngine {
     // device memory
     DevBuf<CommDesc> comm;
     DevBuf<SimDesc> simd;
-    DevBuf<uint32_t> h_word, alive, cnt, nld, hsum, series, csr_thr, csr_inbox, in_edge, in_slot, tables, day_ctr;
+    DevBuf<uint32_t> h_word, alive, cnt, nld, active, hsum, series, csr_thr, csr_inbox, in_edge, in_slot, tables, day_ctr;
     DevBuf<uint16_t> h_next, m_word, m_next;
     DevBuf<int32_t> h_birth;
     DevBuf<unsigned long long> lam;
@@ -130,7 +130,7 @@ void release_graph(ArboEngine* e) {
 void free_batch(ArboEngine* e) {
     release_graph(e);
     dev_free(e->comm); dev_free(e->simd);
-    dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->hsum); dev_free(e->series); dev_free(e->csr_thr); dev_free(e->csr_inbox);
+    dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->active); dev_free(e->hsum); dev_free(e->series); dev_free(e->csr_thr); dev_free(e->csr_inbox);
     dev_free(e->in_edge); dev_free(e->in_slot); dev_free(e->tables); dev_free(e->day_ctr);
     dev_free(e->h_next); dev_free(e->m_word); dev_free(e->m_next); dev_free(e->h_birth);
     dev_free(e->lam); dev_free(e->dens);
@@ -376,6 +376,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
             cd.in_off = in_base + t.in_ptr[c];
             cd.in_len = t.in_ptr[c + 1] - t.in_ptr[c];
             cd.base = t.base[c];
+            cd.flags = params[i].external_infection > 0.0f ? kCommIntro : 0u;
             comm.push_back(cd);
             h_tot += cd.n;
             m_tot += cd.cap;
@@ -396,7 +397,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         (rc = dev_alloc(e, e->h_word, h_tot)) || (rc = dev_alloc(e, e->h_next, h_tot)) ||
         (rc = dev_alloc(e, e->h_birth, h_tot)) || (rc = dev_alloc(e, e->m_word, m_tot)) ||
         (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kInboxEdges * 8)) || (rc = dev_alloc(e, e->nld, comm.size() * 8)) ||
-        (rc = dev_alloc(e, e->hsum, comm.size() * 4)) ||
+        (rc = dev_alloc(e, e->hsum, comm.size() * 4)) || (rc = dev_alloc(e, e->active, comm.size())) ||
         (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->series, ser_tot)) ||
         (rc = dev_alloc(e, e->csr_thr, csr_thr.size())) || (rc = dev_alloc(e, e->csr_inbox, csr_inbox.size())) || (rc = dev_alloc(e, e->in_edge, in_edge.size())) ||
         (rc = dev_alloc(e, e->in_slot, in_slot.size())) || (rc = dev_alloc(e, e->tables, table_pool.size())) ||
@@ -417,6 +418,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     HIP_TRY(e, launch_fill(e->day_ctr.p, 1, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->nld.p, comm.size() * 8, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->hsum.p, comm.size() * 4, 0u, e->stream));   // every community scans its humans on day 0
+    HIP_TRY(e, launch_fill(e->active.p, comm.size(), 0u, e->stream));
     // padding words: humans beyond a community are never read as members, but
     // keep the whole array defined
     HIP_TRY(e, launch_fill(e->h_word.p, h_tot, 0u, e->stream));
@@ -432,6 +434,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.m_next = e->m_next.p;
     v.alive = e->alive.p;
     v.hsum = e->hsum.p;
+    v.active = e->active.p;
     v.cnt = e->cnt.p;
     v.nld = e->nld.p;
     v.lam = e->lam.p;

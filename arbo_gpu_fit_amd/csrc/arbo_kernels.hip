@@ -665,7 +665,11 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     for (uint32_t i0 = 0; i0 < 8 * rl; i0 += kWave) {   // wave-uniform trip count: every lane shuffles
         const uint32_t i = i0 + lane, col = i >> 3, k = i & 7u;
         const uint32_t dst = __shfl(inboxent, col < rl ? col : 0u, kWave);
-        if (i < 8 * rl) v.cnt[(size_t)dst * 8 + k] = s_hist[k * 9 + col];
+        if (i < 8 * rl) {
+            const uint32_t n = s_hist[k * 9 + col];
+            v.cnt[(size_t)dst * 8 + k] = n;
+            if (n) v.active[dst / kInboxEdges] = 1u;   // the destination has events today
+        }
     }
 
     // ---- series row t+1 (datacollect.cu:79-189) ----
@@ -772,6 +776,9 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     const uint32_t g = blockIdx.x, lane = threadIdx.x;
     const uint32_t t = *v.day;
     const CommDesc cd = v.comm[g];
+    const uint32_t act = v.active[g];
+    if (!act && !(cd.flags & kCommIntro)) return;   // no events arrive and no introductions
+    if (act && lane == 0) v.active[g] = 0u;         // consumed
     const SimDesc sd = v.sims[cd.sim];
     const Key key{sd.k0, sd.k1};
     const uint32_t S = sd.S;
@@ -799,7 +806,6 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
         if (n) v.nld[(size_t)g * 8 + lane] = 0;
         tot_dir[lane >> 2] += n;
     }
-    if (cd.c == 0 && lane < 8) v.lam[((size_t)((t + 1) & 1u) * v.n_sims + cd.sim) * 8 + lane] = 0ull;
     __syncthreads();
     for (uint32_t i = lane; i < 8 * in_len; i += kWave) {
         const uint32_t k = i / in_len, e = i % in_len;      // k = dir*4 + s
@@ -1015,7 +1021,10 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v) {
         const uint32_t idx = l - pre[k];
         const uint32_t d = mulhi(draw(key, idx, t, ctr_tag(P_LD_DEST, k >> 2, k & 3u, 0), 0).x, sd.C);
         atomicAdd(&v.nld[(size_t)(sd.comm_off + d) * 8 + k], 1u);
+        v.active[sd.comm_off + d] = 1u;
     }
+    // the rate sums of the next day start at zero
+    if (part == 0 && tid < 8) v.lam[((size_t)((t + 1) & 1u) * v.n_sims + sim) * 8 + tid] = 0ull;
 }
 
 __global__ void k_advance_day(uint32_t* day) {
