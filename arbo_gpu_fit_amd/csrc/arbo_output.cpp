@@ -1,0 +1,37 @@
+// arbo_output.cpp — the nData time-series file of writeIncidence
+// (writeoutput.cpp:13-98) from a simulation's series.
+#include <cstdio>
+#include <string>
+
+#include "arbo_c.h"
+#include "arbo_output.h"
+
+extern "C" {
+
+ARBO_API int arbo_write_ndata(const char* path, const ArboSeries* s, uint32_t rows, uint32_t S, uint32_t sim_run) {
+    if (!path || !s || !s->n_count || !s->n_incidence || !s->n_infected || !s->n_one_subpop || rows == 0 ||
+        S == 0 || S > ARBO_MAX_SEROTYPES)
+        return ARBO_E_INVALID;
+    FILE* f = std::fopen(path, sim_run == 0 ? "wb" : "ab");
+    if (!f) return ARBO_E_INVALID;
+    std::string out;
+    out.reserve((size_t)rows * (16 + 36 * S));
+    if (sim_run == 0) {
+        out += "t,popSize";
+        for (uint32_t v = 0; v < S; ++v) out += ",incidence_DENV" + std::to_string(v + 1);
+        for (uint32_t v = 0; v < S; ++v) out += ",prevalence_DENV" + std::to_string(v + 1);
+        for (uint32_t v = 0; v < S; ++v) out += ",prevalence_DENV" + std::to_string(v + 1) + "_random_community";
+        out += ",simRun";
+    }
+    for (uint32_t t = 0; t < rows; ++t) {
+        out += "\n" + std::to_string(t) + "," + std::to_string(s->n_count[t]);
+        for (uint32_t v = 0; v < S; ++v) out += "," + std::to_string(s->n_incidence[(size_t)t * S + v]);
+        for (uint32_t v = 0; v < S; ++v) out += "," + std::to_string(s->n_infected[(size_t)t * S + v]);
+        for (uint32_t v = 0; v < S; ++v) out += "," + std::to_string(s->n_one_subpop[(size_t)t * S + v]);
+        out += "," + std::to_string(sim_run);
+    }
+    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    return (std::fclose(f) == 0 && ok) ? ARBO_OK : ARBO_E_INVALID;
+}
+
+}  // extern "C"

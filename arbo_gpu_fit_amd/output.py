@@ -1,0 +1,35 @@
+"""ctypes mirror of include/arbo_output.h: the reference's nData files."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .engine import ArboError
+
+PROTOTYPES = {
+    "arbo_write_ndata": (C.c_int, [C.c_char_p, C.POINTER(abi.Series), C.c_uint32, C.c_uint32, C.c_uint32]),
+}
+
+
+def bind(lib: C.CDLL) -> C.CDLL:
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def write_ndata(path: str, series: dict, sim_run: int, lib: C.CDLL | None = None):
+    """Write (sim_run 0) or append one run's series (Engine.series) to an
+    nData file (writeoutput.cpp:13-98)."""
+    lib = bind(lib if lib is not None else abi.load_hip_library())
+    arr = {k: np.ascontiguousarray(series[k], dtype=np.uint32) for k in
+           ("n_count", "m_count", "n_infected", "n_incidence", "n_one_subpop")}
+    rows, S = arr["n_incidence"].shape
+    p = {k: v.ctypes.data_as(C.POINTER(C.c_uint32)) for k, v in arr.items()}
+    s = abi.Series(p["n_count"], p["m_count"], p["n_infected"], p["n_incidence"], p["n_one_subpop"])
+    rc = lib.arbo_write_ndata(path.encode(), C.byref(s), rows, S, sim_run)
+    if rc:
+        raise ArboError(rc, f"arbo_write_ndata({path})")

@@ -1,0 +1,31 @@
+/* arbo_output.h — the reference's output files from engine series.
+ *
+ * arbo_write_ndata replaces writeIncidence (writeoutput.cpp:13-98): the
+ * nData<mcmcRun>.csv time series of one simulation run, created with its
+ * header by run 0 and appended to by later runs.  Columns: t, popSize,
+ * incidence_DENV1..S, prevalence_DENV1..S,
+ * prevalence_DENV1..S_random_community, simRun; one row per day 0..rows-1,
+ * each written after a newline (the file has no trailing newline).
+ *
+ * Conventions as include/arbo_c.h.
+ */
+#ifndef ARBO_OUTPUT_H
+#define ARBO_OUTPUT_H
+#include <stdint.h>
+
+#include "arbo_c.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* path: the file (the caller composes odir/nData<mcmcRun>.csv); series: one
+ * simulation's series as read by arbo_read_series, `rows` rows of S
+ * serotypes; sim_run 0 creates the file, any other appends. */
+ARBO_API int arbo_write_ndata(const char* path, const ArboSeries* series, uint32_t rows, uint32_t serotypes,
+                              uint32_t sim_run);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARBO_OUTPUT_H */

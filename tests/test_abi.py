@@ -9,10 +9,11 @@ import subprocess
 
 import pytest
 
-from arbo_gpu_fit_amd import abi, mcmc, surrogate
+from arbo_gpu_fit_amd import abi, mcmc, output, surrogate
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = {"arbo_c.h": abi.PROTOTYPES, "arbo_mcmc.h": mcmc.PROTOTYPES, "arbo_surrogate.h": surrogate.PROTOTYPES}
+HEADERS = {"arbo_c.h": abi.PROTOTYPES, "arbo_mcmc.h": mcmc.PROTOTYPES, "arbo_surrogate.h": surrogate.PROTOTYPES,
+           "arbo_output.h": output.PROTOTYPES}
 
 
 def declared_functions(header=None):
@@ -39,6 +40,7 @@ def test_hip_library_exports_every_symbol():
     abi.bind(lib)
     mcmc.bind(lib)
     surrogate.bind(lib)
+    output.bind(lib)
     out = subprocess.run(["nm", "-D", "--defined-only", abi.HIP_LIB_PATH], capture_output=True, text=True).stdout
     exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
     assert {n for n in exported if n.startswith("arbo_")} == set(declared_functions())

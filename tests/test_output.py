@@ -1,0 +1,34 @@
+"""The nData file (CPU): arbo_write_ndata against a restatement of the
+reference's writeIncidence (writeoutput.cpp:13-98), text for text, for a
+run that creates the file and a run that appends to it."""
+import numpy as np
+
+from arbo_gpu_fit_amd.output import write_ndata
+
+
+def reference_text(runs, S):
+    out = "t,popSize" + "".join(f",incidence_DENV{v + 1}" for v in range(S))
+    out += "".join(f",prevalence_DENV{v + 1}" for v in range(S))
+    out += "".join(f",prevalence_DENV{v + 1}_random_community" for v in range(S)) + ",simRun"
+    for r, ser in enumerate(runs):
+        for t in range(ser["n_count"].size):
+            out += f"\n{t},{ser['n_count'][t]}"
+            out += "".join(f",{ser['n_incidence'][t, v]}" for v in range(S))
+            out += "".join(f",{ser['n_infected'][t, v]}" for v in range(S))
+            out += "".join(f",{ser['n_one_subpop'][t, v]}" for v in range(S))
+            out += f",{r}"
+    return out
+
+
+def test_ndata_matches_reference_layout(tmp_path):
+    rng = np.random.default_rng(1)
+    S, rows = 3, 9
+    runs = []
+    for _ in range(2):
+        runs.append({"n_count": rng.integers(0, 10**6, rows), "m_count": rng.integers(0, 10**6, rows),
+                     "n_infected": rng.integers(0, 5000, (rows, S)), "n_incidence": rng.integers(0, 500, (rows, S)),
+                     "n_one_subpop": rng.integers(0, 50, (rows, S))})
+    path = str(tmp_path / "nData0.csv")
+    for r, ser in enumerate(runs):
+        write_ndata(path, ser, r)
+    assert open(path).read() == reference_text(runs, S)
