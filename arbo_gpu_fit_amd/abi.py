@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_SEROTYPES = 4
 MAX_COMMUNITY_HUMANS = 8192
 MAX_COMMUNITY_SLOTS = 8192
@@ -19,6 +19,9 @@ E_DEVICE = -2
 E_ALLOC = -3
 E_STATE = -4
 E_CAPACITY = -5
+
+COUNT_POISSON = 0
+COUNT_STOCH_ROUND = 1
 
 F_GRAPH = 1
 F_TIMING = 2
@@ -45,6 +48,7 @@ class Params(C.Structure):
         ("m_ratio", _f32), ("season_amp", _f32), ("season_phase", _f32),
         ("seed_frac_h", _f32), ("seed_frac_m", _f32),
         ("immunity_mode", _u32), ("sero_prev", _f32),
+        ("no_distance", _f32), ("count_law", _u32),
     ]
 
     def as_dict(self) -> dict:

@@ -105,6 +105,8 @@ def base_params(**over) -> abi.Params:
     p.seed_frac_m = 0.001
     p.immunity_mode = 0
     p.sero_prev = 0.0
+    p.no_distance = 0.0
+    p.count_law = abi.COUNT_POISSON
     for k, v in over.items():
         setattr(p, k, v)
     return p
@@ -112,15 +114,18 @@ def base_params(**over) -> abi.Params:
 
 def reference_params(**over) -> abi.Params:
     """The reference's own defaults (SimulationParam.cpp:11-47): bi-Weibull
-    humans, Weibull(4, 23 d) mosquitoes, von Neumann lattice with unit weights,
-    one serotype."""
+    humans, Weibull(4, 23 d) mosquitoes, von Neumann lattice with unit weights
+    and no self entry, one serotype, 75% of events staying at the source
+    (noDistance) and the stochastic-rounding event counts of
+    nepidemic.cu:64-104."""
     p = base_params(
         n_serotypes=1, beta_hm=0.25 * 0.5, beta_mh=0.25 * 0.5, long_distance=0.01,
         external_infection=5.0, neighbourhood=4, include_self=0, dispersal_scale=0.0,
         iip_min=7, iip_max=7, recovery=4.0, eip=6.0,
         h_infant_shape=0.4, h_infant_scale=1.0 / 100000.0 / 365.0, h_loc=8.0 * 365.0,
         h_scale=75.0 * 365.0, h_shape=6.0, m_scale=23.0, m_shape=4.0,
-        m_ratio=1.5, season_amp=0.0, seed_frac_h=0.0, seed_frac_m=0.0)
+        m_ratio=1.5, season_amp=0.0, seed_frac_h=0.0, seed_frac_m=0.0,
+        no_distance=0.75, count_law=abi.COUNT_STOCH_ROUND)
     for k, v in over.items():
         setattr(p, k, v)
     return p

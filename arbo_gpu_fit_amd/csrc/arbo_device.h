@@ -34,10 +34,12 @@ struct CommDesc {
     uint32_t in_off;      // global offset of its in-edges
     uint32_t in_len;
     float base;           // m_ratio * n (f32)
-    uint32_t flags;       // kCommIntro: the simulation has external introductions
+    uint32_t flags;       // kCommIntro | kCommStay | self column << kCommSelfShift
 };
 
-constexpr uint32_t kCommIntro = 1u;
+constexpr uint32_t kCommIntro = 1u;      // the simulation has external introductions
+constexpr uint32_t kCommStay = 2u;       // a fraction no_distance of the events stays at the source
+constexpr uint32_t kCommSelfShift = 8u;  // bits 8..11: the source's own column on its CSR row
 
 struct SimDesc {
     uint32_t k0, k1;              // Philox key = 64-bit seed
@@ -54,6 +56,8 @@ struct SimDesc {
     uint32_t seed_h, seed_m;
     uint32_t death_m_max;         // largest death age the mosquito table can return
     uint32_t death_m_guide_off;   // [256] smallest death age of each life_bucket() of draws
+    uint32_t count_law;           // ARBO_COUNT_POISSON or ARBO_COUNT_STOCH_ROUND
+    float no_distance;            // fraction of events staying at the source (communities with kCommStay)
     uint64_t series_off;          // this simulation's block in the series array
 };
 

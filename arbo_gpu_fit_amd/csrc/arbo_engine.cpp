@@ -338,6 +338,8 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         sd.seed_m = t.seed_m;
         sd.death_m_max = t.death_m_max;
         sd.death_m_guide_off = pool_add(table_pool, seen_u32, t.death_m_guide);
+        sd.count_law = p.count_law;
+        sd.no_distance = p.no_distance;
         death_m_lds = std::max(death_m_lds, t.death_m_max + 1);
         sd.series_off = sh.series_off;
 
@@ -377,6 +379,12 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
             cd.in_len = t.in_ptr[c + 1] - t.in_ptr[c];
             cd.base = t.base[c];
             cd.flags = params[i].external_infection > 0.0f ? kCommIntro : 0u;
+            if (params[i].no_distance > 0.0f) {   // the column of the row that is the community itself
+                uint32_t self = 0;
+                while (self < cd.row_len && t.to[t.row_ptr[c] + self] != c) ++self;
+                if (self == cd.row_len) return fail(e, ARBO_E_INVALID, "community missing from its own CSR row");
+                cd.flags |= kCommStay | (self << kCommSelfShift);
+            }
             comm.push_back(cd);
             h_tot += cd.n;
             m_tot += cd.cap;

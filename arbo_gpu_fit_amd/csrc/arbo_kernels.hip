@@ -622,11 +622,30 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
             if (I > 0) {
                 const float lam = dir == DIR_HM ? (float)Mc / (float)cd.n * sd.beta_hm * (float)I
                                                 : sd.beta_mh * (float)I;
-                const float mu_loc = lam * (1.0f - sd.omega);
-                const float mu_ld = lam * sd.omega;
-                fx = (unsigned long long)floor((double)mu_ld * 16777216.0);
-                const Stream st{key, cd.c, t, ctr_tag(P_NLOC, dir, s, 0)};
-                n = poisson((double)mu_loc, st);
+                const bool staying = (cd.flags & kCommStay) != 0;
+                uint32_t stay = 0;
+                if (sd.count_law == ARBO_COUNT_STOCH_ROUND) {   // nepidemic.cu:67-104
+                    const U4 r = draw(key, cd.c, t, ctr_tag(P_NSR, dir, s, 0), 0);
+                    uint32_t all = stoch_round(lam, r.x);
+                    if (staying) stay = stoch_round((float)all * sd.no_distance, r.y);
+                    all -= stay;
+                    const uint32_t ld = stoch_round((float)all * sd.omega, r.z);
+                    n = all - ld;
+                    fx = ld;   // an event count: the long-distance kernel sends exactly these
+                } else {
+                    const float go = staying ? lam * (1.0f - sd.no_distance) : lam;
+                    const float mu_loc = go * (1.0f - sd.omega);
+                    const float mu_ld = go * sd.omega;
+                    fx = (unsigned long long)floor((double)mu_ld * 16777216.0);
+                    const Stream st{key, cd.c, t, ctr_tag(P_NLOC, dir, s, 0)};
+                    n = poisson((double)mu_loc, st);
+                    if (staying) {
+                        const Stream ss{key, cd.c, t, ctr_tag(P_NSTAY, dir, s, 0)};
+                        stay = poisson((double)(lam * sd.no_distance), ss);
+                    }
+                }
+                // staying events join the count of the source's own column
+                if (stay) atomicAdd(&s_hist[lane * 9 + ((cd.flags >> kCommSelfShift) & 15u)], stay);
             }
         }
         s_n[lane] = n;
@@ -1004,8 +1023,12 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v) {
         uint32_t L = 0;
         if (s < sd.S) {
             const unsigned long long fx = v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + tid];
-            const Stream st{key, INIT_DAY, t, ctr_tag(P_NLD, dir, s, 0)};
-            L = poisson((double)fx * 0x1p-24, st);
+            if (sd.count_law == ARBO_COUNT_STOCH_ROUND) {
+                L = (uint32_t)fx;   // the sources' own long-distance counts
+            } else {
+                const Stream st{key, INIT_DAY, t, ctr_tag(P_NLD, dir, s, 0)};
+                L = poisson((double)fx * 0x1p-24, st);
+            }
         }
         s_L[tid] = L;
     }

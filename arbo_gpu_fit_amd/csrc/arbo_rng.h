@@ -62,6 +62,8 @@ enum Purpose : uint32_t {
     P_M_EIP = 27,      // newly infected mosquito: extrinsic incubation
     P_INTRO = 28,      // external introductions (x3 = 0 count, 1+a attempt a)
     P_LD_DEST = 29,    // destination of the l-th long-distance event (x0 = l)
+    P_NSR = 30,        // stochastic-rounding event count of a source and its splits
+    P_NSTAY = 31,      // Poisson count of a source's events that stay at the source
 };
 
 // Direction of a transmission event.

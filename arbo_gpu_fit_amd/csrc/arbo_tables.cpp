@@ -81,6 +81,8 @@ std::string validate(const ArboShape& sh, const ArboParams& p) {
     if (!(p.external_infection >= 0.0f)) return "external_infection must be >= 0";
     if (!(p.seed_frac_h >= 0.0f && p.seed_frac_h <= 1.0f) || !(p.seed_frac_m >= 0.0f && p.seed_frac_m <= 1.0f)) return "seed fractions must be in [0,1]";
     if (p.immunity_mode > 1 || !(p.sero_prev >= 0.0f && p.sero_prev <= 1.0f)) return "bad immunity settings";
+    if (p.count_law > ARBO_COUNT_STOCH_ROUND) return "count_law must be ARBO_COUNT_POISSON or ARBO_COUNT_STOCH_ROUND";
+    if (!(p.no_distance >= 0.0f && p.no_distance <= 1.0f)) return "no_distance must be in [0,1]";
     if (!(p.dispersal_scale >= 0.0f)) return "dispersal_scale must be >= 0";
     const uint64_t C = (uint64_t)sh.grid_w * sh.grid_h;
     uint64_t nh = 0, nm = 0;
@@ -112,15 +114,19 @@ static void build_network(const ArboParams& p, SimTables& t) {
                     const int xx = (int)x + dx, yy = (int)y + dy;
                     if (xx < 0 || yy < 0 || xx >= (int)t.W || yy >= (int)t.H) continue;
                     const int m = (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy);
-                    if (m == 0 && !p.include_self) continue;
+                    // the community itself: a dispersal target with include_self, and
+                    // the column its staying events (no_distance) are counted in,
+                    // with weight 0 when it is no dispersal target
+                    if (m == 0 && !p.include_self && !(p.no_distance > 0.0f)) continue;
                     if (m == 2 && p.neighbourhood == 4) continue;
                     const double dist = (m == 0) ? 0.0 : ((m == 1) ? 1.0 : 1.4142135623730951);
                     col[n] = (uint32_t)yy * t.W + (uint32_t)xx;
                     w[n] = (p.dispersal_scale > 0.0f) ? dexp(-dist / (double)p.dispersal_scale) : 1.0;
+                    if (m == 0 && !p.include_self) w[n] = 0.0;
                     ++n;
                 }
             }
-            if (n == 0) {  // isolated community: local events stay at home
+            if (n == 0 || (n == 1 && w[0] == 0.0)) {  // isolated community: local events stay at home
                 col[0] = y * t.W + x;
                 w[0] = 1.0;
                 n = 1;

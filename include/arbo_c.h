@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define ARBO_ABI_VERSION 1u
+#define ARBO_ABI_VERSION 2u
 #define ARBO_MAX_SEROTYPES 4u
 #define ARBO_MAX_COMMUNITY_HUMANS 8192u
 #define ARBO_MAX_COMMUNITY_SLOTS 8192u
@@ -95,7 +95,22 @@ typedef struct ArboParams {
        1 = fraction drawn Beta(2,5) per community and serotype */
     uint32_t immunity_mode;
     float sero_prev;
+    /* fraction of a source's events that stay in the source community before
+       the long-distance split (noDistance, nepidemic.cu:98-100); 0 when the
+       community is on its own CSR row (include_self) */
+    float no_distance;
+    /* law of a source's daily event count (nepidemic.cu:64-104), rate lam:
+       ARBO_COUNT_POISSON: stay ~ Poisson(lam no_distance), long-distance
+         ~ Poisson(lam (1 - no_distance) long_distance), the rest local, all
+         independent Poisson;
+       ARBO_COUNT_STOCH_ROUND: the reference's rule, n = stochRound(lam),
+         stay = stochRound(n no_distance), long-distance =
+         stochRound((n - stay) long_distance), the rest local */
+    uint32_t count_law;
 } ArboParams;
+
+#define ARBO_COUNT_POISSON 0u
+#define ARBO_COUNT_STOCH_ROUND 1u
 
 /* Shape of one simulation: a grid_w x grid_h non-wrapping lattice of
  * communities (community id = y*grid_w + x) and the humans of each. */

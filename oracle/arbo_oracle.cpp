@@ -62,7 +62,7 @@ struct Sim {
     std::vector<uint64_t> h_off, m_off;
     uint64_t NH = 0, NM = 0;
     std::vector<float> base;
-    std::vector<uint32_t> row_ptr, to, thr, alias;
+    std::vector<uint32_t> row_ptr, to, thr, alias, self_col;
     std::vector<uint32_t> in_ptr, in_src, in_slot;
     std::vector<uint32_t> Kh, Km, Thy, Tmd;
     std::vector<float> dens;
@@ -178,6 +178,7 @@ uint32_t death_age_given(const std::vector<uint32_t>& K, uint32_t a0, uint32_t r
 void build_network(Sim& s) {
     const ArboParams& p = s.p;
     s.row_ptr.assign(1, 0);
+    s.self_col.clear();
     s.to.clear(); s.thr.clear(); s.alias.clear();
     for (uint32_t y = 0; y < s.H; ++y) {
         for (uint32_t x = 0; x < s.W; ++x) {
@@ -188,18 +189,26 @@ void build_network(Sim& s) {
                     int nx = (int)x + dx, ny = (int)y + dy;
                     if (nx < 0 || ny < 0 || nx >= (int)s.W || ny >= (int)s.H) continue;
                     int manh = std::abs(dx) + std::abs(dy);
-                    if (manh == 0 && !p.include_self) continue;
+                    // self: a dispersal target with include_self; with noDistance > 0
+                    // also the column staying events are counted in (weight 0
+                    // when it is no dispersal target)
+                    const bool stay_col = p.no_distance > 0.0f;
+                    if (manh == 0 && !p.include_self && !stay_col) continue;
                     if (p.neighbourhood == 4 && manh == 2) continue;
                     double dist = manh == 0 ? 0.0 : (manh == 1 ? 1.0 : 1.4142135623730951);
                     double wt = p.dispersal_scale > 0.0f ? exp_d(-dist / (double)p.dispersal_scale) : 1.0;
+                    if (manh == 0 && !p.include_self) wt = 0.0;
+                    if (manh == 0) s.self_col.push_back((uint32_t)cols.size());
                     cols.push_back((uint32_t)ny * s.W + (uint32_t)nx);
                     w.push_back(wt);
                 }
             }
-            if (cols.empty()) {  // isolated: every local event stays home
-                cols.push_back(y * s.W + x);
-                w.push_back(1.0);
+            if (cols.empty() || (cols.size() == 1 && w[0] == 0.0)) {  // isolated: every local event stays home
+                if (cols.empty()) s.self_col.push_back(0);
+                cols.assign(1, y * s.W + x);
+                w.assign(1, 1.0);
             }
+            if (s.self_col.size() < s.row_ptr.size()) s.self_col.push_back(0);   // no own column: never used
             uint32_t n = (uint32_t)cols.size();
             double tot = 0.0;
             for (double v : w) tot = tot + v;
@@ -253,6 +262,8 @@ int validate(ArboEngine* e, const ArboShape& sh, const ArboParams& p) {
     if (!(p.external_infection >= 0.0f)) return fail(e, ARBO_E_INVALID, "external_infection must be >= 0");
     if (!(p.seed_frac_h >= 0.0f && p.seed_frac_h <= 1.0f) || !(p.seed_frac_m >= 0.0f && p.seed_frac_m <= 1.0f)) return fail(e, ARBO_E_INVALID, "seed fractions must be in [0,1]");
     if (p.immunity_mode > 1 || !(p.sero_prev >= 0.0f && p.sero_prev <= 1.0f)) return fail(e, ARBO_E_INVALID, "bad immunity settings");
+    if (p.count_law > ARBO_COUNT_STOCH_ROUND) return fail(e, ARBO_E_INVALID, "bad count_law");
+    if (!(p.no_distance >= 0.0f && p.no_distance <= 1.0f)) return fail(e, ARBO_E_INVALID, "no_distance must be in [0,1]");
     if (!(p.dispersal_scale >= 0.0f)) return fail(e, ARBO_E_INVALID, "dispersal_scale must be >= 0");
     return ARBO_OK;
 }
@@ -388,6 +399,40 @@ void record_row(Sim& s, uint32_t t) {
     }
 }
 
+// Daily event count of source community c, direction dir and serotype v with
+// rate lam > 0 (nVisitingInfected, nepidemic.cu:49-136; mVisitingInfected,
+// mepidemic.cu:48-132).  Returns the events dispersed over the CSR row; stay
+// gets those staying at the source (only when nd > 0) and fx the
+// long-distance part: under ARBO_COUNT_STOCH_ROUND an event count, under
+// ARBO_COUNT_POISSON the rate in 2^-24 units (the simulation's sum is drawn
+// once).
+//   stochastic rounding (nepidemic.cu:67-70, 98-104): n = stochRound(lam),
+//   stay = stochRound(n * noDistance), long-distance =
+//   stochRound((n - stay) * longDistance), the rest dispersed;
+//   Poisson: stay ~ Poisson(lam noDistance), and of the rest
+//   g = lam (1 - noDistance): dispersed ~ Poisson(g (1 - longDistance)),
+//   long-distance ~ Poisson(g longDistance), all independent.
+uint32_t source_counts(uint32_t law, float lam, float omega, float nd, Seed key, uint32_t c, uint32_t t,
+                       uint32_t dir, uint32_t v, uint64_t& fx, uint32_t& stay) {
+    const bool staying = nd > 0.0f;
+    stay = 0;
+    if (law == ARBO_COUNT_STOCH_ROUND) {
+        W4 r = rnd(key, c, t, tag(NSR, dir, v, 0), 0);
+        uint32_t all = sround(lam, r.v[0]);
+        if (staying) stay = sround((float)all * nd, r.v[1]);
+        all -= stay;
+        const uint32_t ld = sround((float)all * omega, r.v[2]);
+        fx = ld;
+        return all - ld;
+    }
+    const float go = staying ? lam * (1.0f - nd) : lam;
+    const float mu_loc = go * (1.0f - omega);
+    const float mu_ld = go * omega;
+    fx = (uint64_t)std::floor((double)mu_ld * 16777216.0);
+    if (staying) stay = poisson_d((double)(lam * nd), key, c, t, tag(NSTAY, dir, v, 0));
+    return poisson_d((double)mu_loc, key, c, t, tag(NLOC, dir, v, 0));
+}
+
 // Phase A of day t for one community: demography, counters, event counts.
 void phase_a(Sim& s, uint32_t c, uint32_t t) {
     const ArboParams& p = s.p;
@@ -479,10 +524,10 @@ void phase_a(Sim& s, uint32_t c, uint32_t t) {
                 if (dir == HM && Ih[v] > 0) lam = (float)Mc / (float)s.N[c] * p.beta_hm * (float)Ih[v];
                 if (dir == MH && Im[v] > 0) lam = p.beta_mh * (float)Im[v];
                 if (lam > 0.0f) {
-                    float mu_loc = lam * (1.0f - p.long_distance);
-                    float mu_ld = lam * p.long_distance;
-                    fx = (uint64_t)std::floor((double)mu_ld * 16777216.0);
-                    uint32_t n = poisson_d((double)mu_loc, s.key, c, t, tag(NLOC, dir, v, 0));
+                    uint32_t stay;
+                    const uint32_t n = source_counts(p.count_law, lam, p.long_distance, p.no_distance, s.key, c, t,
+                                                     dir, v, fx, stay);
+                    cnt[(dir * 4 + v) * 9 + s.self_col[c]] += stay;
                     // event k takes word k%4 of block k/4; the product word * row
                     // length gives the column (high half) and the alias test
                     // uniform (low half)
@@ -589,8 +634,9 @@ void long_distance(Sim& s, uint32_t t) {
     for (uint32_t dir = 0; dir < 2; ++dir)
         for (uint32_t v = 0; v < s.S; ++v) {
             const uint32_t k = dir * 4 + v;
-            const double mu = (double)s.lam[k] * 0x1p-24;
-            const uint32_t L = poisson_d(mu, s.key, DAY_INIT, t, tag(NLD, dir, v, 0));
+            const uint32_t L = s.p.count_law == ARBO_COUNT_STOCH_ROUND
+                                   ? (uint32_t)s.lam[k]   // the sources' own long-distance counts
+                                   : poisson_d((double)s.lam[k] * 0x1p-24, s.key, DAY_INIT, t, tag(NLD, dir, v, 0));
             for (uint32_t l = 0; l < L; ++l) {
                 const uint32_t d = range(rnd(s.key, l, t, tag(LD_DEST, dir, v, 0), 0).v[0], s.C);
                 s.nld[(size_t)d * 8 + k]++;
@@ -838,5 +884,10 @@ uint32_t oracle_poisson(double mu, uint64_t seed, uint32_t c0, uint32_t c1, uint
     return poisson_d(mu, k, c0, c1, c2);
 }
 uint32_t oracle_sround(float x, uint32_t r) { return sround(x, r); }
+uint32_t oracle_source_counts(uint32_t law, float lam, float omega, float nd, uint64_t seed, uint32_t c,
+                              uint32_t t, uint32_t dir, uint32_t v, uint64_t* fx, uint32_t* stay) {
+    Seed k{(uint32_t)seed, (uint32_t)(seed >> 32)};
+    return source_counts(law, lam, omega, nd, k, c, t, dir, v, *fx, *stay);
+}
 
 }  // extern "C"

@@ -46,7 +46,7 @@ enum : uint32_t {
     INIT_H = 1, INIT_M = 2, INIT_MT = 3, IMMUNE = 4,
     H_BIRTH = 10, H_REC = 11, M_BIRTH = 12, M_TARGET = 13,
     NLOC = 20, DEST = 21, NLD = 22, ROT = 23, TGT = 24, TGT_LD = 25,
-    H_IIP = 26, M_EIP = 27, INTRO = 28, LD_DEST = 29
+    H_IIP = 26, M_EIP = 27, INTRO = 28, LD_DEST = 29, NSR = 30, NSTAY = 31
 };
 enum : uint32_t { HM = 0, MH = 1 };
 constexpr uint32_t DAY_INIT = 0xFFFFFFFFu;

@@ -5,14 +5,15 @@ the 10x10-lattice workload (1e5 humans, 365 days, in full), the reference's own 
 (von Neumann lattice, unit weights, one serotype, bi-Weibull humans,
 Weibull(4, 23 d) mosquitoes, external introductions), isolated and ragged
 communities (1 human; the maximum of 8,192 humans and slots; a community with
-no mosquito slots), large Poisson means (the PTRS branch), Beta(2,5) immunity
-and two simulations of the 256-simulation batch.
+no mosquito slots), large Poisson means (the PTRS branch), Beta(2,5) immunity,
+both event-count laws with and without a staying fraction, and two
+simulations of the 256-simulation batch.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from arbo_gpu_fit_amd import configs
+from arbo_gpu_fit_amd import abi, configs
 from arbo_gpu_fit_amd.engine import SimSpec
 
 
@@ -50,6 +51,24 @@ def case_high_lambda():
     return [SimSpec(3, 3, np.full(9, 2000, np.uint32), p, 15)], 60
 
 
+def case_stoch_round_moore():
+    """The reference's stochastic-rounding counts on a Moore lattice with self
+    entries and long-distance events."""
+    p = configs.base_params(count_law=abi.COUNT_STOCH_ROUND, long_distance=0.05, seed_frac_h=0.01,
+                            seed_frac_m=0.01, beta_hm=0.5, beta_mh=0.5)
+    return [SimSpec(6, 5, np.full(30, 700, np.uint32), p, 16)], 90
+
+
+def case_poisson_stay():
+    """Poisson counts with a staying fraction on a lattice without self
+    entries, and one with them (staying and dispersed events share the
+    source's column)."""
+    a = configs.base_params(no_distance=0.3, include_self=0, long_distance=0.02, seed_frac_h=0.01, seed_frac_m=0.01)
+    b = configs.base_params(no_distance=0.6, include_self=1, neighbourhood=4, dispersal_scale=0.0,
+                            count_law=abi.COUNT_STOCH_ROUND, seed_frac_h=0.01, seed_frac_m=0.01)
+    return [SimSpec(5, 5, np.full(25, 600, np.uint32), a, 17), SimSpec(4, 6, np.full(24, 400, np.uint32), b, 18)], 90
+
+
 def case_batch_pair():
     sims = [configs.batch_sim(i) for i in range(24)]
     sims.sort(key=lambda s: int(s.humans.sum()))
@@ -63,6 +82,8 @@ CASES = {
     "ragged": case_ragged,
     "beta25": case_beta25,
     "high_lambda": case_high_lambda,
+    "stoch_round_moore": case_stoch_round_moore,
+    "poisson_stay": case_poisson_stay,
     "batch_pair": case_batch_pair,
 }
 

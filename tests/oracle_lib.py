@@ -31,6 +31,10 @@ def load_oracle():
     lib.oracle_poisson.restype = C.c_uint32
     lib.oracle_sround.argtypes = [C.c_float, C.c_uint32]
     lib.oracle_sround.restype = C.c_uint32
+    lib.oracle_source_counts.argtypes = [C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_uint64, C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint32)]
+    lib.oracle_source_counts.restype = C.c_uint32
     lib.oracle_tables.argtypes = [abi.EnginePtr, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32,
                                   C.POINTER(C.c_uint32)]
     lib.oracle_tables.restype = C.c_int
