@@ -69,6 +69,13 @@ class State(C.Structure):
                 ("m_word", _u16p), ("m_next", _u16p)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("census_unroll", _u32), ("early_mosq", _u32), ("xcd_order", _u32), ("ld_split", _u32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 EnginePtr = C.c_void_p
 
 # name -> (restype, argtypes); the exported symbols of include/arbo_c.h
@@ -86,6 +93,10 @@ PROTOTYPES = {
     "arbo_set_day": (C.c_int, [EnginePtr, _u32]),
     "arbo_kernel_time": (C.c_int, [EnginePtr, _u32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "arbo_reset_timing": (C.c_int, [EnginePtr]),
+    "arbo_tuning_candidates": (C.c_int, [C.POINTER(Tuning), _u32, _u32p]),
+    "arbo_set_tuning": (C.c_int, [EnginePtr, C.POINTER(Tuning)]),
+    "arbo_get_tuning": (C.c_int, [EnginePtr, C.POINTER(Tuning)]),
+    "arbo_autotune": (C.c_int, [EnginePtr, _u32, C.POINTER(Tuning), C.POINTER(C.c_double)]),
     "arbo_destroy": (None, [EnginePtr]),
     "arbo_last_error": (C.c_char_p, [EnginePtr]),
 }

@@ -95,9 +95,11 @@ struct EngineView {
     uint32_t* nld;                 // [G][8] long-distance arrivals of the current day
     uint32_t* active;              // [G] 1 when events arrive today (set by the sources, cleared by the
                                    // destination's event kernel)
-    uint32_t* hsum;                // [G][4] human summary: earliest day any member has an event (0 =
-                                   // unknown), infectious counts by serotype (two 16-bit pairs), pad
-    unsigned long long* lam;       // [2 parity][n_sims][8]  long-distance rate, 2^-24 fixed point
+    uint32_t* hsum;                // [G][4] summary: earliest day any human has an event (0 = unknown),
+                                   // infectious humans by serotype (two 16-bit pairs), mosquito scan
+                                   // limit + 1 in 8-slot vectors (0 = unknown)
+    unsigned long long* lam;       // [2 parity][n_sims][8] long-distance events of the day: the rate in
+                                   // 2^-24 units (Poisson counts) or the count (stochastic rounding)
     uint32_t* series;
     const uint32_t* csr_inbox;     // per CSR entry: inbox row (destination * 9 + in-edge index)
     const uint32_t* csr_thr;       // bits 0..24 threshold, 27..31 alias slot
@@ -111,5 +113,34 @@ struct EngineView {
     uint32_t max_days;
     uint32_t death_m_lds;          // entries of the largest mosquito death table (LDS staging)
 };
+
+// Launch knobs of the day kernels.  None of them changes a result: every
+// random draw is keyed by model entities, never by workgroup or lane.
+struct Tuning {
+    uint32_t unroll;     // demography: 16-byte census loads in flight per lane
+    uint32_t early_m;    // demography: of the first batch, mosquito loads issued before the humans
+    uint32_t xcd;        // bit 0 demography, bit 1 events: the workgroups one XCD receives take a
+                         // contiguous range of communities
+    uint32_t ld_split;   // long-distance kernel: workgroups per simulation
+};
+constexpr uint32_t kDemographyShapes[][2] = {{4, 4}, {4, 2}, {8, 4}, {2, 2}};   // (unroll, early_m)
+constexpr uint32_t kMaxLdSplit = 64;
+constexpr uint32_t kXcds = 8;   // gfx950: workgroups go to the 8 XCDs in turn
+
+// Community of workgroup b of n when the workgroups of XCD b % 8 take the
+// contiguous range [x q + min(x, r), ...) of q + (x < r) communities, where
+// n = 8 q + r.  A bijection of [0, n).
+__host__ __device__ inline uint32_t xcd_block(uint32_t b, uint32_t n) {
+    const uint32_t x = b % kXcds, i = b / kXcds, q = n / kXcds, r = n % kXcds;
+    return x * q + (x < r ? x : r) + i;
+}
+
+hipError_t launch_init(const EngineView& v, hipStream_t st);
+hipError_t launch_demography(const EngineView& v, const Tuning& tu, hipStream_t st);
+hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st);
+hipError_t launch_longdist(const EngineView& v, const Tuning& tu, hipStream_t st);
+hipError_t launch_advance(uint32_t* day, hipStream_t st);
+hipError_t launch_fill(uint32_t* p, uint64_t n, uint32_t value, hipStream_t st);
+bool tuning_supported(const Tuning& tu);
 
 }  // namespace arbo

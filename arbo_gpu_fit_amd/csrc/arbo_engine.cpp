@@ -19,15 +19,6 @@
 #include "arbo_device.h"
 #include "arbo_tables.h"
 
-namespace arbo {
-hipError_t launch_init(const EngineView& v, hipStream_t st);
-hipError_t launch_demography(const EngineView& v, hipStream_t st);
-hipError_t launch_events(const EngineView& v, hipStream_t st);
-hipError_t launch_longdist(const EngineView& v, hipStream_t st);
-hipError_t launch_advance(uint32_t* day, hipStream_t st);
-hipError_t launch_fill(uint32_t* p, uint64_t n, uint32_t value, hipStream_t st);
-}  // namespace arbo
-
 using namespace arbo;
 
 namespace {
@@ -79,6 +70,7 @@ struct ArboEngine {
     // timing
     std::vector<hipEvent_t> ev_pool, ev_used;
     double kernel_ms[2] = {0.0, 0.0};
+    Tuning tuning{4, 4, 0, 8};
     uint64_t kernel_launches[2] = {0, 0};
 };
 
@@ -181,9 +173,9 @@ int drain_timing(ArboEngine* e) {
 }
 
 int launch_day(ArboEngine* e) {
-    HIP_TRY(e, launch_demography(e->view, e->stream));
-    HIP_TRY(e, launch_longdist(e->view, e->stream));
-    HIP_TRY(e, launch_events(e->view, e->stream));
+    HIP_TRY(e, launch_demography(e->view, e->tuning, e->stream));
+    HIP_TRY(e, launch_longdist(e->view, e->tuning, e->stream));
+    HIP_TRY(e, launch_events(e->view, e->tuning, e->stream));
     HIP_TRY(e, launch_advance(e->day_ctr.p, e->stream));
     return ARBO_OK;
 }
@@ -192,10 +184,10 @@ int launch_day_timed(ArboEngine* e) {
     hipEvent_t a, b, c;
     int rc = record_timing_event(e, &a);
     if (rc) return rc;
-    HIP_TRY(e, launch_demography(e->view, e->stream));
+    HIP_TRY(e, launch_demography(e->view, e->tuning, e->stream));
     if ((rc = record_timing_event(e, &b))) return rc;
-    HIP_TRY(e, launch_longdist(e->view, e->stream));
-    HIP_TRY(e, launch_events(e->view, e->stream));
+    HIP_TRY(e, launch_longdist(e->view, e->tuning, e->stream));
+    HIP_TRY(e, launch_events(e->view, e->tuning, e->stream));
     if ((rc = record_timing_event(e, &c))) return rc;
     HIP_TRY(e, launch_advance(e->day_ctr.p, e->stream));
     // bound the number of outstanding events
@@ -487,6 +479,105 @@ int arbo_batch_step(ArboEngine* e, uint32_t n_days) {
         if (rc) return rc;
     }
     e->day += n_days;
+    return ARBO_OK;
+}
+
+namespace {
+
+Tuning from_abi(const ArboTuning& t) { return Tuning{t.census_unroll, t.early_mosq, t.xcd_order, t.ld_split}; }
+ArboTuning to_abi(const Tuning& t) { return ArboTuning{t.unroll, t.early_m, t.xcd, t.ld_split}; }
+
+int apply_tuning(ArboEngine* e, const Tuning& tu) {
+    if (!tuning_supported(tu)) return fail(e, ARBO_E_INVALID, "unsupported tuning");
+    HIP_TRY(e, hipSetDevice(e->device));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    e->tuning = tu;
+    release_graph(e);   // the graph holds the old kernels
+    if (e->ready && (e->flags & ARBO_F_GRAPH)) return build_graph(e);
+    return ARBO_OK;
+}
+
+}  // namespace
+
+int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n) {
+    if (!n) return ARBO_E_INVALID;
+    uint32_t k = 0;
+    for (const auto& d : kDemographyShapes)
+        for (uint32_t x = 0; x <= 3; x += 3) {
+            if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8};
+            ++k;
+        }
+    *n = k;
+    return ARBO_OK;
+}
+
+int arbo_set_tuning(ArboEngine* e, const ArboTuning* t) {
+    if (!e || !t) return ARBO_E_INVALID;
+    return apply_tuning(e, from_abi(*t));
+}
+
+int arbo_get_tuning(const ArboEngine* e, ArboTuning* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    *out = to_abi(e->tuning);
+    return ARBO_OK;
+}
+
+int arbo_autotune(ArboEngine* e, uint32_t days_each, ArboTuning* chosen, double* ms_per_day) {
+    if (!e || days_each == 0) return ARBO_E_INVALID;
+    if (!e->ready) return fail(e, ARBO_E_STATE, "arbo_batch_init has not been called");
+    uint32_t n = 0;
+    arbo_tuning_candidates(nullptr, 0, &n);
+    std::vector<ArboTuning> cand(n);
+    arbo_tuning_candidates(cand.data(), n, &n);
+    if ((uint64_t)e->day + (uint64_t)n * days_each > e->max_days)
+        return fail(e, ARBO_E_CAPACITY, "autotuning needs candidates x days_each days of series capacity");
+    HIP_TRY(e, hipSetDevice(e->device));
+    // Each candidate runs days_each days with events around the demography
+    // kernel and around the rest of the day.  The demography knobs come from
+    // the candidate with the fastest demography, the event kernel's order bit
+    // from the one with the fastest rest of the day.
+    const uint32_t ne = 3 * days_each;
+    std::vector<hipEvent_t> ev(ne, nullptr);
+    int rc = ARBO_OK;
+    for (uint32_t i = 0; i < ne && rc == ARBO_OK; ++i)
+        if (hipEventCreate(&ev[i]) != hipSuccess) rc = fail(e, ARBO_E_DEVICE, "autotune: hipEventCreate");
+    double best_a = 1e300, best_b = 1e300;
+    Tuning keep_a = e->tuning, keep_b = e->tuning;
+    for (uint32_t k = 0; k < n && rc == ARBO_OK; ++k) {
+        const Tuning tu = from_abi(cand[k]);
+        if ((rc = apply_tuning(e, tu))) break;
+        hipError_t st = hipSuccess;
+        for (uint32_t d = 0; d < days_each && st == hipSuccess; ++d) {
+            st = hipEventRecord(ev[3 * d], e->stream);
+            if (st == hipSuccess) st = launch_demography(e->view, e->tuning, e->stream);
+            if (st == hipSuccess) st = hipEventRecord(ev[3 * d + 1], e->stream);
+            if (st == hipSuccess) st = launch_longdist(e->view, e->tuning, e->stream);
+            if (st == hipSuccess) st = launch_events(e->view, e->tuning, e->stream);
+            if (st == hipSuccess) st = hipEventRecord(ev[3 * d + 2], e->stream);
+            if (st == hipSuccess) st = launch_advance(e->day_ctr.p, e->stream);
+        }
+        if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
+        double ta = 0.0, tb = 0.0;
+        for (uint32_t d = 0; d < days_each && st == hipSuccess; ++d) {
+            float x = 0.0f, y = 0.0f;
+            st = hipEventElapsedTime(&x, ev[3 * d], ev[3 * d + 1]);
+            if (st == hipSuccess) st = hipEventElapsedTime(&y, ev[3 * d + 1], ev[3 * d + 2]);
+            ta += x;
+            tb += y;
+        }
+        if (st != hipSuccess) { rc = fail(e, ARBO_E_DEVICE, std::string("autotune: ") + hipGetErrorString(st)); break; }
+        e->day += days_each;
+        if (ta < best_a) { best_a = ta; keep_a = tu; }
+        if (tb < best_b) { best_b = tb; keep_b = tu; }
+    }
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (rc) return rc;
+    Tuning keep = keep_a;
+    keep.xcd = (keep_a.xcd & 1u) | (keep_b.xcd & 2u);
+    if ((rc = apply_tuning(e, keep))) return rc;
+    if (chosen) *chosen = to_abi(keep);
+    if (ms_per_day) *ms_per_day = (best_a + best_b) / days_each;
     return ARBO_OK;
 }
 

@@ -41,12 +41,9 @@ namespace arbo {
 
 constexpr uint32_t kMaxWords = ARBO_MAX_COMMUNITY_SLOTS / 32;  // alive-bitmap words per community
 constexpr uint32_t kWordsPerLane = kMaxWords / kWave;          // 4
-constexpr uint32_t kUnroll = 4;                                // 16-byte loads in flight per lane
-#ifndef ARBO_EARLY_M
-#define ARBO_EARLY_M 4
-#endif
-constexpr uint32_t kEarlyM = ARBO_EARLY_M;                     // mosquito vectors loaded before the humans
-static_assert(kEarlyM >= 1 && kEarlyM <= kUnroll, "early mosquito batch within the first unrolled batch");
+// The demography kernel's census loads in flight per lane (UNROLL) and, of
+// the first batch, the mosquito vectors issued before the human scan
+// (EARLY_M) are template parameters chosen per engine (Tuning).
 constexpr uint32_t kChunk = 256;                               // events per claim pass
 constexpr uint32_t kPerLane = kChunk / kWave;                  // 4
 constexpr uint32_t kHash = 512;                                // claim-table entries (2 x chunk)
@@ -341,8 +338,10 @@ __device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& c
     }
 }
 
+template <uint32_t UNROLL, uint32_t EARLY_M, bool XCD>
 __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
-    const uint32_t g = blockIdx.x, lane = threadIdx.x;
+    static_assert(EARLY_M >= 1 && EARLY_M <= UNROLL, "early mosquito batch within the first unrolled batch");
+    const uint32_t g = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, lane = threadIdx.x;
     const uint32_t t = *v.day;
     const CommDesc cd = v.comm[g];
     // Humans are scanned only on days when one of them may die or change
@@ -359,14 +358,14 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     const uint32_t mlim = (hs.w == 0u || hs.w - 1u > mvec) ? mvec : hs.w - 1u;
     const uint4* w4p = reinterpret_cast<const uint4*>(v.h_word + cd.h_off);
     const uint4* m4p = reinterpret_cast<const uint4*>(v.m_word + cd.m_off);
-    uint4 hb[kUnroll], mb[kEarlyM];
+    uint4 hb[UNROLL], mb[EARLY_M];
 #pragma unroll
-    for (uint32_t u = 0; u < kUnroll; ++u) {
+    for (uint32_t u = 0; u < UNROLL; ++u) {
         const uint32_t q = u * kWave + lane;
         if (scan_h && q < nvec) hb[u] = w4p[q];
     }
 #pragma unroll
-    for (uint32_t u = 0; u < kEarlyM; ++u) {
+    for (uint32_t u = 0; u < EARLY_M; ++u) {
         const uint32_t q = u * kWave + lane;
         if (q < mlim) mb[u] = m4p[q];
     }
@@ -432,19 +431,19 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
         hc.ih23 = hs.z;
     }
 #pragma unroll
-    for (uint32_t u = 0; u < kUnroll; ++u) {
+    for (uint32_t u = 0; u < UNROLL; ++u) {
         const uint32_t q = u * kWave + lane;
         if (scan_h && q < nvec) human_vec(v, cd, sd, key, t, q, hb[u], hc);
     }
-    for (uint32_t q0 = kUnroll * kWave; scan_h && q0 < nvec; q0 += kUnroll * kWave) {
-        uint4 w4[kUnroll];
+    for (uint32_t q0 = UNROLL * kWave; scan_h && q0 < nvec; q0 += UNROLL * kWave) {
+        uint4 w4[UNROLL];
 #pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) {
+        for (uint32_t u = 0; u < UNROLL; ++u) {
             const uint32_t q = q0 + u * kWave + lane;
             if (q < nvec) w4[u] = w4p[q];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kUnroll; ++u) {
+        for (uint32_t u = 0; u < UNROLL; ++u) {
             const uint32_t q = q0 + u * kWave + lane;
             if (q < nvec) human_vec(v, cd, sd, key, t, q, w4[u], hc);
         }
@@ -460,24 +459,24 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
         // per 16-bit half: t mod 4096 plus a 4096 guard, so (t - death day) mod 4096
         // is the half's low 12 bits with no borrow between halves
         const uint32_t T2 = ((t & 0xFFFu) * 0x00010001u) | 0x10001000u;
-        for (uint32_t q0 = 0; q0 < mlim; q0 += kUnroll * kWave) {
-            uint4 raw[kUnroll];
+        for (uint32_t q0 = 0; q0 < mlim; q0 += UNROLL * kWave) {
+            uint4 raw[UNROLL];
             if (q0 == 0) {
 #pragma unroll
-                for (uint32_t u = 0; u < kUnroll; ++u) {
+                for (uint32_t u = 0; u < UNROLL; ++u) {
                     const uint32_t q = u * kWave + lane;
-                    if (u < kEarlyM) raw[u] = mb[u < kEarlyM ? u : 0];
+                    if (u < EARLY_M) raw[u] = mb[u < EARLY_M ? u : 0];
                     else if (q < mlim) raw[u] = m4p[q];
                 }
             } else {
 #pragma unroll
-                for (uint32_t u = 0; u < kUnroll; ++u) {
+                for (uint32_t u = 0; u < UNROLL; ++u) {
                     const uint32_t q = q0 + u * kWave + lane;
                     if (q < mlim) raw[u] = m4p[q];
                 }
             }
 #pragma unroll
-            for (uint32_t u = 0; u < kUnroll; ++u) {
+            for (uint32_t u = 0; u < UNROLL; ++u) {
                 const uint32_t q = q0 + u * kWave + lane;
                 if (q >= mlim) continue;
                 const uint32_t rw[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
@@ -791,8 +790,9 @@ __device__ __forceinline__ uint32_t claimed_rank(const uint32_t* tab, uint32_t k
 
 constexpr uint32_t kEligible = 1u << 31;
 
+template <bool XCD>
 __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
-    const uint32_t g = blockIdx.x, lane = threadIdx.x;
+    const uint32_t g = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, lane = threadIdx.x;
     const uint32_t t = *v.day;
     const CommDesc cd = v.comm[g];
     const uint32_t act = v.active[g];
@@ -1010,10 +1010,9 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
 // of all sources); event l lands on community mulhi(Philox(l), C).  Only the
 // number landing on each destination matters (their targets are drawn at the
 // destination), so the destinations are counted with integer atomics.
-constexpr uint32_t kLdSplit = 8;   // workgroups per simulation
-
-__global__ __launch_bounds__(256) void k_day_longdist(EngineView v) {
-    const uint32_t sim = blockIdx.x / kLdSplit, part = blockIdx.x % kLdSplit, tid = threadIdx.x;
+__global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t split) {
+    // split workgroups per simulation
+    const uint32_t sim = blockIdx.x / split, part = blockIdx.x % split, tid = threadIdx.x;
     const uint32_t t = *v.day;
     const SimDesc sd = v.sims[sim];
     const Key key{sd.k0, sd.k1};
@@ -1037,7 +1036,7 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v) {
     pre[0] = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) pre[k + 1] = pre[k] + s_L[k];
-    for (uint32_t l = part * 256 + tid; l < pre[8]; l += kLdSplit * 256) {
+    for (uint32_t l = part * 256 + tid; l < pre[8]; l += split * 256) {
         uint32_t k = 0;
 #pragma unroll
         for (uint32_t q = 1; q < 8; ++q) k += (l >= pre[q]) ? 1u : 0u;
@@ -1066,18 +1065,42 @@ hipError_t launch_init(const EngineView& v, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_demography(const EngineView& v, hipStream_t st) {
-    hipLaunchKernelGGL(k_day_demography, dim3(v.n_comm), dim3(kWave), (size_t)v.death_m_lds * 4, st, v);
+// Demography variants: (UNROLL, EARLY_M) pairs of kDemographyShapes.
+template <bool XCD>
+static hipError_t launch_demography_x(const EngineView& v, const Tuning& tu, hipStream_t st) {
+    const dim3 grid(v.n_comm), block(kWave);
+    const size_t lds = (size_t)v.death_m_lds * 4;
+    if (tu.unroll == 4 && tu.early_m == 4)
+        hipLaunchKernelGGL((k_day_demography<4, 4, XCD>), grid, block, lds, st, v);
+    else if (tu.unroll == 4 && tu.early_m == 2)
+        hipLaunchKernelGGL((k_day_demography<4, 2, XCD>), grid, block, lds, st, v);
+    else if (tu.unroll == 8 && tu.early_m == 4)
+        hipLaunchKernelGGL((k_day_demography<8, 4, XCD>), grid, block, lds, st, v);
+    else if (tu.unroll == 2 && tu.early_m == 2)
+        hipLaunchKernelGGL((k_day_demography<2, 2, XCD>), grid, block, lds, st, v);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-hipError_t launch_events(const EngineView& v, hipStream_t st) {
-    hipLaunchKernelGGL(k_day_events, dim3(v.n_comm), dim3(kWave), 0, st, v);
+bool tuning_supported(const Tuning& tu) {
+    bool shape = false;
+    for (const auto& d : kDemographyShapes) shape = shape || (d[0] == tu.unroll && d[1] == tu.early_m);
+    return shape && tu.xcd <= 3 && tu.ld_split >= 1 && tu.ld_split <= kMaxLdSplit;
+}
+
+hipError_t launch_demography(const EngineView& v, const Tuning& tu, hipStream_t st) {
+    return (tu.xcd & 1u) ? launch_demography_x<true>(v, tu, st) : launch_demography_x<false>(v, tu, st);
+}
+
+hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st) {
+    if (tu.xcd & 2u) hipLaunchKernelGGL(k_day_events<true>, dim3(v.n_comm), dim3(kWave), 0, st, v);
+    else hipLaunchKernelGGL(k_day_events<false>, dim3(v.n_comm), dim3(kWave), 0, st, v);
     return hipGetLastError();
 }
 
-hipError_t launch_longdist(const EngineView& v, hipStream_t st) {
-    hipLaunchKernelGGL(k_day_longdist, dim3(v.n_sims * kLdSplit), dim3(256), 0, st, v);
+hipError_t launch_longdist(const EngineView& v, const Tuning& tu, hipStream_t st) {
+    hipLaunchKernelGGL(k_day_longdist, dim3(v.n_sims * tu.ld_split), dim3(256), 0, st, v, tu.ld_split);
     return hipGetLastError();
 }
 

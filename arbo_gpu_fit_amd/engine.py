@@ -166,6 +166,35 @@ class Engine:
     def reset_timing(self):
         self._check(self.lib.arbo_reset_timing(self._h))
 
+    def tuning_candidates(self) -> list[dict]:
+        n = C.c_uint32()
+        self._check(self.lib.arbo_tuning_candidates(None, 0, C.byref(n)))
+        arr = (abi.Tuning * n.value)()
+        self._check(self.lib.arbo_tuning_candidates(arr, n.value, C.byref(n)))
+        return [t.as_dict() for t in arr]
+
+    def set_tuning(self, **knobs):
+        """Set launch knobs (census_unroll, early_mosq, xcd_order, ld_split);
+        unnamed knobs keep their values.  Results never depend on them."""
+        t = abi.Tuning()
+        self._check(self.lib.arbo_get_tuning(self._h, C.byref(t)))
+        for k, v in knobs.items():
+            setattr(t, k, v)
+        self._check(self.lib.arbo_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = abi.Tuning()
+        self._check(self.lib.arbo_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, days_each: int = 2) -> tuple[dict, float]:
+        """Time every candidate tuning for days_each simulated days and keep
+        the fastest; returns it and its ms per day.  The run advances by
+        len(tuning_candidates()) * days_each days."""
+        t, ms = abi.Tuning(), C.c_double()
+        self._check(self.lib.arbo_autotune(self._h, days_each, C.byref(t), C.byref(ms)))
+        return t.as_dict(), ms.value
+
 
 def simulation(spec: SimSpec, max_time: int, lib: C.CDLL | None = None) -> dict:
     """One stochastic run (simulation.cpp:18-191): t=0 census, max_time days,

@@ -1,15 +1,17 @@
 """Benchmark: person-days per second of the batched dengue-IBM day loop on one
 MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--sims 256] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--sims 256] [--no-cpu] [--autotune-days D]
 
 Workload: the 256-simulation batch (communities log-uniform 400..10,000 squared
 to a lattice, 200..2,000 humans per community, per-simulation transmission,
 long-distance and seasonal parameters, 1,825 days).  One step = one simulated
 day of every simulation of the batch: the demography kernel and the
 event-application kernel over every community.  W warm-up days, then K timed
-days (W + K = 1,825 by default, the whole configured run).  Inputs are
-synthetic and resident in HBM before timing starts.
+days (W + K = 1,825 by default, the whole configured run).  The first warm-up
+days time every candidate launch tuning for D days each and keep the fastest
+(results do not depend on the tuning).  Inputs are synthetic and resident in
+HBM before timing starts.
 
 The JSON line carries:
   roofline       the demography kernel (the dominant launch of the day) against
@@ -188,6 +190,8 @@ def main():
     ap.add_argument("--cpu-days", type=int, default=365)
     ap.add_argument("--cpu-serial-days", type=int, default=365)
     ap.add_argument("--graph", action="store_true", help="replay the day loop as hipGraphs (no per-kernel timing)")
+    ap.add_argument("--autotune-days", type=int, default=2,
+                    help="days per candidate launch tuning, timed within the warm-up (0: default tuning)")
     ap.add_argument("--mcmc", action="store_true", help="MCMC ESS/hour, plain vs surrogate, instead")
     ap.add_argument("--chains", type=int, default=64)
     ap.add_argument("--iters", type=int, default=160)
@@ -243,6 +247,13 @@ def main():
 
     done = 0
     chunk = 100
+    n_cand = len(eng.tuning_candidates())
+    tune_ms = None
+    if args.autotune_days and n_cand * args.autotune_days <= args.warmup:
+        chosen, tune_ms = eng.autotune(args.autotune_days)
+        done = n_cand * args.autotune_days
+        log(f"[bench] autotuned over {n_cand} launch tunings x {args.autotune_days} days: {chosen}, "
+            f"{tune_ms:.3f} ms/day")
     while done < args.warmup:
         n = min(chunk, args.warmup - done)
         eng.step(n)
@@ -290,6 +301,8 @@ def main():
             "mosquito_slots": slots,
             "days": days,
             "graph": bool(args.graph),
+            "tuning": eng.tuning(),
+            "autotuned": tune_ms is not None,
         },
     }
     step_bytes = 7 * (humans + slots) + 32 * sero_comms

@@ -193,6 +193,32 @@ ARBO_API int arbo_set_day(ArboEngine* e, uint32_t day);
 ARBO_API int arbo_kernel_time(ArboEngine* e, uint32_t kind, double* ms, uint64_t* launches);
 ARBO_API int arbo_reset_timing(ArboEngine* e);
 
+/* Launch knobs of the day kernels.  No knob changes a result: every random
+ * draw is keyed by model entities (simulation, community, individual, day,
+ * event), never by workgroup or lane, so runs under any tuning are
+ * bit-identical.  No counterpart in the reference, whose launch shapes come
+ * from the device properties (cudaidentities.h:14-37). */
+typedef struct ArboTuning {
+    uint32_t census_unroll;   /* demography kernel: 16-byte census loads in flight per lane (2, 4, 8) */
+    uint32_t early_mosq;      /* of the first batch, mosquito loads issued before the human scan */
+    uint32_t xcd_order;       /* bit 0 demography, bit 1 event kernel: the workgroups one XCD receives
+                                 take a contiguous range of communities */
+    uint32_t ld_split;        /* long-distance kernel: workgroups per simulation, 1..64 */
+} ArboTuning;
+
+/* The tunings arbo_autotune tries (out may be NULL to count them). */
+ARBO_API int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n);
+/* Set or read the engine's tuning (default {4, 4, 0, 8}); ARBO_E_INVALID for a
+ * combination the library was not built with. */
+ARBO_API int arbo_set_tuning(ArboEngine* e, const ArboTuning* t);
+ARBO_API int arbo_get_tuning(const ArboEngine* e, ArboTuning* out);
+/* Time every candidate on the engine's own batch for days_each days each
+ * (those days are simulated: the run advances by candidates x days_each days,
+ * with the same results as under any fixed tuning) and keep the fastest
+ * demography knobs and the fastest event-kernel order; ms_per_day gets the
+ * sum of their day times.  chosen and ms_per_day may be NULL. */
+ARBO_API int arbo_autotune(ArboEngine* e, uint32_t days_each, ArboTuning* chosen, double* ms_per_day);
+
 ARBO_API void arbo_destroy(ArboEngine* e);
 ARBO_API const char* arbo_last_error(const ArboEngine* e);
 

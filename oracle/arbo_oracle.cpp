@@ -91,6 +91,7 @@ struct ArboEngine {
     int threads = 0;
     bool ready = false;
     std::string err;
+    ArboTuning tuning{4, 4, 0, 8};   // launch knobs: accepted and reported, no effect on the CPU
 };
 
 namespace {
@@ -831,6 +832,52 @@ int arbo_kernel_time(ArboEngine* e, uint32_t, double* ms, uint64_t* launches) {
 }
 
 int arbo_reset_timing(ArboEngine* e) { return e ? ARBO_OK : ARBO_E_INVALID; }
+
+// Launch tunings: the same candidate set as the GPU library, so a caller's
+// tuning calls run unchanged; the CPU engine's results never depend on them.
+static const uint32_t kShapes[4][2] = {{4, 4}, {4, 2}, {8, 4}, {2, 2}};
+
+int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n) {
+    if (!n) return ARBO_E_INVALID;
+    uint32_t k = 0;
+    for (const auto& d : kShapes)
+        for (uint32_t x = 0; x <= 3; x += 3) {
+            if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8};
+            ++k;
+        }
+    *n = k;
+    return ARBO_OK;
+}
+
+int arbo_set_tuning(ArboEngine* e, const ArboTuning* t) {
+    if (!e || !t) return ARBO_E_INVALID;
+    bool shape = false;
+    for (const auto& d : kShapes) shape = shape || (d[0] == t->census_unroll && d[1] == t->early_mosq);
+    if (!shape || t->xcd_order > 3 || t->ld_split < 1 || t->ld_split > 64) return fail(e, ARBO_E_INVALID, "unsupported tuning");
+    e->tuning = *t;
+    return ARBO_OK;
+}
+
+int arbo_get_tuning(const ArboEngine* e, ArboTuning* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    *out = e->tuning;
+    return ARBO_OK;
+}
+
+int arbo_autotune(ArboEngine* e, uint32_t days_each, ArboTuning* chosen, double* ms_per_day) {
+    if (!e || days_each == 0) return ARBO_E_INVALID;
+    if (!e->ready) return fail(e, ARBO_E_STATE, "arbo_batch_init has not been called");
+    uint32_t n = 0;
+    arbo_tuning_candidates(nullptr, 0, &n);
+    if ((uint64_t)e->day + (uint64_t)n * days_each > e->max_days)
+        return fail(e, ARBO_E_CAPACITY, "autotuning needs candidates x days_each days of series capacity");
+    const double t0 = omp_get_wtime();
+    const int rc = arbo_batch_step(e, n * days_each);
+    if (rc) return rc;
+    if (chosen) *chosen = e->tuning;
+    if (ms_per_day) *ms_per_day = (omp_get_wtime() - t0) * 1e3 / (n * days_each);
+    return ARBO_OK;
+}
 
 void arbo_destroy(ArboEngine* e) { delete e; }
 

@@ -1,0 +1,70 @@
+"""Launch tunings of the day kernels change no result (GPU).
+
+Every candidate of arbo_tuning_candidates (census loads in flight, early
+mosquito loads, XCD-contiguous community order) and the extremes of the
+long-distance split give the oracle's census and series bit for bit, and an
+autotuned run, graph replay included, equals an untuned one."""
+import numpy as np
+import pytest
+
+from arbo_gpu_fit_amd import abi
+from arbo_gpu_fit_amd.engine import Engine
+from tests.cases import CASES
+from tests.test_gpu_parity import _compare_series, _compare_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _reference(oracle_lib, specs, T):
+    orc = Engine(oracle_lib, threads=8)
+    orc.init(specs, T)
+    orc.step(T)
+    out = [(orc.state(i), orc.series(i)) for i in range(len(specs))]
+    orc.close()
+    return out
+
+
+@pytest.mark.parametrize("case", ["ragged", "batch_pair"])
+def test_every_tuning_is_bit_identical(case, hip_lib, oracle_lib):
+    specs, T = CASES[case]()
+    ref = _reference(oracle_lib, specs, T)
+    gpu = Engine(hip_lib)
+    cands = gpu.tuning_candidates()
+    cands += [dict(cands[0], ld_split=1), dict(cands[1], ld_split=64)]
+    for cand in cands:
+        print(f"[tuning] case={case} tuning={cand}", flush=True)
+        gpu.init(specs, T)
+        gpu.set_tuning(**cand)
+        assert gpu.tuning() == cand
+        gpu.step(T)
+        for i, (st, se) in enumerate(ref):
+            _compare_state(gpu.state(i), st, f"{cand} sim {i}")
+            _compare_series(gpu.series(i), se, f"{cand} sim {i}")
+    gpu.close()
+
+
+@pytest.mark.parametrize("flags", [0, abi.F_GRAPH])
+def test_autotuned_run_equals_untuned(flags, hip_lib, oracle_lib):
+    specs, T = CASES["lattice10"]()
+    ref = _reference(oracle_lib, specs, T)
+    gpu = Engine(hip_lib, flags=flags)
+    gpu.init(specs, T)
+    gpu.step(10)
+    n = len(gpu.tuning_candidates())
+    chosen, ms = gpu.autotune(days_each=2)
+    print(f"[tuning] autotune flags={flags}: {chosen} {ms:.3f} ms/day", flush=True)
+    assert gpu.day == 10 + 2 * n
+    assert chosen == gpu.tuning() and ms > 0
+    gpu.step(T - gpu.day)
+    _compare_state(gpu.state(0), ref[0][0], "autotuned")
+    _compare_series(gpu.series(0), ref[0][1], "autotuned")
+    gpu.close()
+
+
+def test_unsupported_tuning_is_rejected(hip_lib):
+    gpu = Engine(hip_lib)
+    with pytest.raises(RuntimeError):
+        gpu.set_tuning(census_unroll=3)
+    with pytest.raises(RuntimeError):
+        gpu.set_tuning(ld_split=0)
+    gpu.close()

@@ -277,3 +277,30 @@ def test_claim_min_equals_serial(n, events, init, chunk):
         a, wa = serial_apply(state, events, elig)
         b, wb = claim_apply(state, events, elig, chunk)
         assert a == b and wa == wb
+
+
+def test_tuning_calls_on_the_oracle(oracle_lib):
+    """The oracle accepts the GPU library's tuning calls; autotuning simulates
+    its candidate days with unchanged results."""
+    specs = [SimSpec(3, 3, np.full(9, 300, np.uint32), configs.base_params(seed_frac_h=0.02), 5)]
+    a = Engine(oracle_lib, threads=2)
+    a.init(specs, 40)
+    cands = a.tuning_candidates()
+    assert len(cands) == 8 and cands[0] == {"census_unroll": 4, "early_mosq": 4, "xcd_order": 0, "ld_split": 8}
+    a.set_tuning(xcd_order=1, ld_split=3)
+    assert a.tuning()["ld_split"] == 3
+    with pytest.raises(RuntimeError):
+        a.set_tuning(census_unroll=3)
+    a.step(5)
+    chosen, ms = a.autotune(days_each=2)
+    assert a.day == 5 + 16 and ms > 0
+    with pytest.raises(RuntimeError):
+        a.autotune(days_each=3)   # 24 more days exceed the 40-day capacity
+    a.step(40 - a.day)
+    b = Engine(oracle_lib, threads=2)
+    b.init(specs, 40)
+    b.step(40)
+    for k in ("n_incidence", "n_count", "m_count"):
+        assert np.array_equal(a.series(0)[k], b.series(0)[k])
+    a.close()
+    b.close()
