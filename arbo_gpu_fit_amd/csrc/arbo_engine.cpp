@@ -532,52 +532,55 @@ int arbo_autotune(ArboEngine* e, uint32_t days_each, ArboTuning* chosen, double*
     if ((uint64_t)e->day + (uint64_t)n * days_each > e->max_days)
         return fail(e, ARBO_E_CAPACITY, "autotuning needs candidates x days_each days of series capacity");
     HIP_TRY(e, hipSetDevice(e->device));
-    // Each candidate runs days_each days with events around the demography
-    // kernel and around the rest of the day.  The demography knobs come from
-    // the candidate with the fastest demography, the event kernel's order bit
-    // from the one with the fastest rest of the day.
-    const uint32_t ne = 3 * days_each;
+    // The candidates take turns one day at a time, in forward order on even
+    // rounds and backward on odd ones, so that every candidate's days have the
+    // same mean and a steady drift of the epidemic favours none.  Events
+    // bracket the demography kernel and the rest of the day; the demography
+    // knobs come from the candidate with the least demography time, the event
+    // kernel's order bit from the one with the least rest-of-day time.
+    const uint32_t ne = 3 * n;
     std::vector<hipEvent_t> ev(ne, nullptr);
     int rc = ARBO_OK;
     for (uint32_t i = 0; i < ne && rc == ARBO_OK; ++i)
         if (hipEventCreate(&ev[i]) != hipSuccess) rc = fail(e, ARBO_E_DEVICE, "autotune: hipEventCreate");
-    double best_a = 1e300, best_b = 1e300;
-    Tuning keep_a = e->tuning, keep_b = e->tuning;
-    for (uint32_t k = 0; k < n && rc == ARBO_OK; ++k) {
-        const Tuning tu = from_abi(cand[k]);
-        if ((rc = apply_tuning(e, tu))) break;
+    std::vector<double> ta(n, 0.0), tb(n, 0.0);
+    for (uint32_t r = 0; r < days_each && rc == ARBO_OK; ++r) {
         hipError_t st = hipSuccess;
-        for (uint32_t d = 0; d < days_each && st == hipSuccess; ++d) {
-            st = hipEventRecord(ev[3 * d], e->stream);
+        for (uint32_t j = 0; j < n && rc == ARBO_OK && st == hipSuccess; ++j) {
+            const uint32_t k = (r & 1u) ? n - 1 - j : j;
+            if ((rc = apply_tuning(e, from_abi(cand[k])))) break;
+            st = hipEventRecord(ev[3 * k], e->stream);
             if (st == hipSuccess) st = launch_demography(e->view, e->tuning, e->stream);
-            if (st == hipSuccess) st = hipEventRecord(ev[3 * d + 1], e->stream);
+            if (st == hipSuccess) st = hipEventRecord(ev[3 * k + 1], e->stream);
             if (st == hipSuccess) st = launch_longdist(e->view, e->tuning, e->stream);
             if (st == hipSuccess) st = launch_events(e->view, e->tuning, e->stream);
-            if (st == hipSuccess) st = hipEventRecord(ev[3 * d + 2], e->stream);
+            if (st == hipSuccess) st = hipEventRecord(ev[3 * k + 2], e->stream);
             if (st == hipSuccess) st = launch_advance(e->day_ctr.p, e->stream);
-        }
-        if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
-        double ta = 0.0, tb = 0.0;
-        for (uint32_t d = 0; d < days_each && st == hipSuccess; ++d) {
+            if (st == hipSuccess) st = hipStreamSynchronize(e->stream);
             float x = 0.0f, y = 0.0f;
-            st = hipEventElapsedTime(&x, ev[3 * d], ev[3 * d + 1]);
-            if (st == hipSuccess) st = hipEventElapsedTime(&y, ev[3 * d + 1], ev[3 * d + 2]);
-            ta += x;
-            tb += y;
+            if (st == hipSuccess) st = hipEventElapsedTime(&x, ev[3 * k], ev[3 * k + 1]);
+            if (st == hipSuccess) st = hipEventElapsedTime(&y, ev[3 * k + 1], ev[3 * k + 2]);
+            if (st == hipSuccess) {
+                ta[k] += x;
+                tb[k] += y;
+                e->day += 1;
+            }
         }
-        if (st != hipSuccess) { rc = fail(e, ARBO_E_DEVICE, std::string("autotune: ") + hipGetErrorString(st)); break; }
-        e->day += days_each;
-        if (ta < best_a) { best_a = ta; keep_a = tu; }
-        if (tb < best_b) { best_b = tb; keep_b = tu; }
+        if (rc == ARBO_OK && st != hipSuccess) rc = fail(e, ARBO_E_DEVICE, std::string("autotune: ") + hipGetErrorString(st));
+    }
+    uint32_t ka = 0, kb = 0;
+    for (uint32_t k = 1; k < n; ++k) {
+        if (ta[k] < ta[ka]) ka = k;
+        if (tb[k] < tb[kb]) kb = k;
     }
     for (hipEvent_t x : ev)
         if (x) (void)hipEventDestroy(x);
     if (rc) return rc;
-    Tuning keep = keep_a;
-    keep.xcd = (keep_a.xcd & 1u) | (keep_b.xcd & 2u);
+    Tuning keep = from_abi(cand[ka]);
+    keep.xcd = (keep.xcd & 1u) | (cand[kb].xcd_order & 2u);
     if ((rc = apply_tuning(e, keep))) return rc;
     if (chosen) *chosen = to_abi(keep);
-    if (ms_per_day) *ms_per_day = (best_a + best_b) / days_each;
+    if (ms_per_day) *ms_per_day = (ta[ka] + tb[kb]) / days_each;
     return ARBO_OK;
 }
 

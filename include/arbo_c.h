@@ -214,9 +214,10 @@ ARBO_API int arbo_set_tuning(ArboEngine* e, const ArboTuning* t);
 ARBO_API int arbo_get_tuning(const ArboEngine* e, ArboTuning* out);
 /* Time every candidate on the engine's own batch for days_each days each
  * (those days are simulated: the run advances by candidates x days_each days,
- * with the same results as under any fixed tuning) and keep the fastest
- * demography knobs and the fastest event-kernel order; ms_per_day gets the
- * sum of their day times.  chosen and ms_per_day may be NULL. */
+ * with the same results as under any fixed tuning; the candidates take turns
+ * a day at a time, forward and backward in alternate rounds) and keep the
+ * fastest demography knobs and the fastest event-kernel order; ms_per_day
+ * gets the sum of their day times.  chosen and ms_per_day may be NULL. */
 ARBO_API int arbo_autotune(ArboEngine* e, uint32_t days_each, ArboTuning* chosen, double* ms_per_day);
 
 ARBO_API void arbo_destroy(ArboEngine* e);
