@@ -268,6 +268,19 @@ int eval_engine(ArboMcmc* m, ArboEngine* eng, const ArboEngineApi& api, uint32_t
     return ARBO_OK;
 }
 
+// Cholesky factor of the adapted proposal covariance 2.38^2/d Sigma_n (+ a
+// small ridge); false while the pool is too small (the fixed proposal).
+bool am_factor(const ArboMcmc* m, double* L) {
+    if (!(m->n_pool >= (double)m->desc.adapt_after && m->n_pool > 1.0)) return false;
+    double S[D * D];
+    for (uint32_t i = 0; i < D; ++i)
+        for (uint32_t j = 0; j < D; ++j) {
+            S[i * D + j] = 2.38 * 2.38 / D * m->m2[i * D + j] / (m->n_pool - 1.0);
+            if (i == j) S[i * D + j] += 1e-12 + 1e-10 * m->sd0[i] * m->sd0[i];
+        }
+    return cholesky(S, L);
+}
+
 template <class Eval>
 int run_iterations(ArboMcmc* m, uint32_t n_iters, Eval eval) {
     const uint32_t C = m->C;
@@ -290,18 +303,8 @@ int run_iterations(ArboMcmc* m, uint32_t n_iters, Eval eval) {
     std::vector<uint8_t> acc(C);
     for (uint32_t step = 0; step < n_iters; ++step) {
         const uint32_t it = m->iter;
-        // adapted proposal covariance 2.38^2/d Sigma_n (+ a small ridge)
         double L[D * D];
-        bool adapted = m->n_pool >= (double)m->desc.adapt_after && m->n_pool > 1.0;
-        if (adapted) {
-            double S[D * D];
-            for (uint32_t i = 0; i < D; ++i)
-                for (uint32_t j = 0; j < D; ++j) {
-                    S[i * D + j] = 2.38 * 2.38 / D * m->m2[i * D + j] / (m->n_pool - 1.0);
-                    if (i == j) S[i * D + j] += 1e-12 + 1e-10 * m->sd0[i] * m->sd0[i];
-                }
-            adapted = cholesky(S, L);
-        }
+        const bool adapted = am_factor(m, L);
         uint32_t ns = 0;
         for (uint32_t c = 0; c < C; ++c) {
             propose(m, c, it, L, adapted, &prop[(size_t)c * D]);
@@ -480,7 +483,7 @@ ARBO_API int arbo_nb_loglik(const uint32_t* sim_weekly, const uint32_t* data, ui
 
 namespace {
 
-enum : uint32_t { Q_MALA_NORMAL = 5, Q_MALA_ACCEPT = 6 };
+enum : uint32_t { Q_MALA_NORMAL = 5, Q_MALA_ACCEPT = 6, Q_SCREEN = 7 };
 
 void features_of(const ArboMcmc* m, const double* th, float* x) {
     for (uint32_t i = 0; i < 4; ++i) {
@@ -606,12 +609,91 @@ namespace {
 
 int check_da(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da) {
     if (!sur || !da) return fail(m, ARBO_E_INVALID, "surrogate and descriptor are required");
-    if (da->sub_steps == 0 || da->sub_steps > 1000 || !(da->step0 > 0.0) ||
-        !(da->target_accept > 0.0 && da->target_accept < 1.0))
+    if (da->mode > ARBO_DA_SCREEN) return fail(m, ARBO_E_INVALID, "bad delayed-acceptance mode");
+    if (da->mode == ARBO_DA_LANGEVIN && (da->sub_steps == 0 || da->sub_steps > 1000 || !(da->step0 > 0.0) ||
+                                         !(da->target_accept > 0.0 && da->target_accept < 1.0)))
         return fail(m, ARBO_E_INVALID, "bad delayed-acceptance descriptor");
     uint32_t in = 0, outw = 0;
     if (arbo_sur_shape(sur, &in, &outw) != ARBO_OK || in != D || outw != m->desc.n_weeks)
         return fail(m, ARBO_E_INVALID, "surrogate shape does not match (6 inputs, n_weeks outputs)");
+    return ARBO_OK;
+}
+
+// Screened delayed acceptance (ARBO_DA_SCREEN): the adaptive-Metropolis
+// proposal, a surrogate first stage, the simulator only for the proposals
+// that pass it.  `ibm` gives the second stage's log-likelihoods.
+template <class Ibm>
+int run_screen_impl(ArboMcmc* m, ArboSurrogate* sur, uint32_t n_iters, Ibm ibm) {
+    using clk = std::chrono::steady_clock;
+    const uint32_t C = m->C;
+    std::vector<double> s_lp(C);   // surrogate log posterior of the current states
+    {
+        const auto t0 = clk::now();
+        int rc = sur_logpost(m, sur, C, m->theta.data(), s_lp.data(), nullptr);
+        m->sur_s += std::chrono::duration<double>(clk::now() - t0).count();
+        if (rc != ARBO_OK) return rc;
+    }
+    std::vector<double> prop((size_t)C * D), sub((size_t)C * D), sub_slp(C), pass((size_t)C * D), pass_slp(C),
+        pass_ll(C);
+    std::vector<uint32_t> sub_chain(C), pass_chain(C);
+    std::vector<uint8_t> acc(C);
+    for (uint32_t step = 0; step < n_iters; ++step) {
+        const uint32_t it = m->iter;
+        double L[D * D];
+        const bool adapted = am_factor(m, L);
+        uint32_t ns = 0;
+        for (uint32_t c = 0; c < C; ++c) {
+            propose(m, c, it, L, adapted, &prop[(size_t)c * D]);
+            acc[c] = 0;
+            if (std::isfinite(log_prior(m, &prop[(size_t)c * D]))) {   // checkParams (MCMC.cpp:164-186)
+                std::memcpy(&sub[(size_t)ns * D], &prop[(size_t)c * D], sizeof(double) * D);
+                sub_chain[ns++] = c;
+            }
+        }
+        m->n_prop += C;
+        m->n_support += ns;
+        // first stage: the surrogate posterior ratio
+        const auto t0 = clk::now();
+        int rc = sur_logpost(m, sur, ns, sub.data(), sub_slp.data(), nullptr);
+        m->sur_s += std::chrono::duration<double>(clk::now() - t0).count();
+        if (rc != ARBO_OK) return rc;
+        uint32_t np = 0;
+        for (uint32_t k = 0; k < ns; ++k) {
+            const uint32_t c = sub_chain[k];
+            const U4 r = draw(m->key, c, it, Q_SCREEN, 0);
+            if (std::log(1.0 - u01(r.x, r.y)) < sub_slp[k] - s_lp[c]) {
+                std::memcpy(&pass[(size_t)np * D], &sub[(size_t)k * D], sizeof(double) * D);
+                pass_slp[np] = sub_slp[k];
+                pass_chain[np++] = c;
+            }
+        }
+        m->da_steps += ns;
+        m->da_acc += np;
+        // second stage: the simulator at the proposals that passed
+        rc = ibm(np, pass.data(), pass_chain.data(), it, pass_ll.data());
+        if (rc != ARBO_OK) return rc;
+        for (uint32_t k = 0; k < np; ++k) {
+            const uint32_t c = pass_chain[k];
+            const double* tp = &pass[(size_t)k * D];
+            const double lp_new = log_prior(m, tp);
+            // L(theta') L_s(theta) / (L(theta) L_s(theta')); the surrogate terms carry the prior
+            const double log_a = (pass_ll[k] + lp_new - pass_slp[k]) - (m->loglik[c] + m->logprior[c] - s_lp[c]);
+            const U4 r = draw(m->key, c, it, Q_ACCEPT, 0);
+            if (std::log(1.0 - u01(r.x, r.y)) < log_a) {
+                std::memcpy(&m->theta[(size_t)c * D], tp, sizeof(double) * D);
+                m->loglik[c] = pass_ll[k];
+                m->logprior[c] = lp_new;
+                s_lp[c] = pass_slp[k];
+                acc[c] = 1;
+                m->n_acc++;
+            }
+        }
+        for (uint32_t c = 0; c < C; ++c) pool_add(m, &m->theta[(size_t)c * D]);
+        m->h_theta.insert(m->h_theta.end(), m->theta.begin(), m->theta.end());
+        m->h_loglik.insert(m->h_loglik.end(), m->loglik.begin(), m->loglik.end());
+        m->h_acc.insert(m->h_acc.end(), acc.begin(), acc.end());
+        m->iter++;
+    }
     return ARBO_OK;
 }
 
@@ -639,6 +721,7 @@ int run_da_impl(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da, uint32_t 
         }
         m->started = true;
     }
+    if (da->mode == ARBO_DA_SCREEN) return run_screen_impl(m, sur, n_iters, ibm);
     if (!m->da_init) {
         m->da_log_eps = std::log(da->step0);
         m->da_init = true;

@@ -100,9 +100,11 @@ def summarise(m: mcmc.Mcmc, wall: float, burn: int) -> dict:
 
 
 def run_fit(engine: Engine, data: np.ndarray, iters: int, chains: int = 64, seed: int = 1, sur: Surrogate | None = None,
-            sub_steps: int = 10, theta0=None, burn: int | None = None, chunk: int = 10) -> tuple[mcmc.Mcmc, dict]:
-    """A plain (sur None) or delayed-acceptance fit of `data`; returns the
-    driver and its summary over the iterations after `burn`."""
+            sub_steps: int = 10, theta0=None, burn: int | None = None, chunk: int = 10,
+            da_mode: int = mcmc.DA_LANGEVIN) -> tuple[mcmc.Mcmc, dict]:
+    """A plain (sur None) or delayed-acceptance fit of `data` (first stage
+    `da_mode`); returns the driver and its summary over the iterations after
+    `burn`."""
     g, humans = configs.mcmc_shape()
     cfg = mcmc.McmcConfig(n_chains=chains, n_weeks=len(data), priors=configs.mcmc_priors(),
                           adapt_after=100 * mcmc.D, nb_size=configs.MCMC_NB_SIZE, seed=seed)
@@ -114,9 +116,10 @@ def run_fit(engine: Engine, data: np.ndarray, iters: int, chains: int = 64, seed
         if sur is None:
             m.run(engine, n)
         else:
-            m.run_da(engine, sur, n, sub_steps=sub_steps)
+            m.run_da(engine, sur, n, sub_steps=sub_steps, mode=da_mode)
         done += n
-        log(f"[fit] {'delayed-acceptance' if sur is not None else 'plain'} {done}/{iters} iterations, "
+        kind = "plain" if sur is None else ("screened" if da_mode == mcmc.DA_SCREEN else "delayed-acceptance")
+        log(f"[fit] {kind} {done}/{iters} iterations, "
             f"{time.perf_counter() - t0:.1f} s")
     wall = time.perf_counter() - t0
     out = summarise(m, wall, iters // 4 if burn is None else burn)

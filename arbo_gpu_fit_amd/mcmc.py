@@ -38,7 +38,11 @@ class McmcDesc(C.Structure):
 
 
 class DaDesc(C.Structure):
-    _fields_ = [("sub_steps", _u32), ("reserved", _u32), ("step0", C.c_double), ("target_accept", C.c_double)]
+    _fields_ = [("sub_steps", _u32), ("mode", _u32), ("step0", C.c_double), ("target_accept", C.c_double)]
+
+
+DA_LANGEVIN = 0   # surrogate Langevin steps, then the simulator at the end point
+DA_SCREEN = 1     # the adaptive-Metropolis proposal screened by the surrogate
 
 
 class EngineApi(C.Structure):
@@ -193,15 +197,17 @@ class Mcmc:
         self._check(self.lib.arbo_mcmc_run_target(self._h, self._cb, None, n_iters))
 
     def run_da(self, engine: Engine, sur, n_iters: int, sub_steps: int = 10, step0: float = 1.0,
-               target_accept: float = 0.574, api: EngineApi | None = None):
-        """n_iters delayed-acceptance iterations: `sub_steps` Langevin steps
-        on the surrogate `sur` (a surrogate.Surrogate), then the simulator."""
-        da = DaDesc(sub_steps, 0, step0, target_accept)
+               target_accept: float = 0.574, api: EngineApi | None = None, mode: int = DA_LANGEVIN):
+        """n_iters delayed-acceptance iterations with the surrogate `sur` (a
+        surrogate.Surrogate) as first stage: `sub_steps` Langevin steps on it
+        (DA_LANGEVIN) or a screen of the adaptive-Metropolis proposal
+        (DA_SCREEN); then the simulator."""
+        da = DaDesc(sub_steps, mode, step0, target_accept)
         self._check(self.lib.arbo_mcmc_run_da(self._h, engine._h, C.byref(api) if api is not None else None, sur._h,
                                               C.byref(da), n_iters))
 
     def run_da_target(self, loglik, sur, n_iters: int, sub_steps: int = 10, step0: float = 1.0,
-                      target_accept: float = 0.574):
+                      target_accept: float = 0.574, mode: int = DA_LANGEVIN):
         """Delayed acceptance with a Python second-stage log-likelihood."""
         def cb(ctx, n, th, out):
             try:
@@ -213,7 +219,7 @@ class Mcmc:
             except Exception:
                 return -1
         self._cb = LOGLIK_FN(cb)
-        da = DaDesc(sub_steps, 0, step0, target_accept)
+        da = DaDesc(sub_steps, mode, step0, target_accept)
         self._check(self.lib.arbo_mcmc_run_da_target(self._h, self._cb, None, sur._h, C.byref(da), n_iters))
 
     def da_counters(self) -> dict:

@@ -81,20 +81,31 @@ typedef int (*ArboLogLikFn)(void* ctx, uint32_t n, const double* theta, double* 
 typedef struct ArboMcmc ArboMcmc;
 
 /* Delayed acceptance with the surrogate (no counterpart in the reference).
- * Each iteration every chain runs sub_steps Metropolis-adjusted Langevin
- * steps on the surrogate posterior (prior x negative-binomial likelihood of
- * the surrogate's mean incidence), preconditioned by the chains' pooled
- * covariance, then the simulator judges the end point:
- *   alpha2 = min(1, L(theta') L_s(theta) / (L(theta) L_s(theta')))
- * (Christen & Fox 2005; the surrogate-transition method of Liu 2001), which
- * leaves the posterior under the simulator's likelihood exact.  The step
- * size adapts (diminishing Robbins-Monro) to target_accept of the surrogate
- * steps. */
+ * The surrogate posterior is the prior x the negative-binomial likelihood of
+ * the surrogate's mean incidence.  Two first stages:
+ *
+ * ARBO_DA_LANGEVIN: each iteration every chain runs sub_steps
+ *   Metropolis-adjusted Langevin steps on the surrogate posterior,
+ *   preconditioned by the chains' pooled covariance, then the simulator
+ *   judges the end point:
+ *     alpha2 = min(1, L(theta') L_s(theta) / (L(theta) L_s(theta')))
+ *   (the surrogate-transition method of Liu 2001).  The step size adapts
+ *   (diminishing Robbins-Monro) to target_accept of the surrogate steps.
+ * ARBO_DA_SCREEN: each iteration every chain draws the adaptive-Metropolis
+ *   proposal of arbo_mcmc_run; the surrogate screens it,
+ *     alpha1 = min(1, pi_s(theta') / pi_s(theta)),
+ *   and only proposals that pass are simulated, for the same alpha2
+ *   (Christen & Fox 2005).  An iteration whose proposals all fail the screen
+ *   runs no simulation.
+ *
+ * Either way the posterior under the simulator's likelihood stays exact. */
+#define ARBO_DA_LANGEVIN 0u
+#define ARBO_DA_SCREEN 1u
 typedef struct ArboDaDesc {
-    uint32_t sub_steps;       /* surrogate steps per simulator evaluation, 1..1000 */
-    uint32_t reserved;
-    double step0;             /* initial step in preconditioner units */
-    double target_accept;     /* surrogate-step acceptance the step size adapts to (0.574 for MALA) */
+    uint32_t sub_steps;       /* ARBO_DA_LANGEVIN: surrogate steps per simulator evaluation, 1..1000 */
+    uint32_t mode;            /* ARBO_DA_LANGEVIN or ARBO_DA_SCREEN */
+    double step0;             /* ARBO_DA_LANGEVIN: initial step in preconditioner units */
+    double target_accept;     /* ARBO_DA_LANGEVIN: surrogate-step acceptance the step size adapts to (0.574) */
 } ArboDaDesc;
 
 /* Create the driver.  shape: the model's lattice and humans per community

@@ -2,7 +2,8 @@
 
 * Exactness: the second stage is a known Gaussian target and the surrogate a
   network that knows nothing of it; the chains must still sample that exact
-  Gaussian (the first stage only filters proposals).
+  Gaussian (the first stage only filters proposals), with Langevin sub-steps
+  and with the screened adaptive-Metropolis proposal.
 * Parity: the same delayed-acceptance run over the GPU engine and over the
   CPU oracle's engine gives identical chains (bit-exact simulations).
 """
@@ -20,7 +21,8 @@ from tests.test_mcmc import _tiny_driver
 pytestmark = pytest.mark.gpu
 
 
-def test_delayed_acceptance_keeps_the_target(hip_lib):
+@pytest.mark.parametrize("mode", [mcmc.DA_LANGEVIN, mcmc.DA_SCREEN])
+def test_delayed_acceptance_keeps_the_target(hip_lib, mode):
     lib = mcmc.bind(hip_lib)
     W = 4
     priors = [(0, 0.0, 1.0), (0, 0.0, 1.0), (0, -3.0, -1.0), (0, 0.0, 1.0), (1, 0.0, 2 * math.pi), (0, 0.01, 1.0)]
@@ -37,7 +39,7 @@ def test_delayed_acceptance_keeps_the_target(hip_lib):
         d = th - mean
         return -0.5 * np.einsum("ni,ij,nj->n", d, prec, d)
 
-    print("[da] exactness", flush=True)
+    print(f"[da] exactness mode={mode}", flush=True)
     sur = Surrogate([6, 32, 32, W], max_batch=256, seed=5, lib=hip_lib)
     p = sur.get_params()
     p *= 0.2                                   # a weakly informative, unrelated surrogate
@@ -46,7 +48,7 @@ def test_delayed_acceptance_keeps_the_target(hip_lib):
     cfg = mcmc.McmcConfig(n_chains=64, n_weeks=W, priors=priors, adapt_after=300, seed=8)
     m = mcmc.Mcmc(cfg, 1, 1, [10], configs.base_params(), np.array([3, 9, 7, 12], np.uint32),
                   theta0=np.tile(mean, (64, 1)), lib=lib)
-    m.run_da_target(target, sur, 2500, sub_steps=3, step0=0.5)
+    m.run_da_target(target, sur, 2500, sub_steps=3, step0=0.5, mode=mode)
     th, _, acc = m.history()
     post = th[300:]
     samples = post.reshape(-1, 6)
@@ -57,12 +59,16 @@ def test_delayed_acceptance_keeps_the_target(hip_lib):
     # the spread too, to the precision the effective sample size allows
     assert np.all(np.abs(samples.std(axis=0) / sd - 1) < 5 / np.sqrt(2 * n_eff))
     c = m.da_counters()
-    assert abs(c["surrogate_accepted"] / c["surrogate_steps"] - 0.574) < 0.15
+    if mode == mcmc.DA_LANGEVIN:
+        assert abs(c["surrogate_accepted"] / c["surrogate_steps"] - 0.574) < 0.15
+    else:   # the screen passes some proposals and stops others
+        assert 0.05 < c["surrogate_accepted"] / c["surrogate_steps"] < 0.95
     m.close()
     sur.close()
 
 
-def test_delayed_acceptance_equal_over_gpu_and_oracle(hip_lib, oracle_lib):
+@pytest.mark.parametrize("mode", [mcmc.DA_LANGEVIN, mcmc.DA_SCREEN])
+def test_delayed_acceptance_equal_over_gpu_and_oracle(hip_lib, oracle_lib, mode):
     lib = mcmc.bind(hip_lib)
     print("[da] gpu vs oracle", flush=True)
     sur = Surrogate([6, 16, 6], max_batch=64, seed=2, lib=hip_lib)
@@ -74,10 +80,10 @@ def test_delayed_acceptance_equal_over_gpu_and_oracle(hip_lib, oracle_lib):
         m, orc = _tiny_driver(lib, oracle_lib, seed=33, n_chains=8, n_particles=1)
         if use_gpu:
             eng = Engine(hip_lib)
-            m.run_da(eng, sur, 3, sub_steps=2)
+            m.run_da(eng, sur, 3, sub_steps=2, mode=mode)
             eng.close()
         else:
-            m.run_da(orc, sur, 3, sub_steps=2, api=mcmc.engine_api(oracle_lib))
+            m.run_da(orc, sur, 3, sub_steps=2, api=mcmc.engine_api(oracle_lib), mode=mode)
         histories.append((m.history(), m.counters(), m.da_counters()))
         m.close()
         orc.close()
