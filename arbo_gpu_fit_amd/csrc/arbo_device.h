@@ -18,7 +18,7 @@ constexpr uint32_t kWave = 64;            // threads per workgroup: one wavefron
 constexpr uint32_t kCntPerComm = 72;      // event counts of a source: [dir 2][serotype 4][row slot 9]
 constexpr uint32_t kInboxEdges = 9;       // in-edges per destination (Moore-8 + self)
 constexpr uint32_t kHumanAlign = 4;       // humans per 16-byte vector
-constexpr uint32_t kMosqAlign = 32;       // mosquito slots per alive-bitmap word (and 4 vectors)
+constexpr uint32_t kMosqAlign = 128;      // mosquito slots per 16-byte group of alive-bitmap words
 
 struct CommDesc {
     uint64_t h_off;       // global offset of the first human

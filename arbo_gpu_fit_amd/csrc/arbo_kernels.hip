@@ -587,6 +587,7 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     }
     // slots that died today and stay empty; the alive bitmap of the day
     uint32_t top = 0;
+    uint32_t awv[kWordsPerLane] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (uint32_t k = 0; k < kWordsPerLane; ++k) {
         const uint32_t w = kWordsPerLane * lane + k;
@@ -603,9 +604,13 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
             v.m_word[cd.m_off + 32u * w + e] = 3;
         }
         const uint32_t aw = ~(d & ~reborn) & valid_bits(w, cd.cap);
-        v.alive[(cd.m_off >> 5) + w] = aw;
+        awv[k] = aw;
         if (aw) top = 32u * w + 32u - (uint32_t)__clz((int)aw);   // one past the highest alive slot
     }
+    // the lane's four bitmap words in one store (communities start on 128-slot
+    // boundaries and own whole multiples of 128 slots)
+    if (kWordsPerLane * lane < nwords)
+        reinterpret_cast<uint4*>(v.alive + (cd.m_off >> 5))[lane] = make_uint4(awv[0], awv[1], awv[2], awv[3]);
     top = wave_max(top);
     if (lane == 0)   // the day's summary: human part (unchanged when not scanned) and mosquito limit
         reinterpret_cast<uint4*>(v.hsum)[g] = make_uint4(minev, ih01, ih23, ((top + 7) >> 3) + 1u);
@@ -859,13 +864,13 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     const uint32_t tq_m = s_qseg[DIR_HM][nseg];
     if (tq_m > 0) {
         const uint32_t nwords = (cd.cap + 31) >> 5;
-        uint32_t wv[kWordsPerLane], c = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < kWordsPerLane; ++k) {
-            const uint32_t w = kWordsPerLane * lane + k;
-            wv[k] = w < nwords ? v.alive[(cd.m_off >> 5) + w] : 0u;
-            c += __popc(wv[k]);
+        uint32_t wv[kWordsPerLane] = {0u, 0u, 0u, 0u}, c = 0;
+        if (kWordsPerLane * lane < nwords) {   // one 16-byte load (words past the capacity are zero)
+            const uint4 a4 = reinterpret_cast<const uint4*>(v.alive + (cd.m_off >> 5))[lane];
+            wv[0] = a4.x; wv[1] = a4.y; wv[2] = a4.z; wv[3] = a4.w;
         }
+#pragma unroll
+        for (uint32_t k = 0; k < kWordsPerLane; ++k) c += __popc(wv[k]);
         uint32_t Md;
         uint32_t pre = wave_exclusive_scan(c, Md);
 #pragma unroll
