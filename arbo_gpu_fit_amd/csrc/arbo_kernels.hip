@@ -757,15 +757,6 @@ __device__ __forceinline__ Quad decode_quad(uint32_t gq, const uint32_t* seg, co
     return q;
 }
 
-// The q-th alive slot of the community, from its bitmap words and their
-// exclusive popcount prefix.
-__device__ __forceinline__ uint32_t select_alive(uint32_t q, const uint32_t* word, const uint16_t* wpre,
-                                                 uint32_t nwords) {
-    uint32_t w, pos;
-    select_bit(q, word, wpre, nwords, w, pos);
-    return 32u * w + pos;
-}
-
 __device__ __forceinline__ uint32_t claim_slot(uint32_t key) { return (key * 2654435761u) >> 23; }
 
 // Record event `rank` of the chunk on member `key`; the table keeps the
@@ -817,6 +808,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     __shared__ uint32_t s_qseg[2][4 * 10 + 1];
     __shared__ uint32_t s_word[kMaxWords];
     __shared__ uint16_t s_wpre[kMaxWords];
+    __shared__ uint16_t s_sel[kMaxWords];   // word holding alive rank 32k, k = 0..(Md-1)/32
     __shared__ uint32_t s_tab[kHash];
 
     if (lane < in_len) {
@@ -876,8 +868,14 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
 #pragma unroll
         for (uint32_t k = 0; k < kWordsPerLane; ++k) {
             const uint32_t w = kWordsPerLane * lane + k;
-            if (w < nwords) { s_word[w] = wv[k]; s_wpre[w] = (uint16_t)pre; }
-            pre += __popc(wv[k]);
+            const uint32_t c = __popc(wv[k]);
+            if (w < nwords) {
+                s_word[w] = wv[k];
+                s_wpre[w] = (uint16_t)pre;
+                const uint32_t m = (pre + 31u) & ~31u;   // the multiple of 32 among this word's ranks, if any
+                if (m < pre + c) s_sel[m >> 5] = (uint16_t)w;
+            }
+            pre += c;
         }
         if (Md > 0) {
             for (uint32_t cq = 0; cq < tq_m; cq += kChunkQuads) {
@@ -886,7 +884,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                 __syncthreads();
                 const uint32_t base = decode_quad(cq, s_seg[DIR_HM], s_qseg[DIR_HM], nseg, in_len, s_rot[DIR_HM], S,
                                                   DIR_HM, s_src, s_slot, cd.c).rank;
-                uint32_t cache[kPerLane], qrank[kQuadsPerLane];
+                uint32_t cache[kPerLane], word[kPerLane], qrank[kQuadsPerLane];
 #pragma unroll
                 for (uint32_t u = 0; u < kQuadsPerLane; ++u) {
                     const uint32_t gq = cq + u * kWave + lane;
@@ -902,10 +900,15 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
 #pragma unroll
                     for (uint32_t e = 0; e < 4; ++e) {
                         if (e >= q.count) break;
-                        const uint32_t slot = select_alive(mulhi(w4[e], Md), s_word, s_wpre, nwords);
-                        if ((v.m_word[cd.m_off + slot] & 3u) == 0) {
+                        const uint32_t rk = mulhi(w4[e], Md);   // alive rank of the target
+                        uint32_t w = s_sel[rk >> 5];
+                        while (w + 1 < nwords && s_wpre[w + 1] <= rk) ++w;
+                        const uint32_t slot = 32u * w + select_in_word(s_word[w], rk - s_wpre[w]);
+                        const uint32_t mw = v.m_word[cd.m_off + slot];
+                        if ((mw & 3u) == 0) {
                             claim(s_tab, slot, qrank[u] + e);
                             cache[4 * u + e] = kEligible | (q.s << kKeyBits) | slot;
+                            word[4 * u + e] = mw;
                         }
                     }
                 }
@@ -917,7 +920,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                     if (claimed_rank(s_tab, slot) != qrank[i >> 2] + (i & 3u)) continue;
                     const uint32_t s = (cache[i] >> kKeyBits) & 3u;
                     const uint64_t gm = cd.m_off + slot;
-                    const uint32_t w = v.m_word[gm];
+                    const uint32_t w = word[i];   // as read by the claim pass: only the winner writes it
                     v.m_word[gm] = (uint16_t)((w & ~0xFu) | (s << 2) | 1u);
                     const U4 r2 = draw(key, cd.m_base + slot, t, P_M_EIP, 0);
                     v.m_next[gm] = (uint16_t)(t + stoch_round(sd.eip, r2.x));
@@ -941,7 +944,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                 __syncthreads();
                 const uint32_t base = decode_quad(cq, s_seg[DIR_MH], s_qseg[DIR_MH], nseg, in_len, s_rot[DIR_MH], S,
                                                   DIR_MH, s_src, s_slot, cd.c).rank;
-                uint32_t cache[kPerLane], qrank[kQuadsPerLane];
+                uint32_t cache[kPerLane], word[kPerLane], qrank[kQuadsPerLane];
 #pragma unroll
                 for (uint32_t u = 0; u < kQuadsPerLane; ++u) {
                     const uint32_t gq = cq + u * kWave + lane;
@@ -962,6 +965,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                         if ((w & 3u) == 0 && !((w >> (4 + s)) & 1u)) {
                             claim(s_tab, h, qrank[u] + e);
                             cache[4 * u + e] = kEligible | h;
+                            word[4 * u + e] = w;
                         }
                     }
                 }
@@ -972,7 +976,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                     const uint32_t h = cache[i] & kKeyMask;
                     if (claimed_rank(s_tab, h) != qrank[i >> 2] + (i & 3u)) continue;
                     const uint64_t gi = cd.h_off + h;
-                    const uint32_t w = v.h_word[gi];
+                    const uint32_t w = word[i];   // as read by the claim pass: only the winner writes it
                     v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
                     const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
                     v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
