@@ -934,10 +934,15 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     if (s_qseg[DIR_MH][nseg] > 0 || sd.intro > 0.0f) {
         const uint32_t N = cd.n;
         bool infected = false;
-        for (uint32_t pos = 0; pos < S; ++pos) {
-            const uint32_t s = (s_rot[DIR_MH] + pos) % S;
-            const uint32_t qb = s_qseg[DIR_MH][pos * (in_len + 1)];
-            const uint32_t qe = s_qseg[DIR_MH][(pos + 1) * (in_len + 1)];
+        // Introductions follow each serotype's events, so with them the
+        // serotype positions run one by one; without them the whole ordered
+        // list runs as one (claim-min gives the same result either way: an
+        // infected human is eligible for no serotype).
+        const bool intro = sd.intro > 0.0f;
+        const uint32_t npos = intro ? S : 1u;
+        for (uint32_t pos = 0; pos < npos; ++pos) {
+            const uint32_t qb = intro ? s_qseg[DIR_MH][pos * (in_len + 1)] : 0u;
+            const uint32_t qe = intro ? s_qseg[DIR_MH][(pos + 1) * (in_len + 1)] : s_qseg[DIR_MH][nseg];
             for (uint32_t cq = qb; cq < qe; cq += kChunkQuads) {
                 const uint32_t cqe = qe - cq < kChunkQuads ? qe : cq + kChunkQuads;
                 for (uint32_t i = lane; i < kHash; i += kWave) s_tab[i] = kEmpty;
@@ -962,9 +967,9 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                         if (e >= q.count) break;
                         const uint32_t h = mulhi(w4[e], N);
                         const uint32_t w = v.h_word[cd.h_off + h];
-                        if ((w & 3u) == 0 && !((w >> (4 + s)) & 1u)) {
+                        if ((w & 3u) == 0 && !((w >> (4 + q.s)) & 1u)) {
                             claim(s_tab, h, qrank[u] + e);
-                            cache[4 * u + e] = kEligible | h;
+                            cache[4 * u + e] = kEligible | (q.s << kKeyBits) | h;
                             word[4 * u + e] = w;
                         }
                     }
@@ -975,6 +980,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                     if (!(cache[i] & kEligible)) continue;
                     const uint32_t h = cache[i] & kKeyMask;
                     if (claimed_rank(s_tab, h) != qrank[i >> 2] + (i & 3u)) continue;
+                    const uint32_t s = (cache[i] >> kKeyBits) & 3u;
                     const uint64_t gi = cd.h_off + h;
                     const uint32_t w = word[i];   // as read by the claim pass: only the winner writes it
                     v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
@@ -984,7 +990,8 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                 }
                 __syncthreads();
             }
-            if (sd.intro > 0.0f) {  // external introductions (nepidemic.cu:261-309)
+            if (intro) {  // external introductions (nepidemic.cu:261-309)
+                const uint32_t s = (s_rot[DIR_MH] + pos) % S;
                 if (lane == 0) {
                     const uint32_t tagi = ctr_tag(P_INTRO, 0, s, 0);
                     uint32_t x = stoch_round(sd.intro / (float)sd.C, draw(key, cd.c, t, tagi, 0).x);
