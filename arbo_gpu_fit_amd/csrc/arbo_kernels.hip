@@ -785,6 +785,9 @@ __device__ __forceinline__ uint32_t claimed_rank(const uint32_t* tab, uint32_t k
 }
 
 constexpr uint32_t kEligible = 1u << 31;
+constexpr uint32_t kHumanKey = ARBO_MAX_COMMUNITY_SLOTS;   // claim keys of humans start here
+static_assert(ARBO_MAX_COMMUNITY_SLOTS + ARBO_MAX_COMMUNITY_HUMANS <= (1u << kKeyBits),
+              "mosquito and human claim keys fit the key field");
 
 template <bool XCD>
 __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
@@ -853,9 +856,10 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     __syncthreads();
 
     // ---- onto mosquitoes: uniform over alive slots, success iff susceptible ----
-    const uint32_t tq_m = s_qseg[DIR_HM][nseg];
+    const uint32_t tq_m = s_qseg[DIR_HM][nseg], tq_h = s_qseg[DIR_MH][nseg];
+    const uint32_t nwords = (cd.cap + 31) >> 5;
+    uint32_t Md = 0;
     if (tq_m > 0) {
-        const uint32_t nwords = (cd.cap + 31) >> 5;
         uint32_t wv[kWordsPerLane] = {0u, 0u, 0u, 0u}, c = 0;
         if (kWordsPerLane * lane < nwords) {   // one 16-byte load (words past the capacity are zero)
             const uint4 a4 = reinterpret_cast<const uint4*>(v.alive + (cd.m_off >> 5))[lane];
@@ -863,159 +867,132 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
         }
 #pragma unroll
         for (uint32_t k = 0; k < kWordsPerLane; ++k) c += __popc(wv[k]);
-        uint32_t Md;
         uint32_t pre = wave_exclusive_scan(c, Md);
 #pragma unroll
         for (uint32_t k = 0; k < kWordsPerLane; ++k) {
             const uint32_t w = kWordsPerLane * lane + k;
-            const uint32_t c = __popc(wv[k]);
+            const uint32_t cw = __popc(wv[k]);
             if (w < nwords) {
                 s_word[w] = wv[k];
                 s_wpre[w] = (uint16_t)pre;
                 const uint32_t m = (pre + 31u) & ~31u;   // the multiple of 32 among this word's ranks, if any
-                if (m < pre + c) s_sel[m >> 5] = (uint16_t)w;
+                if (m < pre + cw) s_sel[m >> 5] = (uint16_t)w;
             }
-            pre += c;
+            pre += cw;
         }
-        if (Md > 0) {
-            for (uint32_t cq = 0; cq < tq_m; cq += kChunkQuads) {
-                const uint32_t cqe = tq_m - cq < kChunkQuads ? tq_m : cq + kChunkQuads;
-                for (uint32_t i = lane; i < kHash; i += kWave) s_tab[i] = kEmpty;
-                __syncthreads();
-                const uint32_t base = decode_quad(cq, s_seg[DIR_HM], s_qseg[DIR_HM], nseg, in_len, s_rot[DIR_HM], S,
-                                                  DIR_HM, s_src, s_slot, cd.c).rank;
-                uint32_t cache[kPerLane], word[kPerLane], qrank[kQuadsPerLane];
+    }
+    const uint32_t mq = Md > 0 ? tq_m : 0u;   // with no alive mosquito every event onto one fails
+    // ---- onto humans: uniform target, success iff S and never infected by s ----
+    const uint32_t N = cd.n;
+    bool infected = false;
+
+    // One claim sequence: mosquito quads [m_lo, m_hi) of their direction's
+    // list, then human quads [h_lo, h_hi) of theirs.  Mosquito slots and
+    // humans (keyed from ARBO_MAX_COMMUNITY_SLOTS up) never share a claim,
+    // so the two directions' events can share chunks; within a direction the
+    // chunk rank 4 p + e keeps the list order.
+    auto apply_run = [&](uint32_t m_lo, uint32_t m_hi, uint32_t h_lo, uint32_t h_hi) {
+        const uint32_t lm = m_hi - m_lo, len = lm + (h_hi - h_lo);
+        for (uint32_t c0 = 0; c0 < len; c0 += kChunkQuads) {
+            for (uint32_t i = lane; i < kHash; i += kWave) s_tab[i] = kEmpty;
+            __syncthreads();
+            uint32_t cache[kPerLane], word[kPerLane];
 #pragma unroll
-                for (uint32_t u = 0; u < kQuadsPerLane; ++u) {
-                    const uint32_t gq = cq + u * kWave + lane;
-                    qrank[u] = 0;
+            for (uint32_t u = 0; u < kQuadsPerLane; ++u) {
 #pragma unroll
-                    for (uint32_t e = 0; e < 4; ++e) cache[4 * u + e] = 0;
-                    if (gq >= cqe) continue;
-                    const Quad q = decode_quad(gq, s_seg[DIR_HM], s_qseg[DIR_HM], nseg, in_len, s_rot[DIR_HM], S,
-                                               DIR_HM, s_src, s_slot, cd.c);
-                    qrank[u] = q.rank - base;
-                    const U4 r = draw(key, q.x0, t, q.x2, q.j);
-                    const uint32_t w4[4] = {r.x, r.y, r.z, r.w};
+                for (uint32_t e = 0; e < 4; ++e) cache[4 * u + e] = 0;
+                const uint32_t p = c0 + u * kWave + lane;
+                if (p >= len) continue;
+                const bool hum = p >= lm;
+                const uint32_t dir = hum ? DIR_MH : DIR_HM;
+                const Quad q = decode_quad(hum ? h_lo + (p - lm) : m_lo + p, s_seg[dir], s_qseg[dir], nseg, in_len,
+                                           s_rot[dir], S, dir, s_src, s_slot, cd.c);
+                const U4 r = draw(key, q.x0, t, q.x2, q.j);
+                const uint32_t w4[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
-                    for (uint32_t e = 0; e < 4; ++e) {
-                        if (e >= q.count) break;
+                for (uint32_t e = 0; e < 4; ++e) {
+                    if (e >= q.count) break;
+                    const uint32_t rank = 4 * (u * kWave + lane) + e;
+                    if (hum) {
+                        const uint32_t h = mulhi(w4[e], N);
+                        const uint32_t w = v.h_word[cd.h_off + h];
+                        if ((w & 3u) == 0 && !((w >> (4 + q.s)) & 1u)) {
+                            claim(s_tab, kHumanKey + h, rank);
+                            cache[4 * u + e] = kEligible | (q.s << kKeyBits) | (kHumanKey + h);
+                            word[4 * u + e] = w;
+                        }
+                    } else {
                         const uint32_t rk = mulhi(w4[e], Md);   // alive rank of the target
                         uint32_t w = s_sel[rk >> 5];
                         while (w + 1 < nwords && s_wpre[w + 1] <= rk) ++w;
                         const uint32_t slot = 32u * w + select_in_word(s_word[w], rk - s_wpre[w]);
                         const uint32_t mw = v.m_word[cd.m_off + slot];
                         if ((mw & 3u) == 0) {
-                            claim(s_tab, slot, qrank[u] + e);
+                            claim(s_tab, slot, rank);
                             cache[4 * u + e] = kEligible | (q.s << kKeyBits) | slot;
                             word[4 * u + e] = mw;
                         }
                     }
                 }
-                __syncthreads();
-#pragma unroll
-                for (uint32_t i = 0; i < kPerLane; ++i) {
-                    if (!(cache[i] & kEligible)) continue;
-                    const uint32_t slot = cache[i] & kKeyMask;
-                    if (claimed_rank(s_tab, slot) != qrank[i >> 2] + (i & 3u)) continue;
-                    const uint32_t s = (cache[i] >> kKeyBits) & 3u;
-                    const uint64_t gm = cd.m_off + slot;
-                    const uint32_t w = word[i];   // as read by the claim pass: only the winner writes it
-                    v.m_word[gm] = (uint16_t)((w & ~0xFu) | (s << 2) | 1u);
-                    const U4 r2 = draw(key, cd.m_base + slot, t, P_M_EIP, 0);
-                    v.m_next[gm] = (uint16_t)(t + stoch_round(sd.eip, r2.x));
-                }
-                __syncthreads();
             }
-        }
-    }
-
-    // ---- onto humans: uniform target, success iff S and never infected by s ----
-    if (s_qseg[DIR_MH][nseg] > 0 || sd.intro > 0.0f) {
-        const uint32_t N = cd.n;
-        bool infected = false;
-        // Introductions follow each serotype's events, so with them the
-        // serotype positions run one by one; without them the whole ordered
-        // list runs as one (claim-min gives the same result either way: an
-        // infected human is eligible for no serotype).
-        const bool intro = sd.intro > 0.0f;
-        const uint32_t npos = intro ? S : 1u;
-        for (uint32_t pos = 0; pos < npos; ++pos) {
-            const uint32_t qb = intro ? s_qseg[DIR_MH][pos * (in_len + 1)] : 0u;
-            const uint32_t qe = intro ? s_qseg[DIR_MH][(pos + 1) * (in_len + 1)] : s_qseg[DIR_MH][nseg];
-            for (uint32_t cq = qb; cq < qe; cq += kChunkQuads) {
-                const uint32_t cqe = qe - cq < kChunkQuads ? qe : cq + kChunkQuads;
-                for (uint32_t i = lane; i < kHash; i += kWave) s_tab[i] = kEmpty;
-                __syncthreads();
-                const uint32_t base = decode_quad(cq, s_seg[DIR_MH], s_qseg[DIR_MH], nseg, in_len, s_rot[DIR_MH], S,
-                                                  DIR_MH, s_src, s_slot, cd.c).rank;
-                uint32_t cache[kPerLane], word[kPerLane], qrank[kQuadsPerLane];
+            __syncthreads();
 #pragma unroll
-                for (uint32_t u = 0; u < kQuadsPerLane; ++u) {
-                    const uint32_t gq = cq + u * kWave + lane;
-                    qrank[u] = 0;
-#pragma unroll
-                    for (uint32_t e = 0; e < 4; ++e) cache[4 * u + e] = 0;
-                    if (gq >= cqe) continue;
-                    const Quad q = decode_quad(gq, s_seg[DIR_MH], s_qseg[DIR_MH], nseg, in_len, s_rot[DIR_MH], S,
-                                               DIR_MH, s_src, s_slot, cd.c);
-                    qrank[u] = q.rank - base;
-                    const U4 r = draw(key, q.x0, t, q.x2, q.j);
-                    const uint32_t w4[4] = {r.x, r.y, r.z, r.w};
-#pragma unroll
-                    for (uint32_t e = 0; e < 4; ++e) {
-                        if (e >= q.count) break;
-                        const uint32_t h = mulhi(w4[e], N);
-                        const uint32_t w = v.h_word[cd.h_off + h];
-                        if ((w & 3u) == 0 && !((w >> (4 + q.s)) & 1u)) {
-                            claim(s_tab, h, qrank[u] + e);
-                            cache[4 * u + e] = kEligible | (q.s << kKeyBits) | h;
-                            word[4 * u + e] = w;
-                        }
-                    }
-                }
-                __syncthreads();
-#pragma unroll
-                for (uint32_t i = 0; i < kPerLane; ++i) {
-                    if (!(cache[i] & kEligible)) continue;
-                    const uint32_t h = cache[i] & kKeyMask;
-                    if (claimed_rank(s_tab, h) != qrank[i >> 2] + (i & 3u)) continue;
-                    const uint32_t s = (cache[i] >> kKeyBits) & 3u;
+            for (uint32_t i = 0; i < kPerLane; ++i) {
+                if (!(cache[i] & kEligible)) continue;
+                const uint32_t k = cache[i] & kKeyMask;
+                if (claimed_rank(s_tab, k) != 4 * ((i >> 2) * kWave + lane) + (i & 3u)) continue;
+                const uint32_t s = (cache[i] >> kKeyBits) & 3u;
+                const uint32_t w = word[i];   // as read by the claim pass: only the winner writes it
+                if (k >= kHumanKey) {
+                    const uint32_t h = k - kHumanKey;
                     const uint64_t gi = cd.h_off + h;
-                    const uint32_t w = word[i];   // as read by the claim pass: only the winner writes it
                     v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
                     const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
                     v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
                     infected = true;
+                } else {
+                    const uint64_t gm = cd.m_off + k;
+                    v.m_word[gm] = (uint16_t)((w & ~0xFu) | (s << 2) | 1u);
+                    const U4 r2 = draw(key, cd.m_base + k, t, P_M_EIP, 0);
+                    v.m_next[gm] = (uint16_t)(t + stoch_round(sd.eip, r2.x));
                 }
-                __syncthreads();
             }
-            if (intro) {  // external introductions (nepidemic.cu:261-309)
-                const uint32_t s = (s_rot[DIR_MH] + pos) % S;
-                if (lane == 0) {
-                    const uint32_t tagi = ctr_tag(P_INTRO, 0, s, 0);
-                    uint32_t x = stoch_round(sd.intro / (float)sd.C, draw(key, cd.c, t, tagi, 0).x);
-                    for (uint32_t att = 0; x > 0 && att < 1000; ++att) {
-                        const uint32_t h = mulhi(draw(key, cd.c, t, tagi, 1 + att).x, N);
-                        const uint64_t gi = cd.h_off + h;
-                        const uint32_t w = v.h_word[gi];
-                        if ((w & 3u) == 0 && !((w >> (4 + s)) & 1u)) {
-                            v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
-                            const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
-                            v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
-                            infected = true;
-                            x--;
-                        }
+            __syncthreads();
+        }
+    };
+
+    if (!(sd.intro > 0.0f)) {
+        apply_run(0, mq, 0, tq_h);   // mosquitoes, then every serotype's human events
+    } else {
+        // Introductions follow each serotype's human events, so the serotype
+        // positions run one by one after the mosquitoes.
+        apply_run(0, mq, 0, 0);
+        for (uint32_t pos = 0; pos < S; ++pos) {
+            apply_run(0, 0, s_qseg[DIR_MH][pos * (in_len + 1)], s_qseg[DIR_MH][(pos + 1) * (in_len + 1)]);
+            const uint32_t s = (s_rot[DIR_MH] + pos) % S;
+            if (lane == 0) {  // external introductions (nepidemic.cu:261-309)
+                const uint32_t tagi = ctr_tag(P_INTRO, 0, s, 0);
+                uint32_t x = stoch_round(sd.intro / (float)sd.C, draw(key, cd.c, t, tagi, 0).x);
+                for (uint32_t att = 0; x > 0 && att < 1000; ++att) {
+                    const uint32_t h = mulhi(draw(key, cd.c, t, tagi, 1 + att).x, N);
+                    const uint64_t gi = cd.h_off + h;
+                    const uint32_t w = v.h_word[gi];
+                    if ((w & 3u) == 0 && !((w >> (4 + s)) & 1u)) {
+                        v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
+                        const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
+                        v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
+                        infected = true;
+                        x--;
                     }
                 }
-                __syncthreads();
             }
+            __syncthreads();
         }
-        // a human exposed today turns infectious on day t + iip + 1 at the earliest
-        if (__any(infected) && lane == 0) {
-            const uint32_t ev = t + sd.iip_min + 1;
-            if (ev < v.hsum[4 * (size_t)g]) v.hsum[4 * (size_t)g] = ev;
-        }
+    }
+    // a human exposed today turns infectious on day t + iip + 1 at the earliest
+    if (__any(infected) && lane == 0) {
+        const uint32_t ev = t + sd.iip_min + 1;
+        if (ev < v.hsum[4 * (size_t)g]) v.hsum[4 * (size_t)g] = ev;
     }
 }
 
