@@ -58,28 +58,30 @@ static_assert(kLifeBuckets <= kGuide, "guide table holds every life bucket");
 static_assert(ARBO_MAX_COMMUNITY_SLOTS <= (1u << kKeyBits) && ARBO_MAX_COMMUNITY_HUMANS <= (1u << kKeyBits),
               "member index must fit the claim key");
 
+// Wavefront reductions, every lane active: DPP within each row of 16 lanes
+// (quad_perm 1,0,3,2; quad_perm 2,3,0,1; row_half_mirror; row_mirror), then
+// the four row results read from lanes 0, 16, 32 and 48.  The result is
+// wave-uniform.
+template <class Op>
+__device__ __forceinline__ uint32_t wave_reduce(uint32_t x, Op op) {
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));
+    return op(op((uint32_t)__builtin_amdgcn_readlane((int)x, 0), (uint32_t)__builtin_amdgcn_readlane((int)x, 16)),
+              op((uint32_t)__builtin_amdgcn_readlane((int)x, 32), (uint32_t)__builtin_amdgcn_readlane((int)x, 48)));
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
+    return wave_reduce(x, [](uint32_t a, uint32_t b) { return a + b; });
 }
 
 __device__ __forceinline__ uint32_t wave_max(uint32_t x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t y = __shfl_xor(x, o, 64);
-        x = y > x ? y : x;
-    }
-    return x;
+    return wave_reduce(x, [](uint32_t a, uint32_t b) { return a > b ? a : b; });
 }
 
 __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t y = __shfl_xor(x, o, 64);
-        x = y < x ? y : x;
-    }
-    return x;
+    return wave_reduce(x, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
 }
 
 // Exclusive prefix sum over the wavefront; `total` gets the wave sum.
