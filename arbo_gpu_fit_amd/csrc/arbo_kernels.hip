@@ -1,9 +1,10 @@
 // arbo_kernels.hip — the day-loop kernels of the dengue IBM for gfx950.
 //
 // One simulated day is two launches over every community of every simulation
-// of the batch, one wave64 workgroup per community (up to 32 communities in
-// flight per CU, so the dependent-latency chain of one community is hidden by
-// the others):
+// of the batch, one wave64 workgroup per community (several communities in
+// flight per SIMD, so the dependent-latency chain of one community is hidden
+// by the others), plus a small per-simulation long-distance kernel between
+// them:
 //
 //  k_day_demography  ndemographic.cu:32-105 + mdemographic.cu:22-150 +
 //                    nVisitingInfected/mVisitingInfected (nepidemic.cu:28-142,
@@ -21,13 +22,16 @@
 //
 //  k_day_events      mepidemic.cu:143-246 (nmTransmission) +
 //                    nepidemic.cu:146-319 (mnTransmission)
-//     gathers the event counts of the community's in-edges, draws its
-//     long-distance arrivals, orders all its events (serotype rotation,
-//     source, index) and applies them in chunks with an LDS claim-min hash
-//     table: the earliest event whose target was eligible at the start of the
-//     chunk wins, which is exactly the serial outcome.  Mosquito targets are
-//     the q-th alive slot, selected from the alive bitmap without touching the
-//     census.  No global atomics on the census.
+//     gathers the event counts of the community's in-edges and its
+//     long-distance arrivals, orders each direction's events (serotype
+//     rotation, source, index) and applies them in chunks with an LDS
+//     claim-min hash table: the earliest event whose target was eligible at
+//     the start of the chunk wins, which is exactly the serial outcome.
+//     Mosquito targets are the q-th alive slot, found from the alive bitmap
+//     through a coarse rank index.  Events onto mosquitoes and onto humans
+//     have disjoint targets, so they share one chunk sequence (and, without
+//     external introductions, so do the serotypes).  No global atomics on
+//     the census.
 //
 // All randomness is Philox keyed by (seed; entity, day, purpose, index), so a
 // simulation's results do not depend on launch shape or batch composition.
