@@ -59,6 +59,7 @@ struct SimDesc {
     uint32_t count_law;           // ARBO_COUNT_POISSON or ARBO_COUNT_STOCH_ROUND
     float no_distance;            // fraction of events staying at the source (communities with kCommStay)
     uint64_t series_off;          // this simulation's block in the series array
+    uint64_t spatial_off;         // its block [(T+1)][S][C] of the per-community series (if kept)
 };
 
 // Series block of one simulation with room for T days (T+1 rows):
@@ -101,6 +102,7 @@ struct EngineView {
     unsigned long long* lam;       // [2 parity][n_sims][8] long-distance events of the day: the rate in
                                    // 2^-24 units (Poisson counts) or the count (stochastic rounding)
     uint32_t* series;
+    uint32_t* spatial;             // per-community infectious humans [(T+1)][S][C] per simulation, or null
     const uint32_t* csr_inbox;     // per CSR entry: inbox row (destination * 9 + in-edge index)
     const uint32_t* csr_thr;       // bits 0..24 threshold, 27..31 alias slot
     const uint32_t* in_edge;       // global source community

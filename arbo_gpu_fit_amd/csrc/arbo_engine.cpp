@@ -41,6 +41,7 @@ struct SimHost {
     std::vector<uint32_t> humans, slots;
     std::vector<uint64_t> h_off, m_off;    // global padded offsets per community
     uint64_t series_off = 0;
+    uint64_t spatial_off = 0;
 };
 
 }  // namespace
@@ -57,7 +58,7 @@ struct ArboEngine {
     // device memory
     DevBuf<CommDesc> comm;
     DevBuf<SimDesc> simd;
-    DevBuf<uint32_t> h_word, alive, cnt, nld, active, hsum, series, csr_thr, csr_inbox, in_edge, in_slot, tables, day_ctr;
+    DevBuf<uint32_t> h_word, alive, cnt, nld, active, hsum, series, spatial, csr_thr, csr_inbox, in_edge, in_slot, tables, day_ctr;
     DevBuf<uint16_t> h_next, m_word, m_next;
     DevBuf<int32_t> h_birth;
     DevBuf<unsigned long long> lam;
@@ -122,7 +123,7 @@ void release_graph(ArboEngine* e) {
 void free_batch(ArboEngine* e) {
     release_graph(e);
     dev_free(e->comm); dev_free(e->simd);
-    dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->active); dev_free(e->hsum); dev_free(e->series); dev_free(e->csr_thr); dev_free(e->csr_inbox);
+    dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->active); dev_free(e->hsum); dev_free(e->series); dev_free(e->spatial); dev_free(e->csr_thr); dev_free(e->csr_inbox);
     dev_free(e->in_edge); dev_free(e->in_slot); dev_free(e->tables); dev_free(e->day_ctr);
     dev_free(e->h_next); dev_free(e->m_word); dev_free(e->m_next); dev_free(e->h_birth);
     dev_free(e->lam); dev_free(e->dens);
@@ -287,7 +288,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     std::vector<float> dens_pool;
     std::map<std::vector<uint32_t>, uint32_t> seen_u32;
     std::map<std::vector<float>, uint32_t> seen_f32;
-    uint64_t h_tot = 0, m_tot = 0, ser_tot = 0;
+    uint64_t h_tot = 0, m_tot = 0, ser_tot = 0, spa_tot = 0;
     uint32_t max_n = 0, max_cap = 0, death_m_lds = 1;
     for (uint32_t i = 0; i < n_sims; ++i) {
         const SimTables& t = tabs[i];
@@ -304,6 +305,8 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         sh.m_off.resize(t.C);
         sh.series_off = ser_tot;
         ser_tot += series_words(max_days, t.S);
+        sh.spatial_off = spa_tot;
+        if (e->flags & ARBO_F_SPATIAL) spa_tot += ((uint64_t)max_days + 1) * t.S * t.C;
 
         SimDesc& sd = e->sim_desc[i];
         std::memset(&sd, 0, sizeof sd);
@@ -334,6 +337,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         sd.no_distance = p.no_distance;
         death_m_lds = std::max(death_m_lds, t.death_m_max + 1);
         sd.series_off = sh.series_off;
+        sd.spatial_off = sh.spatial_off;
 
         const uint32_t row_base = (uint32_t)csr_thr.size();
         for (uint32_t k = 0; k < t.thr.size(); ++k) csr_thr.push_back(t.thr[k] | (t.alias[k] << 27));
@@ -398,7 +402,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         (rc = dev_alloc(e, e->h_birth, h_tot)) || (rc = dev_alloc(e, e->m_word, m_tot)) ||
         (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kInboxEdges * 8)) || (rc = dev_alloc(e, e->nld, comm.size() * 8)) ||
         (rc = dev_alloc(e, e->hsum, comm.size() * 4)) || (rc = dev_alloc(e, e->active, comm.size())) ||
-        (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->series, ser_tot)) ||
+        (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->series, ser_tot)) || (rc = dev_alloc(e, e->spatial, spa_tot)) ||
         (rc = dev_alloc(e, e->csr_thr, csr_thr.size())) || (rc = dev_alloc(e, e->csr_inbox, csr_inbox.size())) || (rc = dev_alloc(e, e->in_edge, in_edge.size())) ||
         (rc = dev_alloc(e, e->in_slot, in_slot.size())) || (rc = dev_alloc(e, e->tables, table_pool.size())) ||
         (rc = dev_alloc(e, e->dens, dens_pool.size())) || (rc = dev_alloc(e, e->day_ctr, 1))) {
@@ -414,6 +418,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     HIP_TRY(e, hipMemcpy(e->tables.p, table_pool.data(), table_pool.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(e, hipMemcpy(e->dens.p, dens_pool.data(), dens_pool.size() * 4, hipMemcpyHostToDevice));
     HIP_TRY(e, launch_fill(e->series.p, ser_tot, 0u, e->stream));
+    if (spa_tot) HIP_TRY(e, launch_fill(e->spatial.p, spa_tot, 0u, e->stream));
     HIP_TRY(e, launch_fill(reinterpret_cast<uint32_t*>(e->lam.p), (uint64_t)2 * n_sims * 8 * 2, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->day_ctr.p, 1, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->nld.p, comm.size() * 8, 0u, e->stream));
@@ -439,6 +444,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.nld = e->nld.p;
     v.lam = e->lam.p;
     v.series = e->series.p;
+    v.spatial = (e->flags & ARBO_F_SPATIAL) ? e->spatial.p : nullptr;
     v.csr_inbox = e->csr_inbox.p;
     v.csr_thr = e->csr_thr.p;
     v.in_edge = e->in_edge.p;
@@ -624,6 +630,17 @@ int arbo_read_series(ArboEngine* e, uint32_t sim, ArboSeries* out) {
     return ARBO_OK;
 }
 
+
+int arbo_read_spatial(ArboEngine* e, uint32_t sim, uint32_t* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    if (!e->ready || sim >= e->n_sims) return fail(e, ARBO_E_INVALID, "bad simulation index");
+    if (!(e->flags & ARBO_F_SPATIAL)) return fail(e, ARBO_E_STATE, "engine created without ARBO_F_SPATIAL");
+    const SimHost& s = e->sims[sim];
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    const size_t words = ((size_t)e->day + 1) * s.S * s.C;
+    HIP_TRY(e, hipMemcpy(out, e->spatial.p + s.spatial_off, words * 4, hipMemcpyDeviceToHost));
+    return ARBO_OK;
+}
 
 int arbo_read_state(ArboEngine* e, uint32_t sim, ArboState* out) {
     if (!e || !out) return ARBO_E_INVALID;

@@ -258,6 +258,7 @@ __global__ __launch_bounds__(kWave) void k_init(EngineView v) {
         const uint32_t ih = unpack_pair(ih01, ih23, lane);
         atomicAdd(&v.series[si.n_inf + lane], ih);
         if (cd.c == 0) v.series[si.n_one + lane] = ih;
+        if (v.spatial) v.spatial[sd.spatial_off + (uint64_t)lane * sd.C + cd.c] = ih;   // row 0
     }
 }
 
@@ -714,6 +715,7 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
         if (ih) atomicAdd(&v.series[si.n_inf + (size_t)row * S + lane], ih);
         if (ic) atomicAdd(&v.series[si.n_inc + (size_t)row * S + lane], ic);
         if (cd.c == 0) v.series[si.n_one + (size_t)row * S + lane] = ih;
+        if (v.spatial) v.spatial[sd.spatial_off + ((uint64_t)row * S + lane) * sd.C + cd.c] = ih;
     }
 }
 

@@ -1,5 +1,6 @@
-// arbo_output.cpp — the nData time-series file of writeIncidence
-// (writeoutput.cpp:13-98) from a simulation's series.
+// arbo_output.cpp — the reference's output files from a simulation's series:
+// nData of writeIncidence (writeoutput.cpp:13-98) and nSpatial of
+// writeSpatial (writeoutput.cpp:194-241).
 #include <cstdio>
 #include <string>
 
@@ -31,6 +32,35 @@ ARBO_API int arbo_write_ndata(const char* path, const ArboSeries* s, uint32_t ro
         out += "," + std::to_string(sim_run);
     }
     const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    return (std::fclose(f) == 0 && ok) ? ARBO_OK : ARBO_E_INVALID;
+}
+
+ARBO_API int arbo_write_nspatial(const char* path, const uint32_t* sp, uint32_t rows, uint32_t S, uint32_t C,
+                                 uint32_t sim_run) {
+    if (!path || !sp || rows == 0 || C == 0 || S == 0 || S > ARBO_MAX_SEROTYPES) return ARBO_E_INVALID;
+    FILE* f = std::fopen(path, sim_run == 0 ? "wb" : "ab");
+    if (!f) return ARBO_E_INVALID;
+    bool ok = true;
+    std::string out;
+    if (sim_run == 0) {
+        out += "t,subPop";
+        for (uint32_t v = 0; v < S; ++v) out += ",prevalence.DENV" + std::to_string(v + 1);
+        out += ",simRun";
+    }
+    const std::string run = "," + std::to_string(sim_run);
+    for (uint32_t t = 0; t < rows && ok; ++t) {
+        const std::string ts = "\n" + std::to_string(t) + ",";
+        for (uint32_t c = 0; c < C; ++c) {
+            out += ts + std::to_string(c);
+            for (uint32_t v = 0; v < S; ++v) out += "," + std::to_string(sp[((size_t)t * S + v) * C + c]);
+        }
+        out += run;   // once per day, after its last community (writeoutput.cpp:237)
+        if (out.size() > (1u << 24)) {   // write in pieces: a long run has many lines
+            ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+            out.clear();
+        }
+    }
+    if (ok && !out.empty()) ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
     return (std::fclose(f) == 0 && ok) ? ARBO_OK : ARBO_E_INVALID;
 }
 

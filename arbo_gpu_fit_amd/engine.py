@@ -158,6 +158,15 @@ class Engine:
                        _ptr(arrs["m_next"], C.c_uint16))
         self._check(self.lib.arbo_write_state(self._h, sim, C.byref(st)))
 
+    def spatial(self, sim: int) -> np.ndarray:
+        """Infectious humans per day, serotype and community,
+        [days + 1, serotypes, communities] (engines created with
+        abi.F_SPATIAL)."""
+        C_, _, _, S = self.sizes(sim)
+        out = np.zeros((self.day + 1, S, C_), np.uint32)
+        self._check(self.lib.arbo_read_spatial(self._h, sim, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
+
     def kernel_time(self, kind: int):
         ms, n = C.c_double(), C.c_uint64()
         self._check(self.lib.arbo_kernel_time(self._h, kind, C.byref(ms), C.byref(n)))

@@ -1,4 +1,5 @@
-"""ctypes mirror of include/arbo_output.h: the reference's nData files."""
+"""ctypes mirror of include/arbo_output.h: the reference's nData and
+nSpatial files."""
 from __future__ import annotations
 
 import ctypes as C
@@ -10,6 +11,8 @@ from .engine import ArboError
 
 PROTOTYPES = {
     "arbo_write_ndata": (C.c_int, [C.c_char_p, C.POINTER(abi.Series), C.c_uint32, C.c_uint32, C.c_uint32]),
+    "arbo_write_nspatial": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_uint32]),
 }
 
 
@@ -33,3 +36,15 @@ def write_ndata(path: str, series: dict, sim_run: int, lib: C.CDLL | None = None
     rc = lib.arbo_write_ndata(path.encode(), C.byref(s), rows, S, sim_run)
     if rc:
         raise ArboError(rc, f"arbo_write_ndata({path})")
+
+
+def write_nspatial(path: str, spatial: np.ndarray, sim_run: int, lib: C.CDLL | None = None):
+    """Write (sim_run 0) or append one run's per-community series
+    (Engine.spatial: [rows, serotypes, communities]) to an nSpatial file
+    (writeoutput.cpp:194-241)."""
+    lib = bind(lib if lib is not None else abi.load_hip_library())
+    a = np.ascontiguousarray(spatial, dtype=np.uint32)
+    rows, S, Cn = a.shape
+    rc = lib.arbo_write_nspatial(path.encode(), a.ctypes.data_as(C.POINTER(C.c_uint32)), rows, S, Cn, sim_run)
+    if rc:
+        raise ArboError(rc, f"arbo_write_nspatial({path})")

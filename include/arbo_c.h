@@ -62,6 +62,7 @@ typedef struct ArboEngineDesc {
 
 #define ARBO_F_GRAPH 1u        /* capture day loops in hipGraphs */
 #define ARBO_F_TIMING 2u       /* time each day kernel with HIP events */
+#define ARBO_F_SPATIAL 4u      /* keep the per-community series of arbo_read_spatial */
 
 /* Model parameters of one simulation: the fields of SimulationParam that reach
  * the kernels (SimulationParam.h:16-54) plus the seasonal, incubation and
@@ -186,6 +187,13 @@ ARBO_API int arbo_write_state(ArboEngine* e, uint32_t sim, const ArboState* in);
 /* Set the day counter of the whole batch (resume from a checkpoint written by
  * arbo_read_state at that day); series rows after `day` are cleared. */
 ARBO_API int arbo_set_day(ArboEngine* e, uint32_t day);
+
+/* Infectious humans per community and serotype at the start of every day
+ * simulated so far (engines created with ARBO_F_SPATIAL): out[(t * S + s) * C
+ * + c] for t = 0..days, the layout of d_nInfectedSubPopCountSeries
+ * (datacollect.cu:10-23) that writeSpatial prints (writeoutput.cpp:194-241);
+ * out holds (days + 1) * S * C words. */
+ARBO_API int arbo_read_spatial(ArboEngine* e, uint32_t sim, uint32_t* out);
 
 /* Kernel timing (ARBO_F_TIMING): total milliseconds and launches of the
  * demography kernel (kind 0) and event-application kernel (kind 1) since the

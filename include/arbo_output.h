@@ -7,6 +7,13 @@
  * prevalence_DENV1..S_random_community, simRun; one row per day 0..rows-1,
  * each written after a newline (the file has no trailing newline).
  *
+ * arbo_write_nspatial replaces writeSpatial (writeoutput.cpp:194-241): the
+ * nSpatial<mcmcRun>.csv file of infectious humans per community and
+ * serotype.  Header t, subPop, prevalence.DENV1..S, simRun; then one line
+ * per day and community (t, subPop, one value per serotype), where, as in
+ * the reference, the simRun column is printed only after the last
+ * community of each day.
+ *
  * Conventions as include/arbo_c.h.
  */
 #ifndef ARBO_OUTPUT_H
@@ -24,6 +31,11 @@ extern "C" {
  * serotypes; sim_run 0 creates the file, any other appends. */
 ARBO_API int arbo_write_ndata(const char* path, const ArboSeries* series, uint32_t rows, uint32_t serotypes,
                               uint32_t sim_run);
+
+/* spatial: rows * serotypes * communities words as read by
+ * arbo_read_spatial; sim_run 0 creates the file, any other appends. */
+ARBO_API int arbo_write_nspatial(const char* path, const uint32_t* spatial, uint32_t rows, uint32_t serotypes,
+                                 uint32_t communities, uint32_t sim_run);
 
 #ifdef __cplusplus
 }

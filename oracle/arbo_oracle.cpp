@@ -81,6 +81,7 @@ struct Sim {
     std::vector<uint32_t> nld;      // [C][2][4] long-distance arrivals of the day
     // series
     std::vector<uint32_t> n_count, m_count, n_inf, n_inc, n_one;
+    std::vector<uint32_t> spatial;   // [(T+1)][S][C] infectious humans per community (ARBO_F_SPATIAL)
 };
 
 }  // namespace
@@ -92,6 +93,7 @@ struct ArboEngine {
     bool ready = false;
     std::string err;
     ArboTuning tuning{4, 4, 0, 8};   // launch knobs: accepted and reported, no effect on the CPU
+    uint32_t flags = 0;
 };
 
 namespace {
@@ -313,6 +315,7 @@ void init_sim(Sim& s, uint32_t max_days) {
     s.n_inf.assign((size_t)(max_days + 1) * S, 0);
     s.n_inc.assign((size_t)(max_days + 1) * S, 0);
     s.n_one.assign((size_t)(max_days + 1) * S, 0);
+    s.spatial.clear();
 }
 
 // t=0 census of one community (demographicintial.cu:86-297 and
@@ -398,6 +401,9 @@ void record_row(Sim& s, uint32_t t) {
         s.n_inc[(size_t)t * S + v] = (uint32_t)inc[v];
         s.n_one[(size_t)t * S + v] = s.Ih[v];
     }
+    if (!s.spatial.empty())   // d_nInfectedSubPopCountSeries (datacollect.cu:10-23)
+        for (uint32_t v = 0; v < S; ++v)
+            for (uint32_t c = 0; c < s.C; ++c) s.spatial[((size_t)t * S + v) * s.C + c] = s.Ih[(size_t)c * S + v];
 }
 
 // Daily event count of source community c, direction dir and serotype v with
@@ -684,6 +690,7 @@ int arbo_create(const ArboEngineDesc* desc, ArboEngine** out) {
     if (!desc || desc->abi_version != ARBO_ABI_VERSION) return ARBO_E_INVALID;
     ArboEngine* e = new ArboEngine();
     e->threads = (int)desc->threads;
+    e->flags = desc->flags;
     *out = e;
     return ARBO_OK;
 }
@@ -726,6 +733,8 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes,
     const int nt = thread_count(e);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
     for (int i = 0; i < (int)n_sims; ++i) init_sim(e->sims[i], max_days);
+    if (e->flags & ARBO_F_SPATIAL)
+        for (Sim& s : e->sims) s.spatial.assign(((size_t)max_days + 1) * s.S * s.C, 0u);
     if ((int)n_sims >= 2 * nt) {
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
         for (int i = 0; i < (int)n_sims; ++i) {
@@ -787,6 +796,15 @@ int arbo_read_series(ArboEngine* e, uint32_t sim, ArboSeries* out) {
     if (out->n_infected) std::memcpy(out->n_infected, s.n_inf.data(), rows * s.S * 4);
     if (out->n_incidence) std::memcpy(out->n_incidence, s.n_inc.data(), rows * s.S * 4);
     if (out->n_one_subpop) std::memcpy(out->n_one_subpop, s.n_one.data(), rows * s.S * 4);
+    return ARBO_OK;
+}
+
+int arbo_read_spatial(ArboEngine* e, uint32_t sim, uint32_t* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    if (!e->ready || sim >= e->sims.size()) return fail(e, ARBO_E_INVALID, "bad simulation index");
+    if (!(e->flags & ARBO_F_SPATIAL)) return fail(e, ARBO_E_STATE, "engine created without ARBO_F_SPATIAL");
+    const Sim& s = e->sims[sim];
+    std::memcpy(out, s.spatial.data(), ((size_t)e->day + 1) * s.S * s.C * 4);
     return ARBO_OK;
 }
 

@@ -129,3 +129,22 @@ def test_reinit_reuses_engine(hip_lib):
             _compare_series(reused.series(k), fresh.series(k), f"re-init member {k}")
         fresh.close()
     reused.close()
+
+
+@pytest.mark.parametrize("name", ["lattice10", "batch_pair"])
+def test_spatial_series_matches_oracle(name, hip_lib, oracle_lib):
+    """The per-community series (ARBO_F_SPATIAL), bit for bit."""
+    from arbo_gpu_fit_amd import abi
+    print(f"[parity] spatial case={name}", flush=True)
+    specs, T = CASES[name]()
+    gpu = Engine(hip_lib, flags=abi.F_SPATIAL)
+    orc = Engine(oracle_lib, flags=abi.F_SPATIAL, threads=8)
+    for e in (gpu, orc):
+        e.init(specs, T)
+        e.step(T)
+    for i in range(len(specs)):
+        a, b = gpu.spatial(i), orc.spatial(i)
+        assert a.shape == b.shape and np.array_equal(a, b), f"sim {i}"
+        assert np.array_equal(a.sum(axis=2), gpu.series(i)["n_infected"])
+    gpu.close()
+    orc.close()

@@ -304,3 +304,25 @@ def test_tuning_calls_on_the_oracle(oracle_lib):
         assert np.array_equal(a.series(0)[k], b.series(0)[k])
     a.close()
     b.close()
+
+
+def test_spatial_series_is_the_per_community_count(oracle_lib):
+    """The per-community series (d_nInfectedSubPopCountSeries) sums to the
+    infectious series over communities, and community 0 is the
+    one-community series (datacollect.cu:10-47)."""
+    spec = SimSpec(5, 4, np.full(20, 400, np.uint32), configs.base_params(seed_frac_h=0.02, seed_frac_m=0.02), 3)
+    eng = Engine(oracle_lib, flags=abi.F_SPATIAL, threads=2)
+    eng.init([spec], 60)
+    eng.step(45)
+    sp = eng.spatial(0)
+    ser = eng.series(0)
+    assert sp.shape == (46, 4, 20)
+    assert np.array_equal(sp.sum(axis=2), ser["n_infected"])
+    assert np.array_equal(sp[:, :, 0], ser["n_one_subpop"])
+    assert sp.sum() > 0
+    eng.close()
+    plain = Engine(oracle_lib, threads=2)
+    plain.init([spec], 5)
+    with pytest.raises(ArboError):
+        plain.spatial(0)
+    plain.close()

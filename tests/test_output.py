@@ -32,3 +32,27 @@ def test_ndata_matches_reference_layout(tmp_path):
     for r, ser in enumerate(runs):
         write_ndata(path, ser, r)
     assert open(path).read() == reference_text(runs, S)
+
+
+def reference_spatial_text(runs):
+    """writeSpatial (writeoutput.cpp:194-241) restated: the simRun column only
+    after the last community of each day."""
+    S = runs[0].shape[1]
+    out = "t,subPop" + "".join(f",prevalence.DENV{v + 1}" for v in range(S)) + ",simRun"
+    for r, sp in enumerate(runs):
+        rows, S, C = sp.shape
+        for t in range(rows):
+            for c in range(C):
+                out += f"\n{t},{c}" + "".join(f",{sp[t, v, c]}" for v in range(S))
+            out += f",{r}"
+    return out
+
+
+def test_nspatial_matches_reference_layout(tmp_path):
+    from arbo_gpu_fit_amd.output import write_nspatial
+    rng = np.random.default_rng(2)
+    runs = [rng.integers(0, 3000, (6, 4, 5)), rng.integers(0, 3000, (6, 4, 5))]
+    path = str(tmp_path / "nSpatial0.csv")
+    for r, sp in enumerate(runs):
+        write_nspatial(path, sp, r)
+    assert open(path).read() == reference_spatial_text(runs)
