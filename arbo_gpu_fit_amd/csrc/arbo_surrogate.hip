@@ -140,7 +140,8 @@ __global__ void k_gather(const float* __restrict__ x, const uint32_t* __restrict
 // and bfloat16), loss partial sums per workgroup into loss[blockIdx.x]
 __global__ __launch_bounds__(256) void k_poisson(const float* __restrict__ f, const float* __restrict__ y,
                                                  uint64_t n, float scale, float* __restrict__ df,
-                                                 uint16_t* __restrict__ dfh, double* __restrict__ part) {
+                                                 uint16_t* __restrict__ dfh, double* __restrict__ part,
+                                                 double* __restrict__ acc) {
     __shared__ double s[256];
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double l = 0.0;
@@ -157,7 +158,10 @@ __global__ __launch_bounds__(256) void k_poisson(const float* __restrict__ f, co
         if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s[0];
+        if (acc) atomicAdd(acc, s[0]);   // running sum over an epoch
+    }
 }
 
 // db[j] = sum_i dz[i, j]
@@ -219,6 +223,7 @@ struct ArboSurrogate {
     std::vector<Buf<uint16_t>> dZh; // bfloat16 gradients
     Buf<float> xin, yin, dxin;
     Buf<double> part;
+    Buf<double> epoch_sum;          // the loss summed over one training epoch
     Buf<float> data_x, data_y, by;
     Buf<uint32_t> idx;
     uint64_t adam_t = 0;
@@ -366,13 +371,13 @@ int upload_x(ArboSurrogate* s, uint32_t n, const float* x) {
 
 // Poisson loss and output gradient for B rows with targets in `y` (device);
 // the loss sum is returned through *loss (synchronises).
-int poisson(ArboSurrogate* s, uint32_t B, const float* y, float scale, double* loss) {
+int poisson(ArboSurrogate* s, uint32_t B, const float* y, float scale, double* loss, double* acc = nullptr) {
     const uint32_t out = s->w[s->L];
     const uint64_t n = (uint64_t)B * out;
     const uint32_t nb = blocks(n, 256);
     if (int rc = alloc(s, s->part, nb)) return rc;
     hipLaunchKernelGGL(k_poisson, dim3(nb), dim3(256), 0, s->stream, s->Z[s->L - 1].p, y, n, scale,
-                       s->dZ[s->L - 1].p, s->dZh[s->L - 1].p, s->part.p);
+                       s->dZ[s->L - 1].p, s->dZh[s->L - 1].p, s->part.p, acc);
     SUR_TRY(s, hipGetLastError());
     if (loss) {
         std::vector<double> h(nb);
@@ -564,6 +569,7 @@ ARBO_API int arbo_sur_train(ArboSurrogate* s, uint32_t n, const float* x, const 
     if (!rc) rc = alloc(s, s->data_y, (size_t)n * out);
     if (!rc) rc = alloc(s, s->by, (size_t)batch * out);
     if (!rc) rc = alloc(s, s->idx, n);
+    if (!rc) rc = alloc(s, s->epoch_sum, 1);
     if (rc) return rc;
     SUR_TRY(s, hipMemcpyAsync(s->data_x.p, x, (size_t)n * in * 4, hipMemcpyHostToDevice, s->stream));
     SUR_TRY(s, hipMemcpyAsync(s->data_y.p, y, (size_t)n * out * 4, hipMemcpyHostToDevice, s->stream));
@@ -580,7 +586,7 @@ ARBO_API int arbo_sur_train(ArboSurrogate* s, uint32_t n, const float* x, const 
         }
         SUR_TRY(s, hipStreamSynchronize(s->stream));   // perm is reused below
         SUR_TRY(s, hipMemcpyAsync(s->idx.p, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s->stream));
-        double tot = 0.0;
+        if (epoch_loss) SUR_TRY(s, hipMemsetAsync(s->epoch_sum.p, 0, sizeof(double), s->stream));
         for (uint32_t b0 = 0; b0 < n; b0 += batch) {
             const uint32_t B = std::min(batch, n - b0);
             hipLaunchKernelGGL(k_gather, dim3(blocks((uint64_t)B * in, 256)), dim3(256), 0, s->stream, s->data_x.p,
@@ -591,14 +597,18 @@ ARBO_API int arbo_sur_train(ArboSurrogate* s, uint32_t n, const float* x, const 
                                s->Xh[0].p, (uint64_t)B * in);
             SUR_TRY(s, hipGetLastError());
             if ((rc = forward(s, B))) return rc;
-            double l = 0.0;
-            const bool want = epoch_loss != nullptr && b0 + batch >= n;   // one sync per epoch
-            if ((rc = poisson(s, B, s->by.p, 1.0f / ((float)B * out), want ? &l : nullptr))) return rc;
-            if (want) tot = l / ((double)B * out);
+            // each batch's loss before its update, summed on the device
+            if ((rc = poisson(s, B, s->by.p, 1.0f / ((float)B * out), nullptr, epoch_loss ? s->epoch_sum.p : nullptr)))
+                return rc;
             if ((rc = backward(s, B, true, false))) return rc;
             if ((rc = adam_step(s, lr))) return rc;
         }
-        if (epoch_loss) epoch_loss[e] = tot;
+        if (epoch_loss) {   // one sync per epoch
+            double tot = 0.0;
+            SUR_TRY(s, hipMemcpyAsync(&tot, s->epoch_sum.p, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+            SUR_TRY(s, hipStreamSynchronize(s->stream));
+            epoch_loss[e] = tot / ((double)n * out);
+        }
     }
     SUR_TRY(s, hipStreamSynchronize(s->stream));
     return ARBO_OK;
@@ -622,6 +632,7 @@ ARBO_API void arbo_sur_destroy(ArboSurrogate* s) {
     release(s->yin);
     release(s->dxin);
     release(s->part);
+    release(s->epoch_sum);
     release(s->data_x);
     release(s->data_y);
     release(s->by);

@@ -59,10 +59,12 @@ def simulate_weekly(engine: Engine, thetas: np.ndarray, seed: int, batch: int = 
 
 
 def train_surrogate(thetas: np.ndarray, weekly: np.ndarray, epochs: int = 200, batch: int = 512,
-                    lr: float = 1e-3, seed: int = 1, lib=None) -> tuple[Surrogate, np.ndarray]:
+                    lr: float = 1e-3, seed: int = 1, lib=None, stage: int = 10) -> tuple[Surrogate, np.ndarray]:
     """The MLP (6 -> 256^4 -> n_weeks, GELU) fitted to the weekly incidence
     by the Poisson loss (Adam), its output bias started at the data's
-    log-mean per week."""
+    log-mean per week.  The learning rate follows a cosine decay from `lr` to
+    lr / 100, changed every `stage` epochs (the optimiser state carries over
+    between stages)."""
     widths = list(SURROGATE_WIDTHS[:-1]) + [weekly.shape[1]]
     sur = Surrogate(widths, max_batch=max(batch, 4096), seed=seed, lib=lib)
     x = features(thetas, configs.mcmc_priors()).astype(np.float32)
@@ -70,8 +72,13 @@ def train_surrogate(thetas: np.ndarray, weekly: np.ndarray, epochs: int = 200, b
     p = sur.get_params()
     p[-weekly.shape[1]:] = np.log(y.mean(axis=0) + 1e-2)
     sur.set_params(p)
-    hist = sur.train(x, y, epochs=epochs, batch=batch, lr=lr, seed=seed)
-    return sur, hist
+    hist = []
+    for e0 in range(0, epochs, stage):
+        n = min(stage, epochs - e0)
+        frac = (e0 + 0.5 * n) / epochs
+        rate = lr * (0.01 + 0.99 * 0.5 * (1.0 + math.cos(math.pi * frac)))
+        hist.append(sur.train(x, y, epochs=n, batch=batch, lr=rate, seed=seed + e0))
+    return sur, np.concatenate(hist)
 
 
 def summarise(m: mcmc.Mcmc, wall: float, burn: int) -> dict:

@@ -207,7 +207,7 @@ def main():
     ap.add_argument("--iters", type=int, default=160)
     ap.add_argument("--screen-iters", type=int, default=480, help="iterations of the screened delayed acceptance")
     ap.add_argument("--mcmc-seed", type=int, default=1)
-    ap.add_argument("--train-runs", type=int, default=16384)
+    ap.add_argument("--train-runs", type=int, default=50000)
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--sub-steps", type=int, default=10)
     ap.add_argument("--save", default="", help="save the surrogate's training set (npz)")

@@ -55,7 +55,8 @@ ARBO_API int arbo_sur_loss_grad(ArboSurrogate* s, uint32_t n, const float* x, co
 
 /* Adam (Kingma & Ba 2015; beta1 0.9, beta2 0.999, eps 1e-8) on the Poisson
  * loss over minibatches of a data set of n rows, reshuffled every epoch
- * (Philox seed); epoch_loss[epochs] gets each epoch's mean loss.  The
+ * (Philox seed); epoch_loss[epochs] gets each epoch's mean loss (every
+ * minibatch scored before its own update).  The
  * optimiser state persists across calls. */
 ARBO_API int arbo_sur_train(ArboSurrogate* s, uint32_t n, const float* x, const float* y, uint32_t epochs,
                             uint32_t batch, float lr, uint64_t seed, double* epoch_loss);
