@@ -26,6 +26,9 @@ COUNT_STOCH_ROUND = 1
 F_GRAPH = 1
 F_TIMING = 2
 F_SPATIAL = 4
+F_HISTORY = 8
+F_AOI = 16          # age-of-infection rings (keeps the history too)
+AOI_RECORD = 5000
 
 _u32 = C.c_uint32
 _f32 = C.c_float
@@ -93,6 +96,8 @@ PROTOTYPES = {
     "arbo_write_state": (C.c_int, [EnginePtr, _u32, C.POINTER(State)]),
     "arbo_set_day": (C.c_int, [EnginePtr, _u32]),
     "arbo_read_spatial": (C.c_int, [EnginePtr, _u32, _u32p]),
+    "arbo_read_history": (C.c_int, [EnginePtr, _u32, C.POINTER(C.c_int32)]),
+    "arbo_read_aoi": (C.c_int, [EnginePtr, _u32, _u32p]),
     "arbo_kernel_time": (C.c_int, [EnginePtr, _u32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "arbo_reset_timing": (C.c_int, [EnginePtr]),
     "arbo_tuning_candidates": (C.c_int, [C.POINTER(Tuning), _u32, _u32p]),

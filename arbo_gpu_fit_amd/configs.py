@@ -78,6 +78,14 @@ def weibull_scale_for_mean(mean: float, shape: float) -> float:
     return mean / math.gamma(1.0 + 1.0 / shape)
 
 
+def _set(p, name: str, value):
+    """Set one field; a name that is no field of the structure is an error
+    (ctypes would otherwise keep it as a plain attribute)."""
+    if name not in {f[0] for f in type(p)._fields_}:
+        raise AttributeError(f"{type(p).__name__} has no field {name!r}")
+    setattr(p, name, value)
+
+
 def base_params(**over) -> abi.Params:
     p = abi.Params()
     p.n_serotypes = 4
@@ -108,7 +116,7 @@ def base_params(**over) -> abi.Params:
     p.no_distance = 0.0
     p.count_law = abi.COUNT_POISSON
     for k, v in over.items():
-        setattr(p, k, v)
+        _set(p, k, v)
     return p
 
 
@@ -127,7 +135,7 @@ def reference_params(**over) -> abi.Params:
         m_ratio=1.5, season_amp=0.0, seed_frac_h=0.0, seed_frac_m=0.0,
         no_distance=0.75, count_law=abi.COUNT_STOCH_ROUND)
     for k, v in over.items():
-        setattr(p, k, v)
+        _set(p, k, v)
     return p
 
 
@@ -190,7 +198,7 @@ def mcmc_spec(theta: dict | None = None, seed: int = 3, **kw) -> SimSpec:
     g, humans = mcmc_shape(**kw)
     p = base_params()
     for k, v in (theta or {}).items():
-        setattr(p, k, v)
+        _set(p, k, v)
     return SimSpec(g, g, humans, p, seed)
 
 

@@ -19,6 +19,8 @@ constexpr uint32_t kCntPerComm = 72;      // event counts of a source: [dir 2][s
 constexpr uint32_t kInboxEdges = 9;       // in-edges per destination (Moore-8 + self)
 constexpr uint32_t kHumanAlign = 4;       // humans per 16-byte vector
 constexpr uint32_t kMosqAlign = 128;      // mosquito slots per 16-byte group of alive-bitmap words
+constexpr uint32_t kAoiRecord = 5000;     // ages per age-of-infection ring
+constexpr uint32_t kAoiWindow = 3650;     // rings record the days t with max_days - t <= this
 
 struct CommDesc {
     uint64_t h_off;       // global offset of the first human
@@ -60,6 +62,7 @@ struct SimDesc {
     float no_distance;            // fraction of events staying at the source (communities with kCommStay)
     uint64_t series_off;          // this simulation's block in the series array
     uint64_t spatial_off;         // its block [(T+1)][S][C] of the per-community series (if kept)
+    uint64_t h_begin, h_end;      // its humans: [h_begin, h_end) of the padded census, h_end a multiple of 4
 };
 
 // Series block of one simulation with room for T days (T+1 rows):
@@ -103,6 +106,12 @@ struct EngineView {
                                    // 2^-24 units (Poisson counts) or the count (stochastic rounding)
     uint32_t* series;
     uint32_t* spatial;             // per-community infectious humans [(T+1)][S][C] per simulation, or null
+    int32_t* hist;                 // per human 4 infection-history words (one per serotype), or null
+    uint8_t* aoi_mark;             // per human: 1 when a mosquito event infected it today (age-of-infection
+                                   // rings kept), else null
+    uint32_t* aoi;                 // [n_sims][4][kAoiRecord] age-of-infection rings
+    uint32_t* aoi_pos;             // [n_sims][4] next position of each ring
+    uint32_t aoi_from;             // first day whose infections enter the rings
     const uint32_t* csr_inbox;     // per CSR entry: inbox row (destination * 9 + in-edge index)
     const uint32_t* csr_thr;       // bits 0..24 threshold, 27..31 alias slot
     const uint32_t* in_edge;       // global source community

@@ -60,7 +60,9 @@ struct ArboEngine {
     DevBuf<SimDesc> simd;
     DevBuf<uint32_t> h_word, alive, cnt, nld, active, hsum, series, spatial, csr_thr, csr_inbox, in_edge, in_slot, tables, day_ctr;
     DevBuf<uint16_t> h_next, m_word, m_next;
-    DevBuf<int32_t> h_birth;
+    DevBuf<int32_t> h_birth, hist;
+    DevBuf<uint8_t> aoi_mark;
+    DevBuf<uint32_t> aoi, aoi_pos;
     DevBuf<unsigned long long> lam;
     DevBuf<float> dens;
     uint64_t series_total = 0;
@@ -125,7 +127,7 @@ void free_batch(ArboEngine* e) {
     dev_free(e->comm); dev_free(e->simd);
     dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->active); dev_free(e->hsum); dev_free(e->series); dev_free(e->spatial); dev_free(e->csr_thr); dev_free(e->csr_inbox);
     dev_free(e->in_edge); dev_free(e->in_slot); dev_free(e->tables); dev_free(e->day_ctr);
-    dev_free(e->h_next); dev_free(e->m_word); dev_free(e->m_next); dev_free(e->h_birth);
+    dev_free(e->h_next); dev_free(e->m_word); dev_free(e->m_next); dev_free(e->h_birth); dev_free(e->hist); dev_free(e->aoi_mark); dev_free(e->aoi); dev_free(e->aoi_pos);
     dev_free(e->lam); dev_free(e->dens);
     e->ready = false;
 }
@@ -389,6 +391,8 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
             max_n = std::max(max_n, cd.n);
             max_cap = std::max(max_cap, cd.cap);
         }
+        e->sim_desc[i].h_begin = sh.h_off[0];
+        e->sim_desc[i].h_end = align_up(h_tot, kHumanAlign);
     }
     h_tot = align_up(h_tot, kHumanAlign) + kHumanAlign;
     m_tot = align_up(m_tot, kMosqAlign) + ARBO_MAX_COMMUNITY_SLOTS;  // room for whole-workgroup vector reads
@@ -399,7 +403,11 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     int rc;
     if ((rc = dev_alloc(e, e->comm, comm.size())) || (rc = dev_alloc(e, e->simd, n_sims)) ||
         (rc = dev_alloc(e, e->h_word, h_tot)) || (rc = dev_alloc(e, e->h_next, h_tot)) ||
-        (rc = dev_alloc(e, e->h_birth, h_tot)) || (rc = dev_alloc(e, e->m_word, m_tot)) ||
+        (rc = dev_alloc(e, e->h_birth, h_tot)) ||
+        (rc = dev_alloc(e, e->hist, (e->flags & (ARBO_F_HISTORY | ARBO_F_AOI)) ? 4 * h_tot : 0)) ||
+        (rc = dev_alloc(e, e->aoi_mark, (e->flags & ARBO_F_AOI) ? h_tot : 0)) ||
+        (rc = dev_alloc(e, e->aoi, (e->flags & ARBO_F_AOI) ? (size_t)n_sims * 4 * kAoiRecord : 0)) ||
+        (rc = dev_alloc(e, e->aoi_pos, (e->flags & ARBO_F_AOI) ? (size_t)n_sims * 4 : 0)) || (rc = dev_alloc(e, e->m_word, m_tot)) ||
         (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kInboxEdges * 8)) || (rc = dev_alloc(e, e->nld, comm.size() * 8)) ||
         (rc = dev_alloc(e, e->hsum, comm.size() * 4)) || (rc = dev_alloc(e, e->active, comm.size())) ||
         (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->series, ser_tot)) || (rc = dev_alloc(e, e->spatial, spa_tot)) ||
@@ -427,6 +435,13 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     // padding words: humans beyond a community are never read as members, but
     // keep the whole array defined
     HIP_TRY(e, launch_fill(e->h_word.p, h_tot, 0u, e->stream));
+    if (e->flags & (ARBO_F_HISTORY | ARBO_F_AOI))
+        HIP_TRY(e, launch_fill(reinterpret_cast<uint32_t*>(e->hist.p), 4 * h_tot, 0xFFFFFFFFu, e->stream));
+    if (e->flags & ARBO_F_AOI) {   // h_tot is a multiple of 4: the marks fill whole words
+        HIP_TRY(e, launch_fill(reinterpret_cast<uint32_t*>(e->aoi_mark.p), h_tot / 4, 0u, e->stream));
+        HIP_TRY(e, launch_fill(e->aoi.p, (uint64_t)n_sims * 4 * kAoiRecord, 0u, e->stream));
+        HIP_TRY(e, launch_fill(e->aoi_pos.p, (uint64_t)n_sims * 4, 0u, e->stream));
+    }
     HIP_TRY(e, launch_fill(reinterpret_cast<uint32_t*>(e->m_word.p), m_tot / 2, 0x00030003u, e->stream));
 
     EngineView& v = e->view;
@@ -445,6 +460,11 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.lam = e->lam.p;
     v.series = e->series.p;
     v.spatial = (e->flags & ARBO_F_SPATIAL) ? e->spatial.p : nullptr;
+    v.hist = (e->flags & (ARBO_F_HISTORY | ARBO_F_AOI)) ? e->hist.p : nullptr;
+    v.aoi_mark = (e->flags & ARBO_F_AOI) ? e->aoi_mark.p : nullptr;
+    v.aoi = e->aoi.p;
+    v.aoi_pos = e->aoi_pos.p;
+    v.aoi_from = max_days > kAoiWindow ? max_days - kAoiWindow : 0u;
     v.csr_inbox = e->csr_inbox.p;
     v.csr_thr = e->csr_thr.p;
     v.in_edge = e->in_edge.p;
@@ -639,6 +659,35 @@ int arbo_read_spatial(ArboEngine* e, uint32_t sim, uint32_t* out) {
     HIP_TRY(e, hipStreamSynchronize(e->stream));
     const size_t words = ((size_t)e->day + 1) * s.S * s.C;
     HIP_TRY(e, hipMemcpy(out, e->spatial.p + s.spatial_off, words * 4, hipMemcpyDeviceToHost));
+    return ARBO_OK;
+}
+
+int arbo_read_history(ArboEngine* e, uint32_t sim, int32_t* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    if (!e->ready || sim >= e->n_sims) return fail(e, ARBO_E_INVALID, "bad simulation index");
+    if (!(e->flags & (ARBO_F_HISTORY | ARBO_F_AOI)))
+        return fail(e, ARBO_E_STATE, "engine created without ARBO_F_HISTORY");
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    const SimHost& s = e->sims[sim];
+    if (s.C == 0) return ARBO_OK;
+    // the simulation's padded block of 4 words per human, then [s][i] in community order
+    const uint64_t lo = s.h_off[0], hi = s.h_off[s.C - 1] + s.humans[s.C - 1];
+    std::vector<int32_t> blk(4 * (hi - lo));
+    HIP_TRY(e, hipMemcpy(blk.data(), e->hist.p + 4 * lo, blk.size() * 4, hipMemcpyDeviceToHost));
+    const uint64_t N = s.n_humans;
+    uint64_t k = 0;
+    for (uint32_t c = 0; c < s.C; ++c)
+        for (uint32_t i = 0; i < s.humans[c]; ++i, ++k)
+            for (uint32_t v = 0; v < s.S; ++v) out[v * N + k] = blk[4 * (s.h_off[c] - lo + i) + v];
+    return ARBO_OK;
+}
+
+int arbo_read_aoi(ArboEngine* e, uint32_t sim, uint32_t* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    if (!e->ready || sim >= e->n_sims) return fail(e, ARBO_E_INVALID, "bad simulation index");
+    if (!(e->flags & ARBO_F_AOI)) return fail(e, ARBO_E_STATE, "engine created without ARBO_F_AOI");
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    HIP_TRY(e, hipMemcpy(out, e->aoi.p + (size_t)sim * 4 * kAoiRecord, 4 * kAoiRecord * 4, hipMemcpyDeviceToHost));
     return ARBO_OK;
 }
 

@@ -230,6 +230,9 @@ __global__ __launch_bounds__(kWave) void k_init(EngineView v) {
         v.h_word[gi] = (dd << 8) | (imm << 4) | (strain << 2) | st;
         v.h_next[gi] = (uint16_t)nxt;
         v.h_birth[gi] = -(int32_t)(a0 + 1);
+        if (v.hist)   // immune at t = 0 (prior immunity or the seeded infection): -2, naive: -1
+            reinterpret_cast<int4*>(v.hist)[gi] = make_int4((imm & 1u) ? -2 : -1, (imm & 2u) ? -2 : -1,
+                                                            (imm & 4u) ? -2 : -1, (imm & 8u) ? -2 : -1);
     }
     for (uint32_t j = lane; j < cd.cap; j += kWave) {
         const uint64_t gm = cd.m_off + j;
@@ -315,6 +318,7 @@ __device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& c
             const uint32_t da = death_age(v.tables + sd.death_h_off, kHumanTable - 1, (r.x >> 8) + 1);
             v.h_word[gi] = (t + da) << 8;
             v.h_birth[gi] = (int32_t)t;
+            if (v.hist) reinterpret_cast<int4*>(v.hist)[gi] = make_int4(-1, -1, -1, -1);   // naive newborn
             hc.minev = t + da < hc.minev ? t + da : hc.minev;
             continue;
         }
@@ -955,6 +959,8 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                     const uint32_t h = k - kHumanKey;
                     const uint64_t gi = cd.h_off + h;
                     v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
+                    if (v.hist) v.hist[4 * gi + s] = (int32_t)t - v.h_birth[gi];   // age at infection
+                    if (v.aoi_mark && t >= v.aoi_from) v.aoi_mark[gi] = 1;
                     const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
                     v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
                     infected = true;
@@ -987,6 +993,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                     const uint32_t w = v.h_word[gi];
                     if ((w & 3u) == 0 && !((w >> (4 + s)) & 1u)) {
                         v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
+                        if (v.hist) v.hist[4 * gi + s] = (int32_t)t - v.h_birth[gi];
                         const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
                         v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
                         infected = true;
@@ -1050,6 +1057,116 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t spl
     if (part == 0 && tid < 8) v.lam[((size_t)((t + 1) & 1u) * v.n_sims + sim) * 8 + tid] = 0ull;
 }
 
+// Age-of-infection rings (nepidemic.cu:229-243), one workgroup per
+// simulation after the event kernel.  The reference appends each novel
+// infection by a mosquito event to ring k (k = serotypes the human had been
+// infected with before) at atomicInc positions, in whatever order threads
+// arrive.  Here a day's infections enter in human order, which makes the
+// rings deterministic: a counting pass gives the day's total per ring, and
+// the writing pass stores only the day's last kAoiRecord records of each
+// ring, so no slot is written twice in one launch.
+__global__ __launch_bounds__(256) void k_day_aoi(EngineView v) {
+    const uint32_t t = *v.day;
+    if (t < v.aoi_from) return;
+    const uint32_t sim = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    constexpr uint32_t kWaves = 256 / kWave, kChunk = 4 * 256;
+    const SimDesc& sd = v.sims[sim];
+    __shared__ uint32_t s_part[kWaves][4];
+    __shared__ uint32_t s_tot[4], s_pos[4];
+    // previous infections of the human infected today: its history words
+    // other than today's (one infection per human and day)
+    auto ring_of = [&](uint64_t gi) -> uint32_t {
+        const int4 h = reinterpret_cast<const int4*>(v.hist)[gi];
+        const uint32_t n = (uint32_t)(h.x != -1) + (uint32_t)(h.y != -1) + (uint32_t)(h.z != -1) + (uint32_t)(h.w != -1);
+        return n ? n - 1u : 0u;   // n >= 1: today's word is set
+    };
+    // counting pass
+    uint32_t c[4] = {0, 0, 0, 0};
+    for (uint64_t base = sd.h_begin; base < sd.h_end; base += kChunk) {
+        const uint64_t gi = base + 4 * tid;
+        if (gi >= sd.h_end) continue;
+        const uint32_t m = *reinterpret_cast<const uint32_t*>(v.aoi_mark + gi);
+        if (!m) continue;
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+            if (!((m >> (8 * e)) & 0xFFu)) continue;
+            const uint32_t r = ring_of(gi + e);
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) c[k] += r == k;   // constant indices: the counters stay in registers
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t w = wave_sum(c[k]);
+        if (lane == 0) s_part[wv][k] = w;
+    }
+    __syncthreads();
+    if (tid < 4) {
+        uint32_t n = 0;
+        for (uint32_t w = 0; w < kWaves; ++w) n += s_part[w][tid];
+        s_tot[tid] = n;
+        s_pos[tid] = v.aoi_pos[4 * sim + tid];
+    }
+    __syncthreads();
+    if ((s_tot[0] | s_tot[1] | s_tot[2] | s_tot[3]) == 0) return;
+    // writing pass: record j of the day (per ring) goes to slot (pos + j) mod kAoiRecord
+    uint32_t* ring = v.aoi + (uint64_t)sim * 4 * kAoiRecord;
+    uint32_t done[4] = {0, 0, 0, 0};   // records of the ring in earlier chunks (same in every thread)
+    for (uint64_t base = sd.h_begin; base < sd.h_end; base += kChunk) {
+        const uint64_t gi = base + 4 * tid;
+        uint32_t m = 0;
+        if (gi < sd.h_end) m = *reinterpret_cast<const uint32_t*>(v.aoi_mark + gi);
+        uint32_t k4[4] = {4, 4, 4, 4};
+        uint32_t mine[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+            if (!((m >> (8 * e)) & 0xFFu)) continue;
+            k4[e] = ring_of(gi + e);
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) mine[k] += k4[e] == k;
+        }
+        uint32_t before[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {   // exclusive prefix over the workgroup
+            uint32_t x = mine[k];
+#pragma unroll
+            for (uint32_t d = 1; d < kWave; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, kWave);
+                if (lane >= d) x += y;
+            }
+            if (lane == kWave - 1) s_part[wv][k] = x;
+            before[k] = x - mine[k];
+        }
+        __syncthreads();
+        uint32_t chunk_tot[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            uint32_t off = 0, all = 0;
+            for (uint32_t w = 0; w < kWaves; ++w) {
+                off += w < wv ? s_part[w][k] : 0u;
+                all += s_part[w][k];
+            }
+            before[k] += off + done[k];
+            chunk_tot[k] = all;
+        }
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                if (k4[e] != k) continue;
+                const uint32_t j = before[k]++;
+                if (j + kAoiRecord >= s_tot[k])   // among the day's last kAoiRecord records of ring k
+                    ring[k * kAoiRecord + (s_pos[k] + j) % kAoiRecord] = (uint32_t)((int32_t)t - v.h_birth[gi + e]);
+            }
+        }
+        if (m) *reinterpret_cast<uint32_t*>(v.aoi_mark + gi) = 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) done[k] += chunk_tot[k];
+        __syncthreads();   // s_part is rewritten by the next chunk
+    }
+    if (tid < 4) v.aoi_pos[4 * sim + tid] = (s_pos[tid] + s_tot[tid]) % kAoiRecord;
+}
+
 __global__ void k_advance_day(uint32_t* day) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *day = *day + 1u;
 }
@@ -1097,6 +1214,7 @@ hipError_t launch_demography(const EngineView& v, const Tuning& tu, hipStream_t 
 hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st) {
     if (tu.xcd & 2u) hipLaunchKernelGGL(k_day_events<true>, dim3(v.n_comm), dim3(kWave), 0, st, v);
     else hipLaunchKernelGGL(k_day_events<false>, dim3(v.n_comm), dim3(kWave), 0, st, v);
+    if (v.aoi_mark) hipLaunchKernelGGL(k_day_aoi, dim3(v.n_sims), dim3(256), 0, st, v);
     return hipGetLastError();
 }
 

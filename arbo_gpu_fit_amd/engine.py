@@ -167,6 +167,29 @@ class Engine:
         self._check(self.lib.arbo_read_spatial(self._h, sim, out.ctypes.data_as(C.POINTER(C.c_uint32))))
         return out
 
+    def history(self, sim: int) -> np.ndarray:
+        """Infection history [serotypes, humans]: -1 never infected, -2
+        immune from the t = 0 census, else the age in days at infection
+        (engines created with abi.F_HISTORY)."""
+        _, nh, _, S = self.sizes(sim)
+        out = np.zeros((S, nh), np.int32)
+        self._check(self.lib.arbo_read_history(self._h, sim, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def aoi(self, sim: int) -> np.ndarray:
+        """Age-of-infection rings [4, 5000]: ring k holds ages at infection
+        of humans with k earlier infections (engines created with
+        abi.F_AOI)."""
+        out = np.zeros((4, abi.AOI_RECORD), np.uint32)
+        self._check(self.lib.arbo_read_aoi(self._h, sim, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
+
+    def census(self, sim: int):
+        """(age, history) of every human after the last simulated day, as
+        writeCensus records them: age = (days done - 1) - birth day."""
+        hb = self.state(sim)["h_birth"].astype(np.int64)
+        return ((self.day - 1) - hb).astype(np.uint32), self.history(sim)
+
     def kernel_time(self, kind: int):
         ms, n = C.c_double(), C.c_uint64()
         self._check(self.lib.arbo_kernel_time(self._h, kind, C.byref(ms), C.byref(n)))

@@ -1,5 +1,5 @@
-"""ctypes mirror of include/arbo_output.h: the reference's nData and
-nSpatial files."""
+"""ctypes mirror of include/arbo_output.h: the reference's nData, nAlive,
+nAOI and nSpatial files."""
 from __future__ import annotations
 
 import ctypes as C
@@ -13,6 +13,9 @@ PROTOTYPES = {
     "arbo_write_ndata": (C.c_int, [C.c_char_p, C.POINTER(abi.Series), C.c_uint32, C.c_uint32, C.c_uint32]),
     "arbo_write_nspatial": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.c_uint32]),
+    "arbo_write_nalive": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_int32), C.c_uint32,
+                                    C.c_uint32, C.c_uint32]),
+    "arbo_write_naoi": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.c_uint32]),
 }
 
 
@@ -48,3 +51,31 @@ def write_nspatial(path: str, spatial: np.ndarray, sim_run: int, lib: C.CDLL | N
     rc = lib.arbo_write_nspatial(path.encode(), a.ctypes.data_as(C.POINTER(C.c_uint32)), rows, S, Cn, sim_run)
     if rc:
         raise ArboError(rc, f"arbo_write_nspatial({path})")
+
+
+def write_nalive(path: str, age: np.ndarray, history: np.ndarray, sim_run: int, lib: C.CDLL | None = None):
+    """Write (sim_run 0) or append one run's partial census (Engine.census:
+    ages [n] and history [serotypes, n]) to an nAlive file
+    (writeoutput.cpp:100-148): the first min(5000, n) humans."""
+    lib = bind(lib if lib is not None else abi.load_hip_library())
+    a = np.ascontiguousarray(age, dtype=np.uint32)
+    h = np.ascontiguousarray(history, dtype=np.int32)
+    S, n = h.shape
+    if a.shape != (n,):
+        raise ValueError(f"age has shape {a.shape}, history {h.shape}")
+    rc = lib.arbo_write_nalive(path.encode(), a.ctypes.data_as(C.POINTER(C.c_uint32)),
+                               h.ctypes.data_as(C.POINTER(C.c_int32)), n, S, sim_run)
+    if rc:
+        raise ArboError(rc, f"arbo_write_nalive({path})")
+
+
+def write_naoi(path: str, aoi: np.ndarray, sim_run: int, lib: C.CDLL | None = None):
+    """Write (sim_run 0) or append one run's age-of-infection rings
+    (Engine.aoi: [4, 5000]) to an nAOI file (writeoutput.cpp:152-191)."""
+    lib = bind(lib if lib is not None else abi.load_hip_library())
+    a = np.ascontiguousarray(aoi, dtype=np.uint32)
+    if a.shape != (4, abi.AOI_RECORD):
+        raise ValueError(f"aoi has shape {a.shape}, expected (4, {abi.AOI_RECORD})")
+    rc = lib.arbo_write_naoi(path.encode(), a.ctypes.data_as(C.POINTER(C.c_uint32)), sim_run)
+    if rc:
+        raise ArboError(rc, f"arbo_write_naoi({path})")

@@ -63,6 +63,9 @@ typedef struct ArboEngineDesc {
 #define ARBO_F_GRAPH 1u        /* capture day loops in hipGraphs */
 #define ARBO_F_TIMING 2u       /* time each day kernel with HIP events */
 #define ARBO_F_SPATIAL 4u      /* keep the per-community series of arbo_read_spatial */
+#define ARBO_F_HISTORY 8u      /* keep each human's infection history (arbo_read_history) */
+#define ARBO_F_AOI 16u         /* keep the age-of-infection rings of arbo_read_aoi (implies ARBO_F_HISTORY) */
+#define ARBO_AOI_RECORD 5000u  /* ages per ring (C_NAOIRECORD, constant.h:22) */
 
 /* Model parameters of one simulation: the fields of SimulationParam that reach
  * the kernels (SimulationParam.h:16-54) plus the seasonal, incubation and
@@ -194,6 +197,25 @@ ARBO_API int arbo_set_day(ArboEngine* e, uint32_t day);
  * (datacollect.cu:10-23) that writeSpatial prints (writeoutput.cpp:194-241);
  * out holds (days + 1) * S * C words. */
 ARBO_API int arbo_read_spatial(ArboEngine* e, uint32_t sim, uint32_t* out);
+
+/* Infection history of every human of a simulation (engines created with
+ * ARBO_F_HISTORY), the reference's d_nHistory: out[s * N + i] for serotype s
+ * and human i (community order), -1 if never infected by s, -2 if immune to s
+ * from the t = 0 census (prior immunity or a seeded infection), otherwise
+ * the age in days on the day of the infection (demographicintial.cu:164-169,
+ * epidemicinitial.cu:59, nepidemic.cu:248,298; a rebirth resets it,
+ * ndemographic.cu:100).  out holds S * N words. */
+ARBO_API int arbo_read_history(ArboEngine* e, uint32_t sim, int32_t* out);
+
+/* Age-of-infection rings of a simulation (engines created with ARBO_F_AOI),
+ * the reference's d_nAgeOfInfection (nepidemic.cu:229-243): every infection
+ * by a mosquito event (not an external introduction) on a day t with
+ * max_days - t <= 3650 writes the age at infection into ring k, k = the
+ * serotypes the human had been infected with before, at the ring's next
+ * position (cycling through ARBO_AOI_RECORD slots).  The infections of one
+ * day enter in human order (community order, then index).  Rings start at
+ * zero.  out[k * ARBO_AOI_RECORD + r], 4 * ARBO_AOI_RECORD words. */
+ARBO_API int arbo_read_aoi(ArboEngine* e, uint32_t sim, uint32_t* out);
 
 /* Kernel timing (ARBO_F_TIMING): total milliseconds and launches of the
  * demography kernel (kind 0) and event-application kernel (kind 1) since the

@@ -37,6 +37,21 @@ ARBO_API int arbo_write_ndata(const char* path, const ArboSeries* series, uint32
 ARBO_API int arbo_write_nspatial(const char* path, const uint32_t* spatial, uint32_t rows, uint32_t serotypes,
                                  uint32_t communities, uint32_t sim_run);
 
+/* The partial human census of writeCensus (writeoutput.cpp:100-148, the
+ * nAlive file): header "age,history1..S,simRun", then one line for each of
+ * the first min(5000, n_humans) humans.  age: each human's age in days
+ * after the last simulated day, (days done - 1) - h_birth; history: the
+ * S * n_humans words of arbo_read_history ([s * n_humans + i]).  sim_run 0
+ * creates the file, any other appends. */
+ARBO_API int arbo_write_nalive(const char* path, const uint32_t* age, const int32_t* history, uint32_t n_humans,
+                               uint32_t serotypes, uint32_t sim_run);
+
+/* The age-of-infection file of writeAOI (writeoutput.cpp:152-191, nAOI):
+ * header "first,second,third,fourth,simRun", then ARBO_AOI_RECORD lines of
+ * the four rings of arbo_read_aoi (aoi[k * ARBO_AOI_RECORD + r]).  sim_run 0
+ * creates the file, any other appends. */
+ARBO_API int arbo_write_naoi(const char* path, const uint32_t* aoi, uint32_t sim_run);
+
 #ifdef __cplusplus
 }
 #endif

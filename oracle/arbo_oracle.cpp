@@ -82,7 +82,17 @@ struct Sim {
     // series
     std::vector<uint32_t> n_count, m_count, n_inf, n_inc, n_one;
     std::vector<uint32_t> spatial;   // [(T+1)][S][C] infectious humans per community (ARBO_F_SPATIAL)
+    std::vector<int32_t> hist;       // [human][4] infection history (ARBO_F_HISTORY), else empty
+    // age-of-infection rings (ARBO_F_AOI): marks of today's infections by
+    // mosquito events, rings [4][AOI_RECORD] and their next positions
+    std::vector<uint8_t> aoi_mark;
+    std::vector<uint32_t> aoi;
+    uint32_t aoi_pos[4] = {0, 0, 0, 0};
+    uint32_t aoi_from = 0;
 };
+
+constexpr uint32_t AOI_RECORD = 5000;   // C_NAOIRECORD (constant.h:22)
+constexpr uint32_t AOI_WINDOW = 3650;   // 10 * C_YEAR (nepidemic.cu:229)
 
 }  // namespace
 
@@ -351,6 +361,8 @@ void init_community(Sim& s, uint32_t c) {
         s.hw[h] = (dd << 8) | (imm << 4) | (strain << 2) | st;
         s.hn[h] = nxt;
         s.hb[h] = -(int32_t)(a0 + 1);
+        if (!s.hist.empty())   // demographicintial.cu:158-171 (-2 immune, -1 naive), epidemicinitial.cu:59
+            for (uint32_t v = 0; v < 4; ++v) s.hist[(size_t)h * 4 + v] = ((imm >> v) & 1u) ? -2 : -1;
     }
     W4 rt = rnd(s.key, c, DAY_INIT, INIT_MT, 0);
     uint32_t alive = sround(s.dens[0] * s.base[c], rt.v[0]);
@@ -455,6 +467,8 @@ void phase_a(Sim& s, uint32_t c, uint32_t t) {
             uint32_t da = death_age(s.Kh, (r.v[0] >> 8) + 1);
             s.hw[h] = (t + da) << 8;
             s.hb[h] = (int32_t)t;
+            if (!s.hist.empty())   // ndemographic.cu:96-102: the newborn has no history
+                for (uint32_t v = 0; v < 4; ++v) s.hist[(size_t)h * 4 + v] = -1;
             continue;
         }
         if (st == 2) {
@@ -575,6 +589,7 @@ void infect_human(Sim& s, uint32_t h, uint32_t v, uint32_t t) {
     uint32_t iip = s.p.iip_min + range(r.v[0], s.p.iip_max - s.p.iip_min + 1);
     s.hw[h] = (s.hw[h] & ~0xFu) | (1u << (4 + v)) | (v << 2) | 1u;
     s.hn[h] = (uint16_t)(t + iip);
+    if (!s.hist.empty()) s.hist[(size_t)h * 4 + v] = (int32_t)t - s.hb[h];   // age at exposure (nepidemic.cu:248,298)
 }
 
 bool human_eligible(uint32_t w, uint32_t v) { return (w & 3u) == 0 && !((w >> (4 + v)) & 1u); }
@@ -617,7 +632,10 @@ void phase_b(Sim& s, uint32_t d, uint32_t t) {
             for (const Event& E : ev[pos]) {
                 W4 r = rnd(s.key, E.x0, t, E.x2, E.k >> 2);
                 uint32_t h = (uint32_t)(s.h_off[d] + range(r.v[E.k & 3], Nd));
-                if (human_eligible(s.hw[h], v)) infect_human(s, h, v, t);
+                if (human_eligible(s.hw[h], v)) {
+                    infect_human(s, h, v, t);
+                    if (!s.aoi_mark.empty() && t >= s.aoi_from) s.aoi_mark[h] = 1;   // nepidemic.cu:229-243
+                }
             }
             if (p.external_infection > 0.0f) {  // nepidemic.cu:261-309
                 uint32_t x = sround(p.external_infection / (float)s.C,
@@ -651,6 +669,23 @@ void long_distance(Sim& s, uint32_t t) {
         }
 }
 
+// The day's infections by mosquito events enter the age-of-infection rings
+// (nepidemic.cu:229-243) in human order: ring k = the serotypes the human had
+// been infected with before (its history words other than today's, which is
+// the only one set today), slot = the ring's next position, cycling through
+// AOI_RECORD slots like atomicInc(&count, C_NAOIRECORD - 1).
+void aoi_day(Sim& s, uint32_t t) {
+    for (uint64_t h = 0; h < s.NH; ++h) {
+        if (!s.aoi_mark[h]) continue;
+        s.aoi_mark[h] = 0;
+        uint32_t n = 0;
+        for (uint32_t v = 0; v < 4; ++v) n += s.hist[h * 4 + v] != -1;
+        const uint32_t k = n - 1;
+        s.aoi[(size_t)k * AOI_RECORD + s.aoi_pos[k]] = (uint32_t)((int32_t)t - s.hb[h]);
+        s.aoi_pos[k] = (s.aoi_pos[k] + 1) % AOI_RECORD;
+    }
+}
+
 void sim_day(Sim& s, uint32_t t, int threads) {
     const int C = (int)s.C;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads) if (threads > 1)
@@ -662,6 +697,7 @@ void sim_day(Sim& s, uint32_t t, int threads) {
     long_distance(s, t);
 #pragma omp parallel for schedule(dynamic, 16) num_threads(threads) if (threads > 1)
     for (int d = 0; d < C; ++d) phase_b(s, (uint32_t)d, t);
+    if (!s.aoi_mark.empty()) aoi_day(s, t);
 }
 
 int thread_count(const ArboEngine* e) { return e->threads > 0 ? e->threads : omp_get_max_threads(); }
@@ -735,6 +771,15 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes,
     for (int i = 0; i < (int)n_sims; ++i) init_sim(e->sims[i], max_days);
     if (e->flags & ARBO_F_SPATIAL)
         for (Sim& s : e->sims) s.spatial.assign(((size_t)max_days + 1) * s.S * s.C, 0u);
+    if (e->flags & (ARBO_F_HISTORY | ARBO_F_AOI))
+        for (Sim& s : e->sims) s.hist.assign((size_t)s.NH * 4, -1);
+    if (e->flags & ARBO_F_AOI)
+        for (Sim& s : e->sims) {
+            s.aoi_mark.assign(s.NH, 0);
+            s.aoi.assign((size_t)4 * AOI_RECORD, 0u);
+            std::fill(s.aoi_pos, s.aoi_pos + 4, 0u);
+            s.aoi_from = max_days > AOI_WINDOW ? max_days - AOI_WINDOW : 0u;
+        }
     if ((int)n_sims >= 2 * nt) {
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
         for (int i = 0; i < (int)n_sims; ++i) {
@@ -805,6 +850,25 @@ int arbo_read_spatial(ArboEngine* e, uint32_t sim, uint32_t* out) {
     if (!(e->flags & ARBO_F_SPATIAL)) return fail(e, ARBO_E_STATE, "engine created without ARBO_F_SPATIAL");
     const Sim& s = e->sims[sim];
     std::memcpy(out, s.spatial.data(), ((size_t)e->day + 1) * s.S * s.C * 4);
+    return ARBO_OK;
+}
+
+int arbo_read_history(ArboEngine* e, uint32_t sim, int32_t* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    if (!e->ready || sim >= e->sims.size()) return fail(e, ARBO_E_INVALID, "bad simulation index");
+    if (!(e->flags & (ARBO_F_HISTORY | ARBO_F_AOI)))
+        return fail(e, ARBO_E_STATE, "engine created without ARBO_F_HISTORY");
+    const Sim& s = e->sims[sim];
+    for (uint64_t h = 0; h < s.NH; ++h)
+        for (uint32_t v = 0; v < s.S; ++v) out[v * s.NH + h] = s.hist[h * 4 + v];
+    return ARBO_OK;
+}
+
+int arbo_read_aoi(ArboEngine* e, uint32_t sim, uint32_t* out) {
+    if (!e || !out) return ARBO_E_INVALID;
+    if (!e->ready || sim >= e->sims.size()) return fail(e, ARBO_E_INVALID, "bad simulation index");
+    if (!(e->flags & ARBO_F_AOI)) return fail(e, ARBO_E_STATE, "engine created without ARBO_F_AOI");
+    std::memcpy(out, e->sims[sim].aoi.data(), (size_t)4 * AOI_RECORD * 4);
     return ARBO_OK;
 }
 

@@ -148,3 +148,46 @@ def test_spatial_series_matches_oracle(name, hip_lib, oracle_lib):
         assert np.array_equal(a.sum(axis=2), gpu.series(i)["n_infected"])
     gpu.close()
     orc.close()
+
+
+@pytest.mark.parametrize("name", ["lattice10", "reference_params", "batch_pair"])
+def test_history_matches_oracle(name, hip_lib, oracle_lib):
+    """The infection history (ARBO_F_HISTORY), bit for bit, and the census
+    ages beside it."""
+    from arbo_gpu_fit_amd import abi
+    print(f"[parity] history case={name}", flush=True)
+    specs, T = CASES[name]()
+    gpu = Engine(hip_lib, flags=abi.F_HISTORY)
+    orc = Engine(oracle_lib, flags=abi.F_HISTORY, threads=8)
+    for e in (gpu, orc):
+        e.init(specs, T)
+        e.step(T)
+    for i in range(len(specs)):
+        (ag, hg), (ao, ho) = gpu.census(i), orc.census(i)
+        assert np.array_equal(hg, ho), f"sim {i}"
+        assert np.array_equal(ag, ao), f"sim {i}"
+        assert (hg >= 0).any()
+    gpu.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("name", ["lattice10", "reference_params", "batch_pair"])
+def test_aoi_rings_match_oracle(name, hip_lib, oracle_lib):
+    """The age-of-infection rings (ARBO_F_AOI), bit for bit."""
+    from arbo_gpu_fit_amd import abi
+    print(f"[parity] aoi case={name}", flush=True)
+    specs, T = CASES[name]()
+    gpu = Engine(hip_lib, flags=abi.F_AOI)
+    orc = Engine(oracle_lib, flags=abi.F_AOI, threads=8)
+    for e in (gpu, orc):
+        e.init(specs, T)
+        e.step(T)
+    recorded = 0
+    for i in range(len(specs)):
+        a, b = gpu.aoi(i), orc.aoi(i)
+        assert np.array_equal(a, b), f"sim {i}: {np.flatnonzero(a != b)[:10]}"
+        recorded += int((a != 0).sum())
+        assert np.array_equal(gpu.history(i), orc.history(i))
+    assert recorded > 0
+    gpu.close()
+    orc.close()

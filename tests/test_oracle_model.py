@@ -326,3 +326,124 @@ def test_spatial_series_is_the_per_community_count(oracle_lib):
     with pytest.raises(ArboError):
         plain.spatial(0)
     plain.close()
+
+
+def test_history_records_each_infection_day(oracle_lib):
+    """The infection history (d_nHistory) follows from the census words day
+    by day: -2 for immunity of the t = 0 census (demographicintial.cu:158-171,
+    epidemicinitial.cu:59), -1 for a newborn (ndemographic.cu:96-102), and on
+    the day t a serotype's immune bit turns on, the age t - birth day
+    (nepidemic.cu:248,298); otherwise it keeps its value.  Exposed humans
+    also carry an incubation end inside [iip_min, iip_max] days after that
+    infection day."""
+    p = configs.base_params(seed_frac_h=0.02, seed_frac_m=0.02, beta_hm=0.6, beta_mh=0.6, sero_prev=0.2)
+    spec = SimSpec(4, 5, np.full(20, 300, np.uint32), p, 5)
+    S = p.n_serotypes
+    eng = Engine(oracle_lib, flags=abi.F_HISTORY, threads=2)
+    eng.init([spec], 60)
+    prev = eng.state(0)
+    hist = eng.history(0)
+    imm0 = (prev["h_word"][None, :] >> (4 + np.arange(S)[:, None])) & 1
+    assert np.array_equal(hist, np.where(imm0 == 1, -2, -1))
+    assert (hist == -2).any() and (hist == -1).any()
+    new_total = reborn_total = 0
+    for t in range(60):
+        eng.step(1)
+        cur, h = eng.state(0), eng.history(0)
+        reborn = cur["h_birth"] != prev["h_birth"]
+        assert np.all(cur["h_birth"][reborn] == t)
+        imm_p = (prev["h_word"][None, :] >> (4 + np.arange(S)[:, None])) & 1
+        imm_c = (cur["h_word"][None, :] >> (4 + np.arange(S)[:, None])) & 1
+        new = (imm_c == 1) & ((imm_p == 0) | reborn[None, :])
+        age = t - cur["h_birth"].astype(np.int64)
+        expect = np.where(new, age[None, :], np.where(reborn[None, :], -1, hist))
+        expect = np.where(imm_c == 1, expect, -1)
+        assert np.array_equal(h, expect), f"day {t}"
+        st, strain = cur["h_word"] & 3, (cur["h_word"] >> 2) & 3
+        exp = np.flatnonzero(st == 1)
+        hv = h[strain[exp], exp]
+        assert np.all(hv >= 0)
+        wait = i16(cur["h_next"][exp] - (cur["h_birth"][exp] + hv))
+        assert np.all((wait >= p.iip_min) & (wait <= p.iip_max))
+        new_total += int(new.sum())
+        reborn_total += int(reborn.sum())
+        prev, hist = cur, h
+    assert new_total > 100 and reborn_total > 0
+    age, hc = eng.census(0)
+    assert np.array_equal(hc, hist)
+    assert np.array_equal(age.astype(np.int64), 59 - prev["h_birth"].astype(np.int64))
+    done = hc >= 0
+    assert np.all(hc[done] <= np.broadcast_to(age, hc.shape)[done])
+    eng.close()
+    plain = Engine(oracle_lib, threads=2)
+    plain.init([spec], 5)
+    with pytest.raises(ArboError):
+        plain.history(0)
+    plain.close()
+
+
+def expected_aoi(eng, T, first_day=0):
+    """Age-of-infection rings restated from the census words day by day
+    (nepidemic.cu:229-243): each infection of day t >= first_day, in human
+    order, goes to ring k = the serotypes already in the human's history, at
+    the ring's next slot mod 5000 (atomicInc(&count, C_NAOIRECORD - 1))."""
+    ring = np.zeros((4, 5000), np.int64)
+    pos = [0, 0, 0, 0]
+    count = [0, 0, 0, 0]
+    prev = eng.state(0)
+    S = eng.specs[0].params.n_serotypes
+    for t in range(T):
+        eng.step(1)
+        cur = eng.state(0)
+        reborn = cur["h_birth"] != prev["h_birth"]
+        imm_p = (prev["h_word"][None, :] >> (4 + np.arange(S)[:, None])) & 1
+        imm_c = (cur["h_word"][None, :] >> (4 + np.arange(S)[:, None])) & 1
+        imm_p = np.where(reborn[None, :], 0, imm_p)
+        new = (imm_c == 1) & (imm_p == 0)
+        assert new.sum(axis=0).max(initial=0) <= 1   # one infection per human and day
+        if t >= first_day:
+            for i in np.flatnonzero(new.any(axis=0)):
+                k = int(imm_p[:, i].sum())
+                ring[k, pos[k]] = t - int(cur["h_birth"][i])
+                pos[k] = (pos[k] + 1) % 5000
+                count[k] += 1
+        prev = cur
+    return ring, count
+
+
+def test_aoi_rings_follow_the_infections(oracle_lib):
+    """Rings cycle: more than 5000 first infections overwrite the oldest."""
+    p = configs.base_params(seed_frac_h=0.03, seed_frac_m=0.03, beta_hm=0.7, beta_mh=0.7, sero_prev=0.3)
+    spec = SimSpec(4, 5, np.full(20, 700, np.uint32), p, 8)
+    eng = Engine(oracle_lib, flags=abi.F_AOI, threads=2)
+    eng.init([spec], 70)
+    ring, count = expected_aoi(eng, 70)
+    assert max(count) > 5000 and min(count) > 0, count
+    assert np.array_equal(eng.aoi(0), ring)
+    assert eng.history(0).shape == (4, 14000)   # the rings keep the history too
+    eng.close()
+
+
+def test_aoi_rings_record_the_last_ten_years_only(oracle_lib):
+    """max_days - t <= 3650 (nepidemic.cu:229): an engine sized for 3,700
+    days records from day 50 on."""
+    p = configs.base_params(seed_frac_h=0.02, seed_frac_m=0.02, beta_hm=0.6, beta_mh=0.6, sero_prev=0.2)
+    spec = SimSpec(3, 3, np.full(9, 500, np.uint32), p, 4)
+    eng = Engine(oracle_lib, flags=abi.F_AOI, threads=2)
+    eng.init([spec], 3700)
+    ring, count = expected_aoi(eng, 60, first_day=50)
+    assert 0 < sum(count) < 5000
+    assert np.array_equal(eng.aoi(0), ring)
+    eng.close()
+
+
+def test_aoi_rings_skip_introductions(oracle_lib):
+    """Introductions (nepidemic.cu:261-309) infect without an AOI record."""
+    p = configs.base_params(seed_frac_h=0.0, seed_frac_m=0.0, beta_hm=0.0, beta_mh=0.0, external_infection=40.0)
+    spec = SimSpec(3, 3, np.full(9, 500, np.uint32), p, 6)
+    eng = Engine(oracle_lib, flags=abi.F_AOI, threads=2)
+    eng.init([spec], 30)
+    eng.step(30)
+    assert (eng.history(0) >= 0).sum() > 100
+    assert not eng.aoi(0).any()
+    eng.close()

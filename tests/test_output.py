@@ -1,6 +1,8 @@
-"""The nData file (CPU): arbo_write_ndata against a restatement of the
-reference's writeIncidence (writeoutput.cpp:13-98), text for text, for a
-run that creates the file and a run that appends to it."""
+"""The output files (CPU): arbo_write_ndata, arbo_write_nspatial and
+arbo_write_nalive against restatements of the reference's writeIncidence
+(writeoutput.cpp:13-98), writeSpatial (writeoutput.cpp:194-241) and
+writeCensus (writeoutput.cpp:100-148), text for text, for a run that creates
+the file and a run that appends to it."""
 import numpy as np
 
 from arbo_gpu_fit_amd.output import write_ndata
@@ -56,3 +58,48 @@ def test_nspatial_matches_reference_layout(tmp_path):
     for r, sp in enumerate(runs):
         write_nspatial(path, sp, r)
     assert open(path).read() == reference_spatial_text(runs)
+
+
+def reference_alive_text(runs):
+    """writeCensus (writeoutput.cpp:100-148) restated: the first
+    min(C_NSIZERECORD = 5000, n) humans, each history column read at
+    [s * n + i]."""
+    S = runs[0][1].shape[0]
+    out = "age" + "".join(f",history{v + 1}" for v in range(S)) + ",simRun"
+    for r, (age, hist) in enumerate(runs):
+        n = age.size
+        for i in range(min(5000, n)):
+            out += f"\n{age[i]}" + "".join(f",{hist[v, i]}" for v in range(S)) + f",{r}"
+    return out
+
+
+def test_nalive_matches_reference_layout(tmp_path):
+    from arbo_gpu_fit_amd.output import write_nalive
+    rng = np.random.default_rng(3)
+    runs = []
+    for n in (7000, 40):   # more humans than a census records, then fewer
+        age = rng.integers(0, 36500, n)
+        hist = rng.integers(-2, 30000, (4, n))
+        runs.append((age, hist))
+    path = str(tmp_path / "nAlive_0.csv")
+    for r, (age, hist) in enumerate(runs):
+        write_nalive(path, age, hist, r)
+    text = open(path).read()
+    assert text == reference_alive_text(runs)
+    assert text.count("\n") == 5000 + 40
+
+
+def test_naoi_matches_reference_layout(tmp_path):
+    """writeAOI (writeoutput.cpp:152-191) restated: 5000 lines of the four
+    rings side by side."""
+    from arbo_gpu_fit_amd.output import write_naoi
+    rng = np.random.default_rng(4)
+    runs = [rng.integers(0, 30000, (4, 5000)), rng.integers(0, 30000, (4, 5000))]
+    path = str(tmp_path / "nAOI_0.csv")
+    for r, a in enumerate(runs):
+        write_naoi(path, a, r)
+    want = "first,second,third,fourth,simRun"
+    for r, a in enumerate(runs):
+        for i in range(5000):
+            want += f"\n{a[0, i]},{a[1, i]},{a[2, i]},{a[3, i]},{r}"
+    assert open(path).read() == want
