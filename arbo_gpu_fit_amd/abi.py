@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_SEROTYPES = 4
 MAX_COMMUNITY_HUMANS = 8192
 MAX_COMMUNITY_SLOTS = 8192
@@ -59,8 +59,16 @@ class Params(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Climate(C.Structure):
+    _fields_ = [("temperature", C.POINTER(C.c_float)), ("humidity", C.POINTER(C.c_float)),
+                ("precipitation", C.POINTER(C.c_float)), ("days", _u32),
+                ("biting_rate", _f32), ("scale_eip", _f32), ("scale_mle", _f32), ("scale_cc", _f32),
+                ("scale_humidity", _f32), ("min_mos_to_human", _f32), ("mos_net_growth", _f32)]
+
+
 class Shape(C.Structure):
-    _fields_ = [("grid_w", _u32), ("grid_h", _u32), ("humans", _u32p), ("mosq_slots", _u32p)]
+    _fields_ = [("grid_w", _u32), ("grid_h", _u32), ("humans", _u32p), ("mosq_slots", _u32p),
+                ("climate", C.POINTER(Climate))]
 
 
 class Series(C.Structure):
@@ -98,6 +106,7 @@ PROTOTYPES = {
     "arbo_read_spatial": (C.c_int, [EnginePtr, _u32, _u32p]),
     "arbo_read_history": (C.c_int, [EnginePtr, _u32, C.POINTER(C.c_int32)]),
     "arbo_read_aoi": (C.c_int, [EnginePtr, _u32, _u32p]),
+    "arbo_climate_day": (C.c_int, [C.POINTER(Climate), _u32, C.POINTER(C.c_double)]),
     "arbo_kernel_time": (C.c_int, [EnginePtr, _u32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "arbo_reset_timing": (C.c_int, [EnginePtr]),
     "arbo_tuning_candidates": (C.c_int, [C.POINTER(Tuning), _u32, _u32p]),

@@ -28,7 +28,7 @@ import math
 import numpy as np
 
 from . import abi
-from .engine import SimSpec
+from .engine import ClimateSpec, SimSpec
 
 # --- Philox4x32-10 over numpy uint64 lanes ---------------------------------
 
@@ -137,6 +137,21 @@ def reference_params(**over) -> abi.Params:
     for k, v in over.items():
         _set(p, k, v)
     return p
+
+
+def seasonal_climate(days: int, t_mean: float = 26.0, t_amp: float = 4.0, h_mean: float = 0.7,
+                     h_amp: float = 0.1, p_mean: float = 0.4, p_amp: float = 0.4, phase: float = 0.0,
+                     **coeffs) -> ClimateSpec:
+    """A smooth synthetic climate, one row per day (the reference reads a
+    smoothed daily file, climate_smooth.csv, that it does not ship): annual
+    sinusoids of temperature (deg C), relative humidity and precipitation,
+    humidity and rain in phase with the temperature.  The coefficients
+    default to the means of the reference's MCMC priors (MCMC.h:37-46) and
+    SimulationParam defaults (bitingRate 0.25, mosNetGrowth 0.5)."""
+    d = np.arange(days, dtype=np.float64)
+    s = np.sin(2.0 * np.pi * d / 365.0 + phase)
+    return ClimateSpec((t_mean + t_amp * s).astype(np.float32), (h_mean + h_amp * s).astype(np.float32),
+                       (p_mean + p_amp * s).astype(np.float32), **coeffs)
 
 
 def lattice10(seed: int = 1) -> tuple[list[SimSpec], int]:

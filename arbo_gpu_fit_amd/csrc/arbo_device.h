@@ -63,7 +63,23 @@ struct SimDesc {
     uint64_t series_off;          // this simulation's block in the series array
     uint64_t spatial_off;         // its block [(T+1)][S][C] of the per-community series (if kept)
     uint64_t h_begin, h_end;      // its humans: [h_begin, h_end) of the padded census, h_end a multiple of 4
+    uint32_t climate;             // 1: per-day beta/eip at clim_off and per-birth-day lifetime rows
+    uint32_t clim_off;            // [max_days][3] beta_hm, beta_mh, eip in the dens pool
 };
+
+// Parameters of day t: the climate's or the constant ones.
+struct DayParams {
+    float beta_hm, beta_mh, eip;
+};
+
+__host__ __device__ inline DayParams day_params(const SimDesc& sd, const float* dens, uint32_t t) {
+    if (!sd.climate) return DayParams{sd.beta_hm, sd.beta_mh, sd.eip};
+    const float* d = dens + sd.clim_off + 3 * (size_t)t;
+    return DayParams{d[0], d[1], d[2]};
+}
+
+// Lifetime row of mosquitoes born on day t (row 0: the t = 0 census).
+__host__ __device__ inline uint32_t birth_row(const SimDesc& sd, uint32_t t) { return sd.climate ? t + 1 : 0u; }
 
 // Series block of one simulation with room for T days (T+1 rows):
 //   n_count[T+1] m_count[T+1] n_inf[(T+1)S] n_inc[(T+1)S] n_one[(T+1)S]

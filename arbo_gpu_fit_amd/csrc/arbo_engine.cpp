@@ -260,7 +260,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     if (n_sims == 0 || !shapes || !params || !seeds) return fail(e, ARBO_E_INVALID, "empty batch");
     if (max_days == 0 || max_days > (1u << 23)) return fail(e, ARBO_E_INVALID, "max_days must be 1..2^23");
     for (uint32_t i = 0; i < n_sims; ++i) {
-        std::string why = validate(shapes[i], params[i]);
+        std::string why = validate(shapes[i], params[i], max_days);
         if (!why.empty()) return fail(e, ARBO_E_INVALID, "simulation " + std::to_string(i) + ": " + why);
     }
     HIP_TRY(e, hipSetDevice(e->device));
@@ -331,6 +331,8 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         sd.init_m_off = pool_add(table_pool, seen_u32, t.init_m);
         sd.immune_off = pool_add(table_pool, seen_u32, t.immune);
         sd.dens_off = pool_add(dens_pool, seen_f32, t.dens);
+        sd.climate = t.climate ? 1u : 0u;
+        if (t.climate) sd.clim_off = pool_add(dens_pool, seen_f32, t.clim);
         sd.seed_h = t.seed_h;
         sd.seed_m = t.seed_m;
         sd.death_m_max = t.death_m_max;
@@ -679,6 +681,12 @@ int arbo_read_history(ArboEngine* e, uint32_t sim, int32_t* out) {
     for (uint32_t c = 0; c < s.C; ++c)
         for (uint32_t i = 0; i < s.humans[c]; ++i, ++k)
             for (uint32_t v = 0; v < s.S; ++v) out[v * N + k] = blk[4 * (s.h_off[c] - lo + i) + v];
+    return ARBO_OK;
+}
+
+int arbo_climate_day(const ArboClimate* c, uint32_t row, double* out) {
+    if (!c || !out || !c->temperature || !c->humidity || !c->precipitation || row >= c->days) return ARBO_E_INVALID;
+    climate_day(*c, row, out);
     return ARBO_OK;
 }
 

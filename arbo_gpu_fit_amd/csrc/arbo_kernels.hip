@@ -401,9 +401,10 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     // and the inbox rows of the community's CSR entries, the next day's
     // seasonal density.
     {
-        const uint32_t* dt = v.tables + sd.death_m_off;
-        const uint32_t* gt = v.tables + sd.death_m_guide_off;
         const uint32_t nd = sd.death_m_max + 1;
+        const uint32_t row = birth_row(sd, t);   // today's births
+        const uint32_t* dt = v.tables + sd.death_m_off + (size_t)row * nd;
+        const uint32_t* gt = v.tables + sd.death_m_guide_off + (size_t)row * kGuide;
         constexpr uint32_t kT = 4, kG = (kLifeBuckets + kWave - 1) / kWave;
         uint32_t tv[kT], gv[kG];
 #pragma unroll
@@ -635,8 +636,9 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
         if (s < S) {
             const uint32_t I = dir == DIR_HM ? unpack_pair(ih01, ih23, s) : unpack_pair(im01, im23, s);
             if (I > 0) {
-                const float lam = dir == DIR_HM ? (float)Mc / (float)cd.n * sd.beta_hm * (float)I
-                                                : sd.beta_mh * (float)I;
+                const DayParams dp = day_params(sd, v.dens, t);
+                const float lam = dir == DIR_HM ? (float)Mc / (float)cd.n * dp.beta_hm * (float)I
+                                                : dp.beta_mh * (float)I;
                 const bool staying = (cd.flags & kCommStay) != 0;
                 uint32_t stay = 0;
                 if (sd.count_law == ARBO_COUNT_STOCH_ROUND) {   // nepidemic.cu:67-104
@@ -968,7 +970,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                     const uint64_t gm = cd.m_off + k;
                     v.m_word[gm] = (uint16_t)((w & ~0xFu) | (s << 2) | 1u);
                     const U4 r2 = draw(key, cd.m_base + k, t, P_M_EIP, 0);
-                    v.m_next[gm] = (uint16_t)(t + stoch_round(sd.eip, r2.x));
+                    v.m_next[gm] = (uint16_t)(t + stoch_round(day_params(sd, v.dens, t).eip, r2.x));
                 }
             }
             __syncthreads();

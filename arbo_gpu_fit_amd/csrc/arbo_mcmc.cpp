@@ -99,6 +99,10 @@ struct ArboMcmc {
     uint32_t grid_w = 0, grid_h = 0;
     std::vector<uint32_t> humans, slots;
     bool has_slots = false;
+    // the climate every proposal's simulations run under (copied), if any
+    bool has_climate = false;
+    std::vector<float> clim_t, clim_h, clim_p;
+    ArboClimate climate{};
     ArboParams base{};
     std::vector<uint32_t> data;
     Key key{0, 0};
@@ -238,7 +242,8 @@ int eval_engine(ArboMcmc* m, ArboEngine* eng, const ArboEngineApi& api, uint32_t
     for (uint32_t i = 0; i < n; ++i)
         for (uint32_t p = 0; p < P; ++p) {
             const uint32_t k = i * P + p;
-            shapes[k] = ArboShape{m->grid_w, m->grid_h, m->humans.data(), m->has_slots ? m->slots.data() : nullptr};
+            shapes[k] = ArboShape{m->grid_w, m->grid_h, m->humans.data(), m->has_slots ? m->slots.data() : nullptr,
+                                  m->has_climate ? &m->climate : nullptr};
             arbo_mcmc_sim_params(th + (size_t)i * D, &m->base, &params[k]);
             seeds[k] = sim_seed(m, chains[i], it, p);
         }
@@ -379,6 +384,21 @@ ARBO_API int arbo_mcmc_create(const ArboMcmcDesc* desc, const ArboShape* shape, 
     m->grid_h = shape->grid_h;
     const size_t C_comm = (size_t)shape->grid_w * shape->grid_h;
     m->humans.assign(shape->humans, shape->humans + C_comm);
+    if (shape->climate) {
+        const ArboClimate& c = *shape->climate;
+        if (!c.temperature || !c.humidity || !c.precipitation) {
+            delete m;
+            return ARBO_E_INVALID;
+        }
+        m->clim_t.assign(c.temperature, c.temperature + c.days);
+        m->clim_h.assign(c.humidity, c.humidity + c.days);
+        m->clim_p.assign(c.precipitation, c.precipitation + c.days);
+        m->climate = c;
+        m->climate.temperature = m->clim_t.data();
+        m->climate.humidity = m->clim_h.data();
+        m->climate.precipitation = m->clim_p.data();
+        m->has_climate = true;
+    }
     if (shape->mosq_slots) {
         m->slots.assign(shape->mosq_slots, shape->mosq_slots + C_comm);
         m->has_slots = true;

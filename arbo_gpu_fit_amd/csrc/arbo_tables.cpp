@@ -36,8 +36,64 @@ static double human_cdf(const ArboParams& p, double a) {
 }
 
 // mdemographic.cu:13-17: F(a) = 1 - exp(-(a/scale)^shape).
-static double mosq_cdf(const ArboParams& p, double a) {
-    return 1.0 - dexp(-powd(a / (double)p.m_scale, (double)p.m_shape));
+static double weibull_cdf(double a, double scale, double shape) { return 1.0 - dexp(-powd(a / scale, shape)); }
+
+static double mosq_cdf(const ArboParams& p, double a) { return weibull_cdf(a, (double)p.m_scale, (double)p.m_shape); }
+
+// ---- climate (SimulationParam.cpp:92-137, simulation.cpp:87,129-131) ----
+
+// c T (T - lo) sqrt(hi - T) inside [lo, hi], else 0 (SimulationParam.cpp:114-136).
+static double briere(double T, double c, double lo, double hi) {
+    if (T < lo || T > hi) return 0.0;
+    return c * T * (T - lo) * std::sqrt(hi - T);
+}
+
+void climate_day(const ArboClimate& c, uint32_t row, double out[4]) {
+    const double T = (double)c.temperature[row], H = (double)c.humidity[row];
+    const double TK = T + 273.15;
+    const double rate = 24.0 * 0.003359 * (TK / 298.0) * dexp((15000.0 / 1.986) * ((1.0 / 298.0) - (1.0 / TK)));
+    const double mu = 0.8692 - 0.159 * T + 0.01116 * (T * T) - 0.0003408 * (T * T * T) + 0.000003809 * (T * T * T * T);
+    out[0] = (double)c.biting_rate * briere(T, 0.000491, 12.22, 37.46);   // nmBitingSuccess
+    out[1] = (double)c.biting_rate * briere(T, 0.000849, 17.05, 35.83);   // mnBitingSuccess
+    out[2] = 1.0 / (rate * (double)c.scale_eip);
+    out[3] = 1.0 / (mu * (double)c.scale_mle * powd(1.0 - H, (double)c.scale_humidity));
+}
+
+// Carrying capacity of climate row `row` (simulation.cpp:87,131):
+// minMosToHuman N (1 + P)^scaleCC.
+static double climate_capacity(const ArboClimate& c, uint32_t row, double N) {
+    return (double)c.min_mos_to_human * N * powd(1.0 + (double)c.precipitation[row], (double)c.scale_cc);
+}
+
+// The expected mosquitoes of the whole simulation, M(0..max_days), kept in
+// float like h_mExpectedPopSize (simulation.cpp:84-132).
+static std::vector<float> climate_target(const ArboClimate& c, double N, uint32_t max_days) {
+    std::vector<float> M((size_t)max_days + 1);
+    M[0] = (float)climate_capacity(c, 0, N);
+    for (uint32_t t = 0; t < max_days; ++t) {
+        const double m = (double)M[t];
+        M[t + 1] = (float)(m + (double)c.mos_net_growth * m * (1.0 - m / climate_capacity(c, t + 1, N)));
+    }
+    return M;
+}
+
+static std::string validate_climate(const ArboClimate& c, uint32_t max_days) {
+    if (!c.temperature || !c.humidity || !c.precipitation) return "climate arrays missing";
+    if (c.days < max_days + 1) return "climate shorter than max_days + 1 days";
+    if (!(c.biting_rate >= 0.0f) || !(c.scale_eip > 0.0f) || !(c.scale_mle > 0.0f) || !(c.min_mos_to_human > 0.0f) ||
+        !(c.mos_net_growth >= 0.0f) || !std::isfinite(c.scale_cc) || !std::isfinite(c.scale_humidity))
+        return "bad climate coefficients";
+    for (uint32_t r = 0; r <= max_days; ++r) {
+        if (!std::isfinite(c.temperature[r]) || !(c.humidity[r] >= 0.0f && c.humidity[r] < 1.0f) ||
+            !(c.precipitation[r] > -1.0f) || !std::isfinite(c.precipitation[r]))
+            return "climate row " + std::to_string(r) + " out of range";
+        double d[4];
+        climate_day(c, r, d);
+        if (!(d[2] > 0.0 && d[2] < 1000.0)) return "climate row " + std::to_string(r) + " gives eip outside (0, 1000) days";
+        if (!(d[3] > 0.0 && d[3] < 1.0e4))
+            return "climate row " + std::to_string(r) + " gives a mosquito lifespan scale outside (0, 1e4) days";
+    }
+    return std::string();
 }
 
 template <class Cdf>
@@ -66,7 +122,7 @@ static void initial_age_table(uint32_t n, uint32_t step, Cdf cdf, std::vector<ui
     for (uint32_t i = 1; i < n; ++i) T[i] = std::max(T[i], T[i - 1]);
 }
 
-std::string validate(const ArboShape& sh, const ArboParams& p) {
+std::string validate(const ArboShape& sh, const ArboParams& p, uint32_t max_days) {
     if (p.n_serotypes < 1 || p.n_serotypes > ARBO_MAX_SEROTYPES) return "n_serotypes must be 1..4";
     if (sh.grid_w < 1 || sh.grid_h < 1 || !sh.humans) return "empty lattice";
     if ((uint64_t)sh.grid_w * sh.grid_h > (1u << 24)) return "too many communities";
@@ -89,8 +145,23 @@ std::string validate(const ArboShape& sh, const ArboParams& p) {
     for (uint64_t c = 0; c < C; ++c) {
         const uint32_t n = sh.humans[c];
         if (n < 1 || n > ARBO_MAX_COMMUNITY_HUMANS) return "humans per community out of range";
+        nh += n;
+    }
+    double clim_peak = 0.0;   // largest M(t) / N of a climate
+    if (sh.climate) {
+        const std::string why = validate_climate(*sh.climate, max_days);
+        if (!why.empty()) return why;
+        for (float m : climate_target(*sh.climate, (double)nh, max_days)) {
+            if (!std::isfinite(m) || !(m >= 0.0f)) return "climate gives an invalid mosquito target";
+            clim_peak = std::max(clim_peak, (double)m / (double)nh);
+        }
+    }
+    nh = 0;
+    for (uint64_t c = 0; c < C; ++c) {
+        const uint32_t n = sh.humans[c];
         const double cap = sh.mosq_slots ? (double)sh.mosq_slots[c]
-                                         : ceil((double)p.m_ratio * (1.0 + (double)p.season_amp) * (double)n);
+                         : sh.climate ? ceil(clim_peak * (double)n) + 1.0
+                                      : ceil((double)p.m_ratio * (1.0 + (double)p.season_amp) * (double)n);
         if (cap > (double)ARBO_MAX_COMMUNITY_SLOTS) return "mosquito slots per community out of range";
         nh += n;
         nm += (uint64_t)cap;
@@ -179,6 +250,61 @@ static void build_network(const ArboParams& p, SimTables& t) {
     }
 }
 
+// Mosquito lifetime rows and their guides.  Row 0 is the tabulated F of
+// m_scale.  With a climate, a mosquito born on day b dies on the first day
+// b + a whose F(a; scale(b + a)) reaches its PLE (mdemographic.cu:72 with the
+// day's mScaleLifeExpectancy); the running maximum over a of those
+// thresholds turns that first passage into a search of a monotone row.
+// Days past the run use the last day's scale.  Every row ends at its first
+// 2^24 entry, and all rows share the largest such length.
+static void build_mosquito_rows(const ArboShape& sh, const ArboParams& p, uint32_t max_days, SimTables& t) {
+    std::vector<std::vector<uint32_t>> rows;
+    rows.emplace_back();
+    threshold_table(kMosqAgeTable, [&](double a) { return mosq_cdf(p, a); }, rows[0]);
+    if (t.climate) {
+        std::vector<double> scale(max_days);
+        for (uint32_t d = 0; d < max_days; ++d) {
+            double x[4];
+            climate_day(*sh.climate, d + 1, x);
+            scale[d] = x[3];
+        }
+        for (uint32_t b = 0; b < max_days; ++b) {
+            std::vector<uint32_t> K(1, 0u);
+            uint32_t run = 0;
+            for (uint32_t a = 1; a < kMosqAgeTable && run < kOne24; ++a) {
+                const uint64_t day = std::min<uint64_t>((uint64_t)b + a, max_days - 1);
+                run = std::max(run, fixed24(weibull_cdf((double)a, scale[day], (double)p.m_shape)));
+                K.push_back(run);
+            }
+            rows.push_back(std::move(K));
+        }
+    }
+    // the largest death age any row returns: its first 2^24 entry, else the table end
+    uint32_t dmax = 1;
+    for (const auto& K : rows) {
+        uint32_t first = kMosqAgeTable - 1;
+        for (uint32_t a = 1; a < K.size(); ++a)
+            if (K[a] >= kOne24) { first = a; break; }
+        dmax = std::max(dmax, first);
+    }
+    t.death_m_max = dmax;
+    t.m_rows = (uint32_t)rows.size();
+    t.death_m.assign((size_t)t.m_rows * (dmax + 1), kOne24);
+    t.death_m_guide.assign((size_t)t.m_rows * kGuideBuckets, dmax);
+    for (uint32_t r = 0; r < t.m_rows; ++r) {
+        uint32_t* K = t.death_m.data() + (size_t)r * (dmax + 1);
+        for (uint32_t a = 0; a <= dmax && a < rows[r].size(); ++a) K[a] = rows[r][a];
+        K[dmax] = kOne24;
+        // the smallest death age of every life_bucket() of 24-bit draws (padding: the largest)
+        for (uint32_t b = 0; b < kLifeBuckets; ++b) {
+            const uint32_t k = life_bucket_kmin(b);
+            uint32_t a = 1;
+            while (a < dmax && K[a] < k) ++a;
+            t.death_m_guide[(size_t)r * kGuideBuckets + b] = a;
+        }
+    }
+}
+
 void build_tables(const ArboShape& sh, const ArboParams& p, uint64_t seed, uint32_t max_days,
                   SimTables& t) {
     t.W = sh.grid_w;
@@ -190,33 +316,43 @@ void build_tables(const ArboShape& sh, const ArboParams& p, uint64_t seed, uint3
     t.base.resize(t.C);
     t.n_humans = 0;
     t.n_slots = 0;
+    for (uint32_t c = 0; c < t.C; ++c) t.n_humans += t.humans[c];
+    t.climate = sh.climate != nullptr;
+    double clim_peak = 0.0;
+    if (t.climate) {   // the target M(t) / N, each community's share by its humans
+        const ArboClimate& cl = *sh.climate;
+        const std::vector<float> M = climate_target(cl, (double)t.n_humans, max_days);
+        t.dens.resize(M.size());
+        for (size_t d = 0; d < M.size(); ++d) {
+            t.dens[d] = (float)((double)M[d] / (double)t.n_humans);
+            clim_peak = std::max(clim_peak, (double)M[d] / (double)t.n_humans);
+        }
+        t.clim.resize((size_t)max_days * 3);
+        for (uint32_t d = 0; d < max_days; ++d) {   // day d reads row d + 1 (simulation.cpp:122)
+            double x[4];
+            climate_day(cl, d + 1, x);
+            t.clim[3 * (size_t)d] = (float)x[0];
+            t.clim[3 * (size_t)d + 1] = (float)x[1];
+            t.clim[3 * (size_t)d + 2] = (float)x[2];
+        }
+    }
     for (uint32_t c = 0; c < t.C; ++c) {
         t.slots[c] = sh.mosq_slots ? sh.mosq_slots[c]
-                                   : (uint32_t)ceil((double)p.m_ratio * (1.0 + (double)p.season_amp) * (double)t.humans[c]);
-        t.base[c] = p.m_ratio * (float)t.humans[c];
-        t.n_humans += t.humans[c];
+                     : t.climate ? (uint32_t)(ceil(clim_peak * (double)t.humans[c]) + 1.0)
+                                 : (uint32_t)ceil((double)p.m_ratio * (1.0 + (double)p.season_amp) * (double)t.humans[c]);
+        t.base[c] = t.climate ? (float)t.humans[c] : p.m_ratio * (float)t.humans[c];
         t.n_slots += t.slots[c];
     }
     threshold_table(kHumanAgeTable, [&](double a) { return human_cdf(p, a); }, t.death_h);
-    threshold_table(kMosqAgeTable, [&](double a) { return mosq_cdf(p, a); }, t.death_m);
-    // the largest death age the mosquito table returns, and the smallest
-    // death age of every life_bucket() of 24-bit draws (padding: the largest)
-    t.death_m_max = kMosqAgeTable - 1;
-    for (uint32_t a = 1; a < kMosqAgeTable; ++a)
-        if (t.death_m[a] >= kOne24) { t.death_m_max = a; break; }
-    t.death_m_guide.assign(kGuideBuckets, t.death_m_max);
-    for (uint32_t b = 0; b < kLifeBuckets; ++b) {
-        const uint32_t k = life_bucket_kmin(b);
-        uint32_t a = 1;
-        while (a < t.death_m_max && t.death_m[a] < k) ++a;
-        t.death_m_guide[b] = a;
-    }
+    build_mosquito_rows(sh, p, max_days, t);
     initial_age_table(kHumanYears, 365, [&](double a) { return human_cdf(p, a); }, t.init_h);
     initial_age_table(kMosqInitDays, 1, [&](double a) { return mosq_cdf(p, a); }, t.init_m);
-    t.dens.resize((size_t)max_days + 1);
-    for (uint32_t d = 0; d <= max_days; ++d) {
-        const double arg = 6.283185307179586 * (double)d / 365.0 + (double)p.season_phase;
-        t.dens[d] = (float)(1.0 + (double)p.season_amp * dsin(arg));
+    if (!t.climate) {
+        t.dens.resize((size_t)max_days + 1);
+        for (uint32_t d = 0; d <= max_days; ++d) {
+            const double arg = 6.283185307179586 * (double)d / 365.0 + (double)p.season_phase;
+            t.dens[d] = (float)(1.0 + (double)p.season_amp * dsin(arg));
+        }
     }
     const Key key{(uint32_t)seed, (uint32_t)(seed >> 32)};
     t.immune.assign((size_t)t.C * t.S, 0u);

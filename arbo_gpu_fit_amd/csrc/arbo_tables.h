@@ -25,12 +25,19 @@ struct SimTables {
     std::vector<uint32_t> humans, slots;   // per community
     std::vector<float> base;               // m_ratio * humans, f32
     std::vector<uint32_t> death_h;         // [kHumanAgeTable]  floor(F_h(a) 2^24)
-    std::vector<uint32_t> death_m;         // [kMosqAgeTable]   floor(F_m(a) 2^24)
-    uint32_t death_m_max = 0;              // first a >= 1 with death_m[a] = 2^24 (else table end)
-    std::vector<uint32_t> death_m_guide;   // [kGuideBuckets] death age of life_bucket_kmin(b)
+    // mosquito lifetimes, m_rows rows of death_m_max + 1 thresholds: row 0
+    // floor(F_m(a) 2^24) of m_scale (the t = 0 census; every birth without a
+    // climate); with a climate, row 1 + b the running maximum over a' <= a of
+    // floor(F_m(a'; scale(b + a')) 2^24) for births on day b
+    std::vector<uint32_t> death_m;
+    uint32_t death_m_max = 0;              // largest death age any row returns (its first 2^24 entry)
+    uint32_t m_rows = 1;
+    std::vector<uint32_t> death_m_guide;   // [m_rows][kGuideBuckets] death age of life_bucket_kmin(b)
+    bool climate = false;
+    std::vector<float> clim;               // with a climate: [max_days][3] beta_hm, beta_mh, eip of day t
     std::vector<uint32_t> init_h;          // [kHumanYears]     initial-age year thresholds
     std::vector<uint32_t> init_m;          // [kMosqInitDays]   initial-age day thresholds
-    std::vector<float> dens;               // [max_days+1] 1 + A sin(2 pi t/365 + phi)
+    std::vector<float> dens;               // [max_days+1] 1 + A sin(2 pi t/365 + phi), or M(t) / N
     std::vector<uint32_t> immune;          // [C*S] 24-bit immunity thresholds
     uint32_t seed_h = 0, seed_m = 0;       // 24-bit seeding thresholds
     // dispersal rows: row_ptr[C+1]; entry e: to[e] (community), thr[e] (24-bit
@@ -44,7 +51,11 @@ struct SimTables {
 };
 
 // Returns an empty string on success, else the reason the input is invalid.
-std::string validate(const ArboShape& shape, const ArboParams& p);
+std::string validate(const ArboShape& shape, const ArboParams& p, uint32_t max_days);
+
+// The parameters climate row `row` gives a day (SimulationParam.cpp:92-137):
+// out = {beta_hm, beta_mh, eip, mosquito Weibull scale}, in double.
+void climate_day(const ArboClimate& c, uint32_t row, double out[4]);
 
 // Builds every table of one simulation.  validate() must have passed.
 void build_tables(const ArboShape& shape, const ArboParams& p, uint64_t seed,

@@ -23,14 +23,43 @@ class ArboError(RuntimeError):
 
 
 @dataclass
+class ClimateSpec:
+    """Daily climate and its coefficients (include/arbo_c.h ArboClimate,
+    SimulationParam.cpp:92-137): temperature (deg C), relative humidity in
+    [0, 1) and precipitation per day, at least max_days + 1 days."""
+    temperature: np.ndarray
+    humidity: np.ndarray
+    precipitation: np.ndarray
+    biting_rate: float = 0.25
+    scale_eip: float = 2.5
+    scale_mle: float = 3.0
+    scale_cc: float = 1.0
+    scale_humidity: float = 1.0
+    min_mos_to_human: float = 3.0
+    mos_net_growth: float = 0.5
+
+    def struct(self, keep: list) -> abi.Climate:
+        """The ABI struct; the arrays it points at are appended to keep."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (self.temperature, self.humidity, self.precipitation)]
+        if not (arrs[0].size == arrs[1].size == arrs[2].size):
+            raise ValueError("climate columns differ in length")
+        keep.extend(arrs)
+        return abi.Climate(*(_ptr(a, C.c_float) for a in arrs), arrs[0].size, self.biting_rate, self.scale_eip,
+                           self.scale_mle, self.scale_cc, self.scale_humidity, self.min_mos_to_human,
+                           self.mos_net_growth)
+
+
+@dataclass
 class SimSpec:
-    """One simulation: lattice, humans per community, parameters and seed."""
+    """One simulation: lattice, humans per community, parameters and seed,
+    optionally the mosquito slots and a climate."""
     grid_w: int
     grid_h: int
     humans: np.ndarray            # uint32 [grid_w*grid_h]
     params: abi.Params
     seed: int
     mosq_slots: np.ndarray | None = None
+    climate: ClimateSpec | None = None
 
     @property
     def communities(self) -> int:
@@ -39,6 +68,20 @@ class SimSpec:
 
 def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def climate_day(climate: ClimateSpec, row: int, lib: C.CDLL | None = None) -> np.ndarray:
+    """{beta_hm, beta_mh, eip, mosquito Weibull scale} that climate row
+    `row` gives a day (arbo_climate_day); day t of a simulation uses row
+    t + 1."""
+    lib = lib if lib is not None else abi.load_hip_library()
+    keep: list = []
+    cl = climate.struct(keep)
+    out = np.zeros(4, np.float64)
+    rc = lib.arbo_climate_day(C.byref(cl), row, out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise ArboError(rc, f"arbo_climate_day(row {row})")
+    return out
 
 
 class Engine:
@@ -94,6 +137,10 @@ class Engine:
                 ms = np.ascontiguousarray(s.mosq_slots, dtype=np.uint32)
                 keep.append(ms)
                 shapes[i].mosq_slots = _ptr(ms, C.c_uint32)
+            if s.climate is not None:
+                cl = s.climate.struct(keep)
+                keep.append(cl)
+                shapes[i].climate = C.pointer(cl)
             params[i] = s.params
             seeds[i] = s.seed & 0xFFFFFFFFFFFFFFFF
         self._check(self.lib.arbo_batch_init(self._h, n, shapes, params, seeds, max_days))

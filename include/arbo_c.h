@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define ARBO_ABI_VERSION 2u
+#define ARBO_ABI_VERSION 3u
 #define ARBO_MAX_SEROTYPES 4u
 #define ARBO_MAX_COMMUNITY_HUMANS 8192u
 #define ARBO_MAX_COMMUNITY_SLOTS 8192u
@@ -116,13 +116,45 @@ typedef struct ArboParams {
 #define ARBO_COUNT_POISSON 0u
 #define ARBO_COUNT_STOCH_ROUND 1u
 
+/* Daily climate of one simulation and the coefficients that turn it into
+ * the day's parameters (SimulationParam::adjustClimate,
+ * SimulationParam.cpp:92-137; the logistic emergence target,
+ * simulation.cpp:87,129-131).  Day t of the simulation reads row t + 1 of
+ * the climate, row 0 sets the initial target; days >= max_days + 1.
+ * With a climate, per day t:
+ *   beta_hm(t) = biting_rate * nmBitingSuccess(T), beta_mh(t) = biting_rate *
+ *     mnBitingSuccess(T), both Briere curves of the temperature T (deg C);
+ *   eip(t) = 1 / (scale_eip * 24 * 0.003359 (TK/298) exp(15000/1.986
+ *     (1/298 - 1/TK))), TK = T + 273.15;
+ *   mosquito Weibull scale(t) = 1 / (scale_mle * mu(T) * (1 - H)^scale_humidity),
+ *     mu a quartic death rate in T; mosquitoes born on day b die on the first
+ *     day b + a with PLE <= F(a; scale(b + a)), m_shape unchanged (the t = 0
+ *     census keeps m_scale);
+ *   the whole simulation's target M(t+1) = M(t) + mos_net_growth M(t) (1 -
+ *     M(t) / K(t+1)), K(t) = min_mos_to_human N (1 + P(t))^scale_cc, M(0) =
+ *     K(0); community c's share is M(t) humans[c] / N.
+ * beta_hm, beta_mh, eip, m_scale, season_amp and season_phase of
+ * ArboParams are then not used by the day loop. */
+typedef struct ArboClimate {
+    const float* temperature;    /* [days] deg C (the climate file's T column) */
+    const float* humidity;       /* [days] relative humidity in [0, 1) (H) */
+    const float* precipitation;  /* [days] precipitation, > -1 (P) */
+    uint32_t days;
+    float biting_rate;           /* bites per mosquito per day (bitingRate) */
+    float scale_eip, scale_mle, scale_cc, scale_humidity;
+    float min_mos_to_human;      /* minMosToHuman */
+    float mos_net_growth;        /* mosNetGrowth */
+} ArboClimate;
+
 /* Shape of one simulation: a grid_w x grid_h non-wrapping lattice of
  * communities (community id = y*grid_w + x) and the humans of each. */
 typedef struct ArboShape {
     uint32_t grid_w, grid_h;
     const uint32_t* humans;      /* [grid_w*grid_h], each 1..ARBO_MAX_COMMUNITY_HUMANS */
     const uint32_t* mosq_slots;  /* [grid_w*grid_h] slot capacity, or NULL for
-                                    ceil(m_ratio*(1+season_amp)*humans[c]) */
+                                    ceil(m_ratio*(1+season_amp)*humans[c]), with a
+                                    climate ceil(max_t M(t)/N * humans[c]) + 1 */
+    const ArboClimate* climate;  /* NULL: the seasonal model of ArboParams */
 } ArboShape;
 
 /* Time series of one simulation, layout of the reference's d_n* arrays
@@ -216,6 +248,12 @@ ARBO_API int arbo_read_history(ArboEngine* e, uint32_t sim, int32_t* out);
  * day enter in human order (community order, then index).  Rings start at
  * zero.  out[k * ARBO_AOI_RECORD + r], 4 * ARBO_AOI_RECORD words. */
 ARBO_API int arbo_read_aoi(ArboEngine* e, uint32_t sim, uint32_t* out);
+
+/* The parameters climate row `row` gives a day (SimulationParam::adjustClimate,
+ * SimulationParam.cpp:92-137), before their rounding to float: out[4] =
+ * {beta_hm, beta_mh, eip, mosquito Weibull scale}.  Day t of a simulation
+ * uses row t + 1. */
+ARBO_API int arbo_climate_day(const ArboClimate* climate, uint32_t row, double* out);
 
 /* Kernel timing (ARBO_F_TIMING): total milliseconds and launches of the
  * demography kernel (kind 0) and event-application kernel (kind 1) since the

@@ -66,6 +66,11 @@ struct Sim {
     std::vector<uint32_t> in_ptr, in_src, in_slot;
     std::vector<uint32_t> Kh, Km, Thy, Tmd;
     std::vector<float> dens;
+    // climate: per-day beta_hm, beta_mh, eip and the lifetime table of each
+    // birth day (the t = 0 census keeps Km)
+    bool climate = false;
+    std::vector<float> day_bhm, day_bmh, day_eip;
+    std::vector<std::vector<uint32_t>> Kmb;
     std::vector<uint32_t> imm_thr;
     uint32_t seed_thr_h = 0, seed_thr_m = 0;
     // census
@@ -132,6 +137,94 @@ uint32_t to24(double f) {
     if (!(v > 0.0)) return 0;
     if (v >= 16777216.0) return ONE24;
     return (uint32_t)v;
+}
+
+// ---- climate-driven days (SimulationParam::adjustClimate,
+// SimulationParam.cpp:92-137; the logistic target, simulation.cpp:84-132) ----
+
+// Briere curve c T (T - lo) sqrt(hi - T), zero outside [lo, hi]
+// (SimulationParam.cpp:114-136).
+double briere(double T, double c, double lo, double hi) {
+    return (T < lo || T > hi) ? 0.0 : c * T * (T - lo) * std::sqrt(hi - T);
+}
+
+// {beta_hm, beta_mh, eip, mosquito Weibull scale} of climate row r.
+void clim_day(const ArboClimate& c, uint32_t r, double out[4]) {
+    const double T = (double)c.temperature[r], H = (double)c.humidity[r];
+    const double TK = T + 273.15;
+    // EIP from the temperature (SimulationParam.cpp:97-100)
+    const double rate = 24.0 * 0.003359 * (TK / 298.0) * exp_d((15000.0 / 1.986) * ((1.0 / 298.0) - (1.0 / TK)));
+    // mosquito death rate quartic and humidity (SimulationParam.cpp:104-107)
+    const double mu = 0.8692 - 0.159 * T + 0.01116 * (T * T) - 0.0003408 * (T * T * T) + 0.000003809 * (T * T * T * T);
+    out[0] = (double)c.biting_rate * briere(T, 0.000491, 12.22, 37.46);
+    out[1] = (double)c.biting_rate * briere(T, 0.000849, 17.05, 35.83);
+    out[2] = 1.0 / (rate * (double)c.scale_eip);
+    out[3] = 1.0 / (mu * (double)c.scale_mle * pw(1.0 - H, (double)c.scale_humidity));
+}
+
+// minMosToHuman N (1 + P)^scaleCC (simulation.cpp:87,131).
+double clim_capacity(const ArboClimate& c, uint32_t r, double N) {
+    return (double)c.min_mos_to_human * N * pw(1.0 + (double)c.precipitation[r], (double)c.scale_cc);
+}
+
+// h_mExpectedPopSize: M(0) = K(0), M(t+1) = M(t) + g M(t) (1 - M(t)/K(t+1)), in float.
+std::vector<float> clim_target(const ArboClimate& c, double N, uint32_t max_days) {
+    std::vector<float> M(max_days + 1);
+    M[0] = (float)clim_capacity(c, 0, N);
+    for (uint32_t t = 0; t < max_days; ++t) {
+        double m = (double)M[t];
+        M[t + 1] = (float)(m + (double)c.mos_net_growth * m * (1.0 - m / clim_capacity(c, t + 1, N)));
+    }
+    return M;
+}
+
+const char* clim_invalid(const ArboClimate& c, uint32_t max_days) {
+    if (!c.temperature || !c.humidity || !c.precipitation) return "climate arrays missing";
+    if (c.days < max_days + 1) return "climate shorter than max_days + 1 days";
+    if (!(c.biting_rate >= 0.0f) || !(c.scale_eip > 0.0f) || !(c.scale_mle > 0.0f) || !(c.min_mos_to_human > 0.0f) ||
+        !(c.mos_net_growth >= 0.0f) || !std::isfinite(c.scale_cc) || !std::isfinite(c.scale_humidity))
+        return "bad climate coefficients";
+    for (uint32_t r = 0; r <= max_days; ++r) {
+        if (!std::isfinite(c.temperature[r]) || !(c.humidity[r] >= 0.0f && c.humidity[r] < 1.0f) ||
+            !(c.precipitation[r] > -1.0f) || !std::isfinite(c.precipitation[r]))
+            return "climate row out of range";
+        double d[4];
+        clim_day(c, r, d);
+        if (!(d[2] > 0.0 && d[2] < 1000.0)) return "climate gives eip out of range";
+        if (!(d[3] > 0.0 && d[3] < 1.0e4)) return "climate gives a mosquito lifespan scale out of range";
+    }
+    return nullptr;
+}
+
+// Climate tables of a simulation: the days' parameters (day t reads row
+// t + 1), the target M(t)/N, and for each birth day b the first-passage
+// table of mdemographic.cu:72 under the day's scale: entry a is the largest
+// floor(F(a'; scale(b + a')) 2^24) over a' <= a (days past the run keep the
+// last day's scale), up to its first 2^24 entry.
+void init_climate(Sim& s, const ArboClimate& c, uint32_t max_days) {
+    s.climate = true;
+    s.day_bhm.resize(max_days); s.day_bmh.resize(max_days); s.day_eip.resize(max_days);
+    std::vector<double> scale(max_days);
+    for (uint32_t t = 0; t < max_days; ++t) {
+        double d[4];
+        clim_day(c, t + 1, d);
+        s.day_bhm[t] = (float)d[0]; s.day_bmh[t] = (float)d[1]; s.day_eip[t] = (float)d[2];
+        scale[t] = d[3];
+    }
+    std::vector<float> M = clim_target(c, (double)s.NH, max_days);
+    s.dens.resize(max_days + 1);
+    for (uint32_t t = 0; t <= max_days; ++t) s.dens[t] = (float)((double)M[t] / (double)s.NH);
+    s.Kmb.assign(max_days, {});
+    for (uint32_t b = 0; b < max_days; ++b) {
+        std::vector<uint32_t>& K = s.Kmb[b];
+        K.assign(1, 0u);
+        uint32_t run = 0;
+        for (uint32_t a = 1; a < MOSQ_AGE_TABLE && run < ONE24; ++a) {
+            const uint32_t day = (uint32_t)std::min<uint64_t>((uint64_t)b + a, max_days - 1);
+            run = std::max(run, to24(1.0 - exp_d(-pw((double)a / scale[day], (double)s.p.m_shape))));
+            K.push_back(run);
+        }
+    }
 }
 
 // Lifetime threshold table: K[a] = floor(F(a) 2^24), non-decreasing.
@@ -289,11 +382,13 @@ void init_sim(Sim& s, uint32_t max_days) {
     s.Km = death_table(MOSQ_AGE_TABLE, [&](double a) { return cdf_m(p, a); });
     s.Thy = age_table(151, 365, [&](double a) { return cdf_h(p, a); });
     s.Tmd = age_table(301, 1, [&](double a) { return cdf_m(p, a); });
-    // seasonal density 1 + A sin(2 pi t/365 + phi)
-    s.dens.resize(max_days + 1);
-    for (uint32_t t = 0; t <= max_days; ++t) {
-        double arg = 6.283185307179586 * (double)t / 365.0 + (double)p.season_phase;
-        s.dens[t] = (float)(1.0 + (double)p.season_amp * sin_d(arg));
+    // seasonal density 1 + A sin(2 pi t/365 + phi) (a climate set it already)
+    if (!s.climate) {
+        s.dens.resize(max_days + 1);
+        for (uint32_t t = 0; t <= max_days; ++t) {
+            double arg = 6.283185307179586 * (double)t / 365.0 + (double)p.season_phase;
+            s.dens[t] = (float)(1.0 + (double)p.season_amp * sin_d(arg));
+        }
     }
     // immunity thresholds per (community, serotype)
     s.imm_thr.assign((size_t)s.C * S, 0);
@@ -519,7 +614,7 @@ void phase_a(Sim& s, uint32_t c, uint32_t t) {
         if (rank < births) {
             // birth `rank` of the community draws word rank%4 of block rank/4
             W4 r = rnd(s.key, c, t, M_BIRTH, rank >> 2);
-            uint32_t da = death_age(s.Km, (r.v[rank & 3u] >> 8) + 1);
+            uint32_t da = death_age(s.climate ? s.Kmb[t] : s.Km, (r.v[rank & 3u] >> 8) + 1);
             s.mw[m] = (uint16_t)(((t + da) & 0xFFFu) << 4);
         } else if (died[j]) {
             s.mw[m] = 3;
@@ -542,8 +637,10 @@ void phase_a(Sim& s, uint32_t c, uint32_t t) {
             uint64_t fx = 0;
             if (v < S) {
                 float lam = 0.0f;
-                if (dir == HM && Ih[v] > 0) lam = (float)Mc / (float)s.N[c] * p.beta_hm * (float)Ih[v];
-                if (dir == MH && Im[v] > 0) lam = p.beta_mh * (float)Im[v];
+                const float bhm = s.climate ? s.day_bhm[t] : p.beta_hm;   // the day's (adjustClimate)
+                const float bmh = s.climate ? s.day_bmh[t] : p.beta_mh;
+                if (dir == HM && Ih[v] > 0) lam = (float)Mc / (float)s.N[c] * bhm * (float)Ih[v];
+                if (dir == MH && Im[v] > 0) lam = bmh * (float)Im[v];
                 if (lam > 0.0f) {
                     uint32_t stay;
                     const uint32_t n = source_counts(p.count_law, lam, p.long_distance, p.no_distance, s.key, c, t,
@@ -618,7 +715,7 @@ void phase_b(Sim& s, uint32_t d, uint32_t t) {
                     if ((w & 3u) != 0) continue;
                     W4 r2 = rnd(s.key, m, t, M_EIP, 0);
                     s.mw[m] = (uint16_t)((w & ~0xFu) | (E.s << 2) | 1u);
-                    s.mn[m] = (uint16_t)(t + sround(p.eip, r2.v[0]));
+                    s.mn[m] = (uint16_t)(t + sround(s.climate ? s.day_eip[t] : p.eip, r2.v[0]));
                 }
         }
     }
@@ -756,10 +853,26 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes,
         uint64_t nh = 0, nm = 0;
         for (uint32_t c = 0; c < s.C; ++c) {
             if (s.N[c] < 1 || s.N[c] > ARBO_MAX_COMMUNITY_HUMANS) return fail(e, ARBO_E_INVALID, "humans per community out of range");
+            nh += s.N[c];
+        }
+        s.climate = false;
+        double peak = 0.0;   // largest M(t)/N under a climate
+        if (sh.climate) {
+            if (const char* why = clim_invalid(*sh.climate, max_days)) return fail(e, ARBO_E_INVALID, why);
+            s.NH = nh;
+            init_climate(s, *sh.climate, max_days);
+            for (float m : clim_target(*sh.climate, (double)nh, max_days)) {
+                if (!std::isfinite(m) || !(m >= 0.0f)) return fail(e, ARBO_E_INVALID, "climate gives an invalid mosquito target");
+                peak = std::max(peak, (double)m / (double)nh);
+            }
+        }
+        nh = 0;
+        for (uint32_t c = 0; c < s.C; ++c) {
             s.cap[c] = sh.mosq_slots ? sh.mosq_slots[c]
-                                     : (uint32_t)std::ceil((double)s.p.m_ratio * (1.0 + (double)s.p.season_amp) * (double)s.N[c]);
+                       : s.climate ? (uint32_t)(std::ceil(peak * (double)s.N[c]) + 1.0)
+                                   : (uint32_t)std::ceil((double)s.p.m_ratio * (1.0 + (double)s.p.season_amp) * (double)s.N[c]);
             if (s.cap[c] > ARBO_MAX_COMMUNITY_SLOTS) return fail(e, ARBO_E_INVALID, "mosquito slots per community out of range");
-            s.base[c] = s.p.m_ratio * (float)s.N[c];
+            s.base[c] = s.climate ? (float)s.N[c] : s.p.m_ratio * (float)s.N[c];
             s.h_off[c] = nh; s.m_off[c] = nm;
             nh += s.N[c]; nm += s.cap[c];
         }
@@ -861,6 +974,12 @@ int arbo_read_history(ArboEngine* e, uint32_t sim, int32_t* out) {
     const Sim& s = e->sims[sim];
     for (uint64_t h = 0; h < s.NH; ++h)
         for (uint32_t v = 0; v < s.S; ++v) out[v * s.NH + h] = s.hist[h * 4 + v];
+    return ARBO_OK;
+}
+
+int arbo_climate_day(const ArboClimate* c, uint32_t row, double* out) {
+    if (!c || !out || !c->temperature || !c->humidity || !c->precipitation || row >= c->days) return ARBO_E_INVALID;
+    clim_day(*c, row, out);
     return ARBO_OK;
 }
 

@@ -6,8 +6,9 @@ the 10x10-lattice workload (1e5 humans, 365 days, in full), the reference's own 
 Weibull(4, 23 d) mosquitoes, external introductions), isolated and ragged
 communities (1 human; the maximum of 8,192 humans and slots; a community with
 no mosquito slots), large Poisson means (the PTRS branch), Beta(2,5) immunity,
-both event-count laws with and without a staying fraction, and two
-simulations of the 256-simulation batch.
+both event-count laws with and without a staying fraction, two
+simulations of the 256-simulation batch, and climate-driven days (per-day
+transmission, EIP and mosquito lifetimes, the logistic emergence target).
 """
 from __future__ import annotations
 
@@ -69,6 +70,20 @@ def case_poisson_stay():
     return [SimSpec(5, 5, np.full(25, 600, np.uint32), a, 17), SimSpec(4, 6, np.full(24, 400, np.uint32), b, 18)], 90
 
 
+def case_climate():
+    """The reference's parameter set under a seasonal climate, and the
+    configured model under a faster-varying one beside it."""
+    T = 200
+    a = configs.reference_params(seed_frac_h=0.02, seed_frac_m=0.02)
+    ca = configs.seasonal_climate(T + 1, biting_rate=0.5)
+    b = configs.base_params(seed_frac_h=0.01, seed_frac_m=0.01)
+    cb = configs.seasonal_climate(T + 1, t_amp=6.0, h_amp=0.2, p_amp=0.6, phase=1.0, biting_rate=0.8,
+                                  mos_net_growth=0.3, scale_mle=2.0)
+    cb.humidity = np.where(np.arange(T + 1) % 3 == 0, cb.humidity, cb.humidity * 0.5).astype(np.float32)
+    return [SimSpec(5, 5, np.full(25, 300, np.uint32), a, 31, climate=ca),
+            SimSpec(4, 4, np.full(16, 500, np.uint32), b, 32, climate=cb)], T
+
+
 def case_batch_pair():
     sims = [configs.batch_sim(i) for i in range(24)]
     sims.sort(key=lambda s: int(s.humans.sum()))
@@ -85,6 +100,7 @@ CASES = {
     "stoch_round_moore": case_stoch_round_moore,
     "poisson_stay": case_poisson_stay,
     "batch_pair": case_batch_pair,
+    "climate": case_climate,
 }
 
 # days at which the full census is compared (plus the last day)
