@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_SEROTYPES = 4
 MAX_COMMUNITY_HUMANS = 8192
 MAX_COMMUNITY_SLOTS = 8192
@@ -68,7 +68,7 @@ class Climate(C.Structure):
 
 class Shape(C.Structure):
     _fields_ = [("grid_w", _u32), ("grid_h", _u32), ("humans", _u32p), ("mosq_slots", _u32p),
-                ("climate", C.POINTER(Climate))]
+                ("climate", C.POINTER(Climate)), ("mosq_share", C.POINTER(C.c_float))]
 
 
 class Series(C.Structure):

@@ -20,7 +20,8 @@ CSRC = os.path.join(PKG, "csrc")
 OUT_DIR = os.path.join(PKG, "_build")
 LIB = os.path.join(OUT_DIR, "libarbo_hip.so")
 
-SOURCES = ["arbo_kernels.hip", "arbo_engine.cpp", "arbo_tables.cpp", "arbo_mcmc.cpp", "arbo_surrogate.hip", "arbo_output.cpp"]
+SOURCES = ["arbo_kernels.hip", "arbo_engine.cpp", "arbo_tables.cpp", "arbo_mcmc.cpp", "arbo_surrogate.hip", "arbo_output.cpp",
+           "arbo_data.cpp"]
 HEADERS = ["arbo_rng.h", "arbo_device.h", "arbo_tables.h"]
 
 ARCH = os.environ.get("ARBO_OFFLOAD_ARCH", "gfx950")

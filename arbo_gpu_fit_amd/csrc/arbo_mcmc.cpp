@@ -101,6 +101,7 @@ struct ArboMcmc {
     bool has_slots = false;
     // the climate every proposal's simulations run under (copied), if any
     bool has_climate = false;
+    std::vector<float> share;   // mosquito shares per community, if given
     std::vector<float> clim_t, clim_h, clim_p;
     ArboClimate climate{};
     ArboParams base{};
@@ -243,7 +244,7 @@ int eval_engine(ArboMcmc* m, ArboEngine* eng, const ArboEngineApi& api, uint32_t
         for (uint32_t p = 0; p < P; ++p) {
             const uint32_t k = i * P + p;
             shapes[k] = ArboShape{m->grid_w, m->grid_h, m->humans.data(), m->has_slots ? m->slots.data() : nullptr,
-                                  m->has_climate ? &m->climate : nullptr};
+                                  m->has_climate ? &m->climate : nullptr, m->share.empty() ? nullptr : m->share.data()};
             arbo_mcmc_sim_params(th + (size_t)i * D, &m->base, &params[k]);
             seeds[k] = sim_seed(m, chains[i], it, p);
         }
@@ -384,6 +385,7 @@ ARBO_API int arbo_mcmc_create(const ArboMcmcDesc* desc, const ArboShape* shape, 
     m->grid_h = shape->grid_h;
     const size_t C_comm = (size_t)shape->grid_w * shape->grid_h;
     m->humans.assign(shape->humans, shape->humans + C_comm);
+    if (shape->mosq_share) m->share.assign(shape->mosq_share, shape->mosq_share + C_comm);
     if (shape->climate) {
         const ArboClimate& c = *shape->climate;
         if (!c.temperature || !c.humidity || !c.precipitation) {

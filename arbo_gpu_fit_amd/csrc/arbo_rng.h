@@ -64,6 +64,7 @@ enum Purpose : uint32_t {
     P_LD_DEST = 29,    // destination of the l-th long-distance event (x0 = l)
     P_NSR = 30,        // stochastic-rounding event count of a source and its splits
     P_NSTAY = 31,      // Poisson count of a source's events that stay at the source
+    P_SIZES = 32,      // community of individual 4 x0 + word (x1 = INIT_DAY) of arbo_community_sizes
 };
 
 // Direction of a transmission event.

@@ -156,12 +156,23 @@ std::string validate(const ArboShape& sh, const ArboParams& p, uint32_t max_days
             clim_peak = std::max(clim_peak, (double)m / (double)nh);
         }
     }
+    if (sh.mosq_share) {
+        double sum = 0.0;
+        for (uint64_t c = 0; c < C; ++c) {
+            if (!(sh.mosq_share[c] > 0.0f) || !std::isfinite(sh.mosq_share[c])) return "mosquito shares must be positive";
+            sum += (double)sh.mosq_share[c];
+        }
+        if (!(std::fabs(sum - 1.0) <= 1e-3)) return "mosquito shares must sum to 1";
+    }
+    const double N = (double)nh;
     nh = 0;
     for (uint64_t c = 0; c < C; ++c) {
         const uint32_t n = sh.humans[c];
+        // humans-equivalent of the community's mosquito share
+        const double hs = sh.mosq_share ? N * (double)sh.mosq_share[c] : (double)n;
         const double cap = sh.mosq_slots ? (double)sh.mosq_slots[c]
-                         : sh.climate ? ceil(clim_peak * (double)n) + 1.0
-                                      : ceil((double)p.m_ratio * (1.0 + (double)p.season_amp) * (double)n);
+                         : sh.climate ? ceil(clim_peak * hs) + 1.0
+                                      : ceil((double)p.m_ratio * (1.0 + (double)p.season_amp) * hs);
         if (cap > (double)ARBO_MAX_COMMUNITY_SLOTS) return "mosquito slots per community out of range";
         nh += n;
         nm += (uint64_t)cap;
@@ -337,10 +348,13 @@ void build_tables(const ArboShape& sh, const ArboParams& p, uint64_t seed, uint3
         }
     }
     for (uint32_t c = 0; c < t.C; ++c) {
+        // humans-equivalent of the community's mosquito share (its humans without shares)
+        const double hs = sh.mosq_share ? (double)t.n_humans * (double)sh.mosq_share[c] : (double)t.humans[c];
         t.slots[c] = sh.mosq_slots ? sh.mosq_slots[c]
-                     : t.climate ? (uint32_t)(ceil(clim_peak * (double)t.humans[c]) + 1.0)
-                                 : (uint32_t)ceil((double)p.m_ratio * (1.0 + (double)p.season_amp) * (double)t.humans[c]);
-        t.base[c] = t.climate ? (float)t.humans[c] : p.m_ratio * (float)t.humans[c];
+                     : t.climate ? (uint32_t)(ceil(clim_peak * hs) + 1.0)
+                                 : (uint32_t)ceil((double)p.m_ratio * (1.0 + (double)p.season_amp) * hs);
+        if (!sh.mosq_share) t.base[c] = t.climate ? (float)t.humans[c] : p.m_ratio * (float)t.humans[c];
+        else t.base[c] = t.climate ? (float)hs : (float)((double)p.m_ratio * hs);
         t.n_slots += t.slots[c];
     }
     threshold_table(kHumanAgeTable, [&](double a) { return human_cdf(p, a); }, t.death_h);

@@ -60,6 +60,7 @@ class SimSpec:
     seed: int
     mosq_slots: np.ndarray | None = None
     climate: ClimateSpec | None = None
+    mosq_share: np.ndarray | None = None   # float [communities], sums to 1
 
     @property
     def communities(self) -> int:
@@ -137,6 +138,11 @@ class Engine:
                 ms = np.ascontiguousarray(s.mosq_slots, dtype=np.uint32)
                 keep.append(ms)
                 shapes[i].mosq_slots = _ptr(ms, C.c_uint32)
+            if s.mosq_share is not None:
+                sh = np.ascontiguousarray(s.mosq_share, dtype=np.float32)
+                assert sh.size == s.communities
+                keep.append(sh)
+                shapes[i].mosq_share = _ptr(sh, C.c_float)
             if s.climate is not None:
                 cl = s.climate.struct(keep)
                 keep.append(cl)

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define ARBO_ABI_VERSION 3u
+#define ARBO_ABI_VERSION 4u
 #define ARBO_MAX_SEROTYPES 4u
 #define ARBO_MAX_COMMUNITY_HUMANS 8192u
 #define ARBO_MAX_COMMUNITY_SLOTS 8192u
@@ -152,9 +152,13 @@ typedef struct ArboShape {
     uint32_t grid_w, grid_h;
     const uint32_t* humans;      /* [grid_w*grid_h], each 1..ARBO_MAX_COMMUNITY_HUMANS */
     const uint32_t* mosq_slots;  /* [grid_w*grid_h] slot capacity, or NULL for
-                                    ceil(m_ratio*(1+season_amp)*humans[c]), with a
-                                    climate ceil(max_t M(t)/N * humans[c]) + 1 */
+                                    ceil(m_ratio*(1+season_amp)*N*share[c]), with a
+                                    climate ceil(max_t M(t) share[c]) + 1 */
     const ArboClimate* climate;  /* NULL: the seasonal model of ArboParams */
+    const float* mosq_share;     /* [grid_w*grid_h] each community's share of the
+                                    simulation's mosquitoes (positive, summing to 1
+                                    within 1e-3; mHeteroPopSize sizes,
+                                    network.cpp:138-159), or NULL: humans[c] / N */
 } ArboShape;
 
 /* Time series of one simulation, layout of the reference's d_n* arrays

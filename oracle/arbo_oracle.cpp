@@ -866,13 +866,26 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes,
                 peak = std::max(peak, (double)m / (double)nh);
             }
         }
+        // mosquito shares (mHeteroPopSize sizes, network.cpp:138-159), else by humans
+        if (sh.mosq_share) {
+            double sum = 0.0;
+            for (uint32_t c = 0; c < s.C; ++c) {
+                if (!(sh.mosq_share[c] > 0.0f) || !std::isfinite(sh.mosq_share[c]))
+                    return fail(e, ARBO_E_INVALID, "mosquito shares must be positive");
+                sum += (double)sh.mosq_share[c];
+            }
+            if (!(std::fabs(sum - 1.0) <= 1e-3)) return fail(e, ARBO_E_INVALID, "mosquito shares must sum to 1");
+        }
+        const double Ntot = (double)nh;
         nh = 0;
         for (uint32_t c = 0; c < s.C; ++c) {
+            const double hs = sh.mosq_share ? Ntot * (double)sh.mosq_share[c] : (double)s.N[c];
             s.cap[c] = sh.mosq_slots ? sh.mosq_slots[c]
-                       : s.climate ? (uint32_t)(std::ceil(peak * (double)s.N[c]) + 1.0)
-                                   : (uint32_t)std::ceil((double)s.p.m_ratio * (1.0 + (double)s.p.season_amp) * (double)s.N[c]);
+                       : s.climate ? (uint32_t)(std::ceil(peak * hs) + 1.0)
+                                   : (uint32_t)std::ceil((double)s.p.m_ratio * (1.0 + (double)s.p.season_amp) * hs);
             if (s.cap[c] > ARBO_MAX_COMMUNITY_SLOTS) return fail(e, ARBO_E_INVALID, "mosquito slots per community out of range");
-            s.base[c] = s.climate ? (float)s.N[c] : s.p.m_ratio * (float)s.N[c];
+            if (!sh.mosq_share) s.base[c] = s.climate ? (float)s.N[c] : s.p.m_ratio * (float)s.N[c];
+            else s.base[c] = s.climate ? (float)hs : (float)((double)s.p.m_ratio * hs);
             s.h_off[c] = nh; s.m_off[c] = nm;
             nh += s.N[c]; nm += s.cap[c];
         }

@@ -7,8 +7,9 @@ Weibull(4, 23 d) mosquitoes, external introductions), isolated and ragged
 communities (1 human; the maximum of 8,192 humans and slots; a community with
 no mosquito slots), large Poisson means (the PTRS branch), Beta(2,5) immunity,
 both event-count laws with and without a staying fraction, two
-simulations of the 256-simulation batch, and climate-driven days (per-day
-transmission, EIP and mosquito lifetimes, the logistic emergence target).
+simulations of the 256-simulation batch, climate-driven days (per-day
+transmission, EIP and mosquito lifetimes, the logistic emergence target), and
+degree-heterogeneous human and mosquito community sizes.
 """
 from __future__ import annotations
 
@@ -84,6 +85,15 @@ def case_climate():
             SimSpec(4, 4, np.full(16, 500, np.uint32), b, 32, climate=cb)], T
 
 
+def case_hetero():
+    """Humans drawn over the communities by degree^1.5 and mosquito shares by
+    degree^2.5 (network.cpp:138-159), von Neumann lattice."""
+    from arbo_gpu_fit_amd.data import community_sizes, mosquito_shares
+    humans = community_sizes(6, 5, 15000, 1.5, seed=7, neighbourhood=4)
+    p = configs.base_params(neighbourhood=4, seed_frac_h=0.01, seed_frac_m=0.01, beta_hm=0.4, beta_mh=0.4)
+    return [SimSpec(6, 5, humans, p, 19, mosq_share=mosquito_shares(6, 5, 2.5, neighbourhood=4))], 90
+
+
 def case_batch_pair():
     sims = [configs.batch_sim(i) for i in range(24)]
     sims.sort(key=lambda s: int(s.humans.sum()))
@@ -101,6 +111,7 @@ CASES = {
     "poisson_stay": case_poisson_stay,
     "batch_pair": case_batch_pair,
     "climate": case_climate,
+    "hetero": case_hetero,
 }
 
 # days at which the full census is compared (plus the last day)

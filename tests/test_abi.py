@@ -9,11 +9,11 @@ import subprocess
 
 import pytest
 
-from arbo_gpu_fit_amd import abi, mcmc, output, surrogate
+from arbo_gpu_fit_amd import abi, data, mcmc, output, surrogate
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADERS = {"arbo_c.h": abi.PROTOTYPES, "arbo_mcmc.h": mcmc.PROTOTYPES, "arbo_surrogate.h": surrogate.PROTOTYPES,
-           "arbo_output.h": output.PROTOTYPES}
+           "arbo_output.h": output.PROTOTYPES, "arbo_data.h": data.PROTOTYPES}
 
 
 def declared_functions(header=None):
@@ -41,6 +41,7 @@ def test_hip_library_exports_every_symbol():
     mcmc.bind(lib)
     surrogate.bind(lib)
     output.bind(lib)
+    data.bind(lib)
     out = subprocess.run(["nm", "-D", "--defined-only", abi.HIP_LIB_PATH], capture_output=True, text=True).stdout
     exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
     assert {n for n in exported if n.startswith("arbo_")} == set(declared_functions())
@@ -69,6 +70,7 @@ int main(void) {
            sizeof(ArboEngineDesc), sizeof(ArboParams), sizeof(ArboShape), sizeof(ArboSeries), sizeof(ArboState));
     printf("ArboPrior %zu\nArboMcmcDesc %zu\nArboEngineApi %zu\n", sizeof(ArboPrior), sizeof(ArboMcmcDesc),
            sizeof(ArboEngineApi));
+    printf("ArboClimate %zu\n", sizeof(ArboClimate));
     FIELDS
     return 0;
 }
@@ -77,6 +79,7 @@ int main(void) {
 
 def test_ctypes_layout_matches_header(tmp_path):
     structs = {"ArboEngineDesc": abi.EngineDesc, "ArboParams": abi.Params, "ArboShape": abi.Shape,
+               "ArboClimate": abi.Climate,
                "ArboSeries": abi.Series, "ArboState": abi.State, "ArboPrior": mcmc.Prior,
                "ArboMcmcDesc": mcmc.McmcDesc, "ArboEngineApi": mcmc.EngineApi}
     fields = "".join(f"F({cname}, {f})" for cname, cls in structs.items() for f, _ in cls._fields_)
