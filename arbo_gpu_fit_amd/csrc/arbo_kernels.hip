@@ -279,6 +279,7 @@ __device__ __forceinline__ uint32_t day_of(uint32_t t, uint32_t nx) {   // absol
 // Four humans of ndemographic.cu:57-103 (one 16-byte vector of packed words,
 // index 4q..4q+3).  The next-event days of the vector are read with one
 // 8-byte load, only when one of its members is infected and survives.
+template <bool HIST>
 __device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& cd, const SimDesc& sd, Key key,
                                           uint32_t t, uint32_t q, uint4 w4, HumanCount& hc) {
     const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
@@ -318,7 +319,7 @@ __device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& c
             const uint32_t da = death_age(v.tables + sd.death_h_off, kHumanTable - 1, (r.x >> 8) + 1);
             v.h_word[gi] = (t + da) << 8;
             v.h_birth[gi] = (int32_t)t;
-            if (v.hist) reinterpret_cast<int4*>(v.hist)[gi] = make_int4(-1, -1, -1, -1);   // naive newborn
+            if (HIST && v.hist) reinterpret_cast<int4*>(v.hist)[gi] = make_int4(-1, -1, -1, -1);   // naive newborn
             hc.minev = t + da < hc.minev ? t + da : hc.minev;
             continue;
         }
@@ -349,7 +350,10 @@ __device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& c
     }
 }
 
-template <uint32_t UNROLL, uint32_t EARLY_M, bool XCD>
+// HIST: the infection history or the per-community series may be kept
+// (v.hist or v.spatial not null); a separate instantiation keeps their
+// stores out of the plain day loop.
+template <uint32_t UNROLL, uint32_t EARLY_M, bool XCD, bool HIST>
 __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
     static_assert(EARLY_M >= 1 && EARLY_M <= UNROLL, "early mosquito batch within the first unrolled batch");
     const uint32_t g = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, lane = threadIdx.x;
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
 #pragma unroll
     for (uint32_t u = 0; u < UNROLL; ++u) {
         const uint32_t q = u * kWave + lane;
-        if (scan_h && q < nvec) human_vec(v, cd, sd, key, t, q, hb[u], hc);
+        if (scan_h && q < nvec) human_vec<HIST>(v, cd, sd, key, t, q, hb[u], hc);
     }
     for (uint32_t q0 = UNROLL * kWave; scan_h && q0 < nvec; q0 += UNROLL * kWave) {
         uint4 w4[UNROLL];
@@ -457,7 +461,7 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
 #pragma unroll
         for (uint32_t u = 0; u < UNROLL; ++u) {
             const uint32_t q = q0 + u * kWave + lane;
-            if (q < nvec) human_vec(v, cd, sd, key, t, q, w4[u], hc);
+            if (q < nvec) human_vec<HIST>(v, cd, sd, key, t, q, w4[u], hc);
         }
     }
 
@@ -721,7 +725,7 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
         if (ih) atomicAdd(&v.series[si.n_inf + (size_t)row * S + lane], ih);
         if (ic) atomicAdd(&v.series[si.n_inc + (size_t)row * S + lane], ic);
         if (cd.c == 0) v.series[si.n_one + (size_t)row * S + lane] = ih;
-        if (v.spatial) v.spatial[sd.spatial_off + ((uint64_t)row * S + lane) * sd.C + cd.c] = ih;
+        if (HIST && v.spatial) v.spatial[sd.spatial_off + ((uint64_t)row * S + lane) * sd.C + cd.c] = ih;
     }
 }
 
@@ -803,7 +807,7 @@ constexpr uint32_t kHumanKey = ARBO_MAX_COMMUNITY_SLOTS;   // claim keys of huma
 static_assert(ARBO_MAX_COMMUNITY_SLOTS + ARBO_MAX_COMMUNITY_HUMANS <= (1u << kKeyBits),
               "mosquito and human claim keys fit the key field");
 
-template <bool XCD>
+template <bool XCD, bool HIST>
 __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     const uint32_t g = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, lane = threadIdx.x;
     const uint32_t t = *v.day;
@@ -816,6 +820,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     const uint32_t S = sd.S;
     const uint32_t in_len = cd.in_len;
     const uint32_t nseg = S * (in_len + 1);
+    const float eip = day_params(sd, v.dens, t).eip;   // extrinsic incubation of today's infections
 
     __shared__ uint32_t s_src[9], s_slot[9];
     __shared__ uint32_t s_ecnt[8][9];
@@ -961,8 +966,10 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                     const uint32_t h = k - kHumanKey;
                     const uint64_t gi = cd.h_off + h;
                     v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
-                    if (v.hist) v.hist[4 * gi + s] = (int32_t)t - v.h_birth[gi];   // age at infection
-                    if (v.aoi_mark && t >= v.aoi_from) v.aoi_mark[gi] = 1;
+                    if (HIST) {
+                        v.hist[4 * gi + s] = (int32_t)t - v.h_birth[gi];   // age at infection
+                        if (v.aoi_mark && t >= v.aoi_from) v.aoi_mark[gi] = 1;
+                    }
                     const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
                     v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
                     infected = true;
@@ -970,7 +977,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                     const uint64_t gm = cd.m_off + k;
                     v.m_word[gm] = (uint16_t)((w & ~0xFu) | (s << 2) | 1u);
                     const U4 r2 = draw(key, cd.m_base + k, t, P_M_EIP, 0);
-                    v.m_next[gm] = (uint16_t)(t + stoch_round(day_params(sd, v.dens, t).eip, r2.x));
+                    v.m_next[gm] = (uint16_t)(t + stoch_round(eip, r2.x));
                 }
             }
             __syncthreads();
@@ -995,7 +1002,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
                     const uint32_t w = v.h_word[gi];
                     if ((w & 3u) == 0 && !((w >> (4 + s)) & 1u)) {
                         v.h_word[gi] = (w & ~0xFu) | (1u << (4 + s)) | (s << 2) | 1u;
-                        if (v.hist) v.hist[4 * gi + s] = (int32_t)t - v.h_birth[gi];
+                        if (HIST) v.hist[4 * gi + s] = (int32_t)t - v.h_birth[gi];
                         const U4 r2 = draw(key, cd.h_base + h, t, P_H_IIP, 0);
                         v.h_next[gi] = (uint16_t)(t + sd.iip_min + mulhi(r2.x, sd.iip_span));
                         infected = true;
@@ -1186,21 +1193,27 @@ hipError_t launch_init(const EngineView& v, hipStream_t st) {
 }
 
 // Demography variants: (UNROLL, EARLY_M) pairs of kDemographyShapes.
-template <bool XCD>
-static hipError_t launch_demography_x(const EngineView& v, const Tuning& tu, hipStream_t st) {
+template <bool XCD, bool HIST>
+static hipError_t launch_demography_xh(const EngineView& v, const Tuning& tu, hipStream_t st) {
     const dim3 grid(v.n_comm), block(kWave);
     const size_t lds = (size_t)v.death_m_lds * 4;
     if (tu.unroll == 4 && tu.early_m == 4)
-        hipLaunchKernelGGL((k_day_demography<4, 4, XCD>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<4, 4, XCD, HIST>), grid, block, lds, st, v);
     else if (tu.unroll == 4 && tu.early_m == 2)
-        hipLaunchKernelGGL((k_day_demography<4, 2, XCD>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<4, 2, XCD, HIST>), grid, block, lds, st, v);
     else if (tu.unroll == 8 && tu.early_m == 4)
-        hipLaunchKernelGGL((k_day_demography<8, 4, XCD>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<8, 4, XCD, HIST>), grid, block, lds, st, v);
     else if (tu.unroll == 2 && tu.early_m == 2)
-        hipLaunchKernelGGL((k_day_demography<2, 2, XCD>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<2, 2, XCD, HIST>), grid, block, lds, st, v);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+template <bool XCD>
+static hipError_t launch_demography_x(const EngineView& v, const Tuning& tu, hipStream_t st) {
+    return (v.hist || v.spatial) ? launch_demography_xh<XCD, true>(v, tu, st)
+                                 : launch_demography_xh<XCD, false>(v, tu, st);
 }
 
 bool tuning_supported(const Tuning& tu) {
@@ -1214,8 +1227,14 @@ hipError_t launch_demography(const EngineView& v, const Tuning& tu, hipStream_t 
 }
 
 hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st) {
-    if (tu.xcd & 2u) hipLaunchKernelGGL(k_day_events<true>, dim3(v.n_comm), dim3(kWave), 0, st, v);
-    else hipLaunchKernelGGL(k_day_events<false>, dim3(v.n_comm), dim3(kWave), 0, st, v);
+    const dim3 grid(v.n_comm), block(kWave);
+    if (tu.xcd & 2u) {
+        if (v.hist) hipLaunchKernelGGL((k_day_events<true, true>), grid, block, 0, st, v);
+        else hipLaunchKernelGGL((k_day_events<true, false>), grid, block, 0, st, v);
+    } else {
+        if (v.hist) hipLaunchKernelGGL((k_day_events<false, true>), grid, block, 0, st, v);
+        else hipLaunchKernelGGL((k_day_events<false, false>), grid, block, 0, st, v);
+    }
     if (v.aoi_mark) hipLaunchKernelGGL(k_day_aoi, dim3(v.n_sims), dim3(256), 0, st, v);
     return hipGetLastError();
 }
