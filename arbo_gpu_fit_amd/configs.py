@@ -240,6 +240,14 @@ def mcmc_priors():
             (2, 10.0, 90.0)]
 
 
+def climate_priors():
+    """The reference's fitted set (MCMC.h:37-50): TruncatedNormal priors of
+    scaleEIP (2.5, 0.5), scaleMLE (3, 0.5), scaleCC (1, 0.4), scaleHumidity
+    (1, 0.4), minMosToHuman (3, 1) and Beta(10, 90) for the observation
+    (reporting) rate; kind 3 is the normal truncated to (0, 10000]."""
+    return [(3, 2.5, 0.5), (3, 3.0, 0.5), (3, 1.0, 0.4), (3, 1.0, 0.4), (3, 3.0, 1.0), (2, 10.0, 90.0)]
+
+
 def mcmc_data(engine, theta=MCMC_TRUTH, n_weeks: int = MCMC_WEEKS, seed: int = 3, obs_seed: int = 7,
               total_humans: int = 1_000_000, communities: int = 1000, nb_size: float = MCMC_NB_SIZE):
     """Synthetic weekly counts: one simulation of the model at `theta` (Philox

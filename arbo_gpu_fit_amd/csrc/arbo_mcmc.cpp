@@ -34,6 +34,7 @@ namespace {
 
 constexpr uint32_t D = ARBO_MCMC_PARAMS;
 constexpr double kTwoPi = 6.283185307179586;
+constexpr double kTruncHi = 10000.0;   // upper end of the truncated-normal priors (MCMC.h:37-46)
 
 // Philox purposes of the driver (key: the driver's seed).
 enum : uint32_t { Q_MIX = 1, Q_NORMAL = 2, Q_ACCEPT = 3, Q_SIM_SEED = 4 };
@@ -63,6 +64,13 @@ double prior_logpdf(const ArboPrior& p, double x) {
             if (!(x > 0.0 && x < 1.0)) return -INFINITY;
             return (p.a - 1.0) * std::log(x) + (p.b - 1.0) * std::log1p(-x) -
                    (std::lgamma(p.a) + std::lgamma(p.b) - std::lgamma(p.a + p.b));
+        case ARBO_PRIOR_TRUNC_NORMAL: {   // N(a, b^2) on (0, kTruncHi], normalised
+            if (!(x > 0.0 && x <= kTruncHi)) return -INFINITY;
+            const double z = (x - p.a) / p.b;
+            const double mass = 0.5 * (std::erfc(-(kTruncHi - p.a) / (p.b * std::sqrt(2.0))) -
+                                       std::erfc(p.a / (p.b * std::sqrt(2.0))));
+            return -0.5 * z * z - std::log(p.b) - 0.5 * std::log(2.0 * M_PI) - std::log(mass);
+        }
     }
     return -INFINITY;
 }
@@ -70,8 +78,16 @@ double prior_logpdf(const ArboPrior& p, double x) {
 bool valid_prior(const ArboPrior& p) {
     if (!(std::isfinite(p.a) && std::isfinite(p.b))) return false;
     if (p.kind == ARBO_PRIOR_BETA) return p.a > 0.0 && p.b > 0.0;
+    if (p.kind == ARBO_PRIOR_TRUNC_NORMAL) return p.b > 0.0 && p.a > -10.0 * p.b && p.a < kTruncHi;
     if (p.kind == ARBO_PRIOR_UNIFORM || p.kind == ARBO_PRIOR_CIRCULAR) return p.b > p.a;
     return false;
+}
+
+// Inputs the parameters 0..4 take: 1 each, 2 for a circular one.
+uint32_t n_inputs(const ArboPrior* priors) {
+    uint32_t n = 0;
+    for (uint32_t i = 0; i + 1 < D; ++i) n += priors[i].kind == ARBO_PRIOR_CIRCULAR ? 2u : 1u;
+    return n;
 }
 
 // Cholesky factor (lower) of a d x d symmetric positive-definite matrix;
@@ -240,14 +256,20 @@ int eval_engine(ArboMcmc* m, ArboEngine* eng, const ArboEngineApi& api, uint32_t
     std::vector<ArboShape> shapes(sims);
     std::vector<ArboParams> params(sims);
     std::vector<uint64_t> seeds(sims);
-    for (uint32_t i = 0; i < n; ++i)
+    const bool fit_climate = m->desc.param_set == ARBO_FIT_CLIMATE;
+    std::vector<ArboClimate> climates(fit_climate ? n : 0);   // each vector's climate coefficients
+    for (uint32_t i = 0; i < n; ++i) {
+        if (fit_climate) arbo_mcmc_sim_climate(th + (size_t)i * D, &m->climate, &climates[i]);
+        const ArboClimate* cl = fit_climate ? &climates[i] : (m->has_climate ? &m->climate : nullptr);
         for (uint32_t p = 0; p < P; ++p) {
             const uint32_t k = i * P + p;
             shapes[k] = ArboShape{m->grid_w, m->grid_h, m->humans.data(), m->has_slots ? m->slots.data() : nullptr,
-                                  m->has_climate ? &m->climate : nullptr, m->share.empty() ? nullptr : m->share.data()};
-            arbo_mcmc_sim_params(th + (size_t)i * D, &m->base, &params[k]);
+                                  cl, m->share.empty() ? nullptr : m->share.data()};
+            if (fit_climate) params[k] = m->base;
+            else arbo_mcmc_sim_params(th + (size_t)i * D, &m->base, &params[k]);
             seeds[k] = sim_seed(m, chains[i], it, p);
         }
+    }
     int rc = api.batch_init(eng, sims, shapes.data(), params.data(), seeds.data(), T);
     if (rc == ARBO_OK) rc = api.batch_step(eng, T);
     if (rc != ARBO_OK) return fail(m, rc, std::string("engine: ") + (api.last_error ? api.last_error(eng) : ""));
@@ -365,6 +387,17 @@ ARBO_API int arbo_mcmc_sim_params(const double* th, const ArboParams* base, Arbo
     return ARBO_OK;
 }
 
+ARBO_API int arbo_mcmc_sim_climate(const double* th, const ArboClimate* base, ArboClimate* out) {
+    if (!th || !base || !out) return ARBO_E_INVALID;
+    *out = *base;
+    out->scale_eip = (float)th[0];
+    out->scale_mle = (float)th[1];
+    out->scale_cc = (float)th[2];
+    out->scale_humidity = (float)th[3];
+    out->min_mos_to_human = (float)th[4];
+    return ARBO_OK;
+}
+
 ARBO_API int arbo_mcmc_create(const ArboMcmcDesc* desc, const ArboShape* shape, const ArboParams* base,
                               const uint32_t* data, const double* theta0, ArboMcmc** out) {
     if (!out) return ARBO_E_INVALID;
@@ -378,6 +411,9 @@ ARBO_API int arbo_mcmc_create(const ArboMcmcDesc* desc, const ArboShape* shape, 
     if ((uint64_t)desc->n_chains * desc->n_particles > (1u << 20)) return ARBO_E_INVALID;
     for (uint32_t i = 0; i < D; ++i)
         if (!valid_prior(desc->priors[i])) return ARBO_E_INVALID;
+    if (n_inputs(desc->priors) > D || desc->priors[D - 1].kind == ARBO_PRIOR_CIRCULAR) return ARBO_E_INVALID;
+    if (desc->param_set > ARBO_FIT_CLIMATE || (desc->param_set == ARBO_FIT_CLIMATE && !shape->climate))
+        return ARBO_E_INVALID;
     ArboMcmc* m = new (std::nothrow) ArboMcmc();
     if (!m) return ARBO_E_ALLOC;
     m->desc = *desc;
@@ -507,18 +543,41 @@ namespace {
 
 enum : uint32_t { Q_MALA_NORMAL = 5, Q_MALA_ACCEPT = 6, Q_SCREEN = 7 };
 
-void features_of(const ArboMcmc* m, const double* th, float* x) {
-    for (uint32_t i = 0; i < 4; ++i) {
-        const ArboPrior& p = m->desc.priors[i];
-        x[i] = (float)((th[i] - 0.5 * (p.a + p.b)) / (0.5 * (p.b - p.a)));
+// Centre and scale of a parameter's surrogate input: (theta - c) / s.
+void input_scale(const ArboPrior& p, double& c, double& s) {
+    switch (p.kind) {
+        case ARBO_PRIOR_TRUNC_NORMAL: c = p.a; s = p.b; return;
+        case ARBO_PRIOR_BETA:
+            c = p.a / (p.a + p.b);
+            s = std::sqrt(p.a * p.b / ((p.a + p.b) * (p.a + p.b) * (p.a + p.b + 1.0)));
+            return;
+        default: c = 0.5 * (p.a + p.b); s = 0.5 * (p.b - p.a); return;
     }
-    x[4] = (float)std::cos(th[4]);
-    x[5] = (float)std::sin(th[4]);
 }
+
+// Surrogate inputs: parameters 0..4 by their priors (a circular one as cos,
+// sin), zeros after; at most six inputs (checked at creation).
+void features_of(const ArboMcmc* m, const double* th, float* x) {
+    uint32_t j = 0;
+    for (uint32_t i = 0; i + 1 < D; ++i) {
+        const ArboPrior& p = m->desc.priors[i];
+        if (p.kind == ARBO_PRIOR_CIRCULAR) {
+            x[j++] = (float)std::cos(th[i]);
+            x[j++] = (float)std::sin(th[i]);
+        } else {
+            double c, s;
+            input_scale(p, c, s);
+            x[j++] = (float)((th[i] - c) / s);
+        }
+    }
+    while (j < D) x[j++] = 0.0f;
+}
+
 
 // log prior gradient (zero for the uniform and circular priors)
 double prior_grad(const ArboPrior& p, double x) {
     if (p.kind == ARBO_PRIOR_BETA) return (p.a - 1.0) / x - (p.b - 1.0) / (1.0 - x);
+    if (p.kind == ARBO_PRIOR_TRUNC_NORMAL) return -(x - p.a) / (p.b * p.b);
     return 0.0;
 }
 
@@ -557,11 +616,19 @@ int sur_logpost(ArboMcmc* m, ArboSurrogate* sur, uint32_t n, const double* th, d
             const double* t = th + (size_t)i * D;
             double* g = grad + (size_t)i * D;
             const float* d = &dx[(size_t)i * D];
-            for (uint32_t k = 0; k < 4; ++k) {
+            uint32_t j = 0;   // the inputs of features_of, in order
+            for (uint32_t k = 0; k + 1 < D; ++k) {
                 const ArboPrior& p = m->desc.priors[k];
-                g[k] = (double)d[k] / (0.5 * (p.b - p.a)) + prior_grad(p, t[k]);
+                if (p.kind == ARBO_PRIOR_CIRCULAR) {
+                    g[k] = -std::sin(t[k]) * (double)d[j] + std::cos(t[k]) * (double)d[j + 1];
+                    j += 2;
+                } else {
+                    double c, s;
+                    input_scale(p, c, s);
+                    g[k] = (double)d[j++] / s;
+                }
+                g[k] += prior_grad(p, t[k]);
             }
-            g[4] = -std::sin(t[4]) * (double)d[4] + std::cos(t[4]) * (double)d[5] + prior_grad(m->desc.priors[4], t[4]);
             g[5] = drho[i] + prior_grad(m->desc.priors[5], t[5]);
         }
     }

@@ -31,7 +31,17 @@ def prior_sample(n: int, seed: int, priors=None) -> np.ndarray:
     rng = np.random.Generator(np.random.PCG64(seed))
     th = np.zeros((n, mcmc.D))
     for i, (k, a, b) in enumerate(priors):
-        th[:, i] = rng.beta(a, b, n) if k == mcmc.PRIOR_BETA else rng.uniform(a, b, n)
+        if k == mcmc.PRIOR_BETA:
+            th[:, i] = rng.beta(a, b, n)
+        elif k == mcmc.PRIOR_TRUNC_NORMAL:   # N(a, b^2) on (0, TRUNC_HI] by rejection
+            v = rng.normal(a, b, 4 * n + 64)
+            v = v[(v > 0.0) & (v <= mcmc.TRUNC_HI)]
+            while v.size < n:
+                w = rng.normal(a, b, 4 * n + 64)
+                v = np.concatenate([v, w[(w > 0.0) & (w <= mcmc.TRUNC_HI)]])
+            th[:, i] = v[:n]
+        else:
+            th[:, i] = rng.uniform(a, b, n)
     return th
 
 

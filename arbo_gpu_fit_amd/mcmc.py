@@ -16,7 +16,10 @@ from .engine import ArboError, Engine
 
 D = 6
 PARAM_NAMES = ("beta_hm", "beta_mh", "log10_long_distance", "season_amp", "season_phase", "reporting")
-PRIOR_UNIFORM, PRIOR_CIRCULAR, PRIOR_BETA = 0, 1, 2
+CLIMATE_PARAM_NAMES = ("scale_eip", "scale_mle", "scale_cc", "scale_humidity", "min_mos_to_human", "reporting")
+PRIOR_UNIFORM, PRIOR_CIRCULAR, PRIOR_BETA, PRIOR_TRUNC_NORMAL = 0, 1, 2, 3
+TRUNC_HI = 10000.0   # upper end of the truncated-normal priors
+FIT_SEASONAL, FIT_CLIMATE = 0, 1
 
 _u32 = C.c_uint32
 _f64p = C.POINTER(C.c_double)
@@ -31,7 +34,7 @@ class Prior(C.Structure):
 class McmcDesc(C.Structure):
     _fields_ = [
         ("abi_version", _u32), ("n_chains", _u32), ("n_particles", _u32), ("n_weeks", _u32),
-        ("skip_weeks", _u32), ("adapt_after", _u32),
+        ("skip_weeks", _u32), ("adapt_after", _u32), ("param_set", _u32), ("reserved0", _u32),
         ("nb_size", C.c_double), ("week_floor", C.c_double), ("mix_beta", C.c_double), ("init_scale", C.c_double),
         ("seed", C.c_uint64), ("priors", Prior * D),
     ]
@@ -68,6 +71,7 @@ PROTOTYPES = {
     "arbo_mcmc_read": (C.c_int, [McmcPtr, _u32, _u32, _f64p, _f64p, C.POINTER(C.c_uint8)]),
     "arbo_mcmc_counters": (C.c_int, [McmcPtr, _u64p, _u64p, _u64p, _u64p, _f64p]),
     "arbo_mcmc_sim_params": (C.c_int, [_f64p, C.POINTER(abi.Params), C.POINTER(abi.Params)]),
+    "arbo_mcmc_sim_climate": (C.c_int, [_f64p, C.POINTER(abi.Climate), C.POINTER(abi.Climate)]),
     "arbo_mcmc_log_prior": (C.c_int, [McmcPtr, _f64p, _f64p]),
     "arbo_mcmc_destroy": (None, [McmcPtr]),
     "arbo_mcmc_last_error": (C.c_char_p, [McmcPtr]),
@@ -129,6 +133,7 @@ class McmcConfig:
     mix_beta: float = 0.05
     init_scale: float = 0.1
     seed: int = 1
+    param_set: int = FIT_SEASONAL
 
     def desc(self) -> McmcDesc:
         d = McmcDesc()
@@ -138,6 +143,7 @@ class McmcConfig:
         d.n_weeks = self.n_weeks
         d.skip_weeks = self.skip_weeks
         d.adapt_after = self.adapt_after
+        d.param_set = self.param_set
         d.nb_size = self.nb_size
         d.week_floor = self.week_floor
         d.mix_beta = self.mix_beta
@@ -152,13 +158,21 @@ class Mcmc:
     """Batched adaptive-Metropolis chains (include/arbo_mcmc.h)."""
 
     def __init__(self, cfg: McmcConfig, grid_w: int, grid_h: int, humans, base: abi.Params, data,
-                 theta0=None, lib: C.CDLL | None = None):
+                 theta0=None, lib: C.CDLL | None = None, climate=None):
+        """climate: an engine.ClimateSpec every simulation runs under (its
+        coefficients are the parameters of the FIT_CLIMATE set); the driver
+        copies it."""
         self.lib = bind(lib if lib is not None else abi.load_hip_library())
         self.cfg = cfg
         self._humans = np.ascontiguousarray(humans, dtype=np.uint32)
         self._data = np.ascontiguousarray(data, dtype=np.uint32)
         assert self._data.size == cfg.n_weeks
         shape = abi.Shape(grid_w, grid_h, self._humans.ctypes.data_as(_u32p), None)
+        keep: list = []
+        if climate is not None:
+            cl = climate.struct(keep)
+            keep.append(cl)
+            shape.climate = C.pointer(cl)
         self.base = base
         t0 = None
         if theta0 is not None:
@@ -293,4 +307,15 @@ def sim_params(lib, theta, base: abi.Params) -> abi.Params:
     rc = bind(lib).arbo_mcmc_sim_params(t.ctypes.data_as(_f64p), C.byref(base), C.byref(out))
     if rc:
         raise ArboError(rc, "arbo_mcmc_sim_params")
+    return out
+
+
+def sim_climate(lib, theta, base) -> abi.Climate:
+    """The climate of one FIT_CLIMATE vector: base (abi.Climate) with its
+    five coefficients set."""
+    out = abi.Climate()
+    t = _f64(theta)
+    rc = bind(lib).arbo_mcmc_sim_climate(t.ctypes.data_as(_f64p), C.byref(base), C.byref(out))
+    if rc:
+        raise ArboError(rc, "arbo_mcmc_sim_climate")
     return out

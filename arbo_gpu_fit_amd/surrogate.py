@@ -132,15 +132,26 @@ class Surrogate:
 
 
 def features(theta, priors) -> np.ndarray:
-    """Surrogate inputs of MCMC parameter vectors [n, 6]: the four linear
-    simulation parameters scaled from their prior range to [-1, 1], the phase
-    as (cos, sin).  The reporting rate is not an input (it scales the
+    """Surrogate inputs of MCMC parameter vectors [n, 6]: parameters 0-4 in
+    order, each by its prior (uniform scaled from its range to [-1, 1],
+    circular as (cos, sin), truncated normal (x - a) / b, Beta standardised),
+    zeros after.  The reporting rate is not an input (it scales the
     surrogate's mean in the likelihood)."""
     th = np.atleast_2d(np.asarray(theta, dtype=np.float64))
     x = np.zeros((th.shape[0], 6))
-    for i in range(4):
-        _, a, b = priors[i]
-        x[:, i] = (th[:, i] - 0.5 * (a + b)) / (0.5 * (b - a))
-    x[:, 4] = np.cos(th[:, 4])
-    x[:, 5] = np.sin(th[:, 4])
+    j = 0
+    for i in range(5):
+        k, a, b = priors[i]
+        if k == 1:      # circular
+            x[:, j], x[:, j + 1] = np.cos(th[:, i]), np.sin(th[:, i])
+            j += 2
+            continue
+        if k == 3:      # truncated normal
+            c, s = a, b
+        elif k == 2:    # Beta
+            c, s = a / (a + b), np.sqrt(a * b / ((a + b) ** 2 * (a + b + 1.0)))
+        else:           # uniform
+            c, s = 0.5 * (a + b), 0.5 * (b - a)
+        x[:, j] = (th[:, i] - c) / s
+        j += 1
     return x

@@ -14,11 +14,17 @@
  * iteration are simulated as one engine batch (include/arbo_c.h), so a step
  * of all chains costs one pass of the day loop over the batch.
  *
- * Parameter vector (ARBO_MCMC_PARAMS = 6), in this order:
- *   0 beta_hm, 1 beta_mh, 2 log10(long-distance fraction), 3 seasonal
- *   amplitude, 4 seasonal phase (circular), 5 reporting rate.
- * Each is mapped onto ArboParams of a base parameter set; the reporting rate
- * scales the simulated weekly incidence in the likelihood.
+ * Parameter vector (ARBO_MCMC_PARAMS = 6), by ArboMcmcDesc.param_set:
+ *   ARBO_FIT_SEASONAL: 0 beta_hm, 1 beta_mh, 2 log10(long-distance
+ *     fraction), 3 seasonal amplitude, 4 seasonal phase (circular), 5
+ *     reporting rate; 0-4 are set in ArboParams of the base parameter set.
+ *   ARBO_FIT_CLIMATE (the reference's fitted set, MCMC.h:37-50 and
+ *     MCMC.cpp:79-86): 0 scale_eip, 1 scale_mle, 2 scale_cc, 3
+ *     scale_humidity, 4 min_mos_to_human, 5 reporting rate; 0-4 are set in
+ *     the ArboClimate of the shape (which must have one).  The reference's
+ *     seventh parameter, the NB dispersion, is nb_size here.
+ * The reporting rate scales the simulated weekly incidence in the
+ * likelihood.
  *
  * Conventions as include/arbo_c.h: 0 or a negative ARBO_E_* code, message in
  * arbo_mcmc_last_error(); one driver per thread.
@@ -37,10 +43,14 @@ extern "C" {
 #define ARBO_MCMC_PARAMS 6u
 #define ARBO_MCMC_MAX_CHAINS 4096u
 
+#define ARBO_FIT_SEASONAL 0u
+#define ARBO_FIT_CLIMATE 1u
+
 enum {
     ARBO_PRIOR_UNIFORM = 0,   /* U[a, b] */
     ARBO_PRIOR_CIRCULAR = 1,  /* U[a, b) on a circle: proposals wrap into [a, b) */
-    ARBO_PRIOR_BETA = 2       /* Beta(a, b) on (0, 1) */
+    ARBO_PRIOR_BETA = 2,      /* Beta(a, b) on (0, 1) */
+    ARBO_PRIOR_TRUNC_NORMAL = 3  /* N(a, b^2) truncated to (0, 10000] (TruncatedNormal, MCMC.h:37-46) */
 };
 
 typedef struct ArboPrior {
@@ -56,6 +66,8 @@ typedef struct ArboMcmcDesc {
     uint32_t n_weeks;         /* weekly counts; each simulation runs 7 n_weeks - 1 days (main.cpp:34) */
     uint32_t skip_weeks;      /* leading weeks left out of the likelihood (MCMC.cpp:238 skips one) */
     uint32_t adapt_after;     /* pooled chain samples before the adapted covariance is used */
+    uint32_t param_set;       /* ARBO_FIT_SEASONAL or ARBO_FIT_CLIMATE */
+    uint32_t reserved0;       /* 0 */
     double nb_size;           /* negative-binomial size r: variance mu + mu^2 / r */
     double week_floor;        /* lower clamp of each week's log-likelihood (MCMC.cpp:243: -250) */
     double mix_beta;          /* probability of the fixed proposal (MCMC.cpp:127: 0.05) */
@@ -133,8 +145,10 @@ ARBO_API int arbo_mcmc_run_da(ArboMcmc* m, ArboEngine* engine, const ArboEngineA
 ARBO_API int arbo_mcmc_run_da_target(ArboMcmc* m, ArboLogLikFn fn, void* ctx, ArboSurrogate* sur,
                                      const ArboDaDesc* da, uint32_t n_iters);
 
-/* Surrogate inputs of n vectors (x[n * 6]): parameters 0-3 scaled from their
- * prior range to [-1, 1], the phase as (cos, sin). */
+/* Surrogate inputs of n vectors (x[n * 6]): parameters 0-4 in order, each
+ * by its prior: uniform scaled from its range to [-1, 1], circular as (cos,
+ * sin), truncated normal as (x - a) / b, Beta standardised by its mean and
+ * sd; unused inputs 0.  The reporting rate (5) is no input. */
 ARBO_API int arbo_mcmc_features(const ArboMcmc* m, uint32_t n, const double* theta, float* x);
 
 /* Surrogate log-posterior and its gradient for n vectors (out[n],
@@ -160,6 +174,10 @@ ARBO_API int arbo_mcmc_read(const ArboMcmc* m, uint32_t first, uint32_t count, d
  * run, proposals accepted, seconds spent in likelihood evaluation. */
 ARBO_API int arbo_mcmc_counters(const ArboMcmc* m, uint64_t* proposals, uint64_t* in_support,
                                 uint64_t* simulations, uint64_t* accepted, double* eval_seconds);
+
+/* Climate of one vector of the ARBO_FIT_CLIMATE set: base with its five
+ * coefficients set (MCMC::copyParams, MCMC.cpp:79-86). */
+ARBO_API int arbo_mcmc_sim_climate(const double* theta, const ArboClimate* base, ArboClimate* out);
 
 /* Simulation parameters of one vector: base with the vector's fields set. */
 ARBO_API int arbo_mcmc_sim_params(const double* theta, const ArboParams* base, ArboParams* out);

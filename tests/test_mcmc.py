@@ -4,6 +4,7 @@ Metropolis core on targets with known answers (a correlated Gaussian; the
 prior itself under a flat likelihood, which pins the circular wrap and the
 Beta density), and the full driver over the CPU oracle's engine on a tiny
 model (determinism, counters, support)."""
+import ctypes as C
 import math
 
 import numpy as np
@@ -182,3 +183,76 @@ def test_features_match_python(lib):
     th = np.array([[0.3, 0.1, -2.5, 0.8, 5.0, 0.1], [0.05, 0.6, -1.0, 0.0, 0.0, 0.5]])
     assert np.allclose(m.features(th), features(th, configs.mcmc_priors()), atol=1e-6)
     m.close()
+
+
+def test_truncated_normal_priors_sampled_under_flat_likelihood(lib):
+    """The reference's fitted set (MCMC.h:37-50): TruncatedNormal priors on
+    (0, 10000] and Beta(10, 90); the chains sample them (KS) and the log
+    prior equals scipy's truncnorm + beta."""
+    priors = configs.climate_priors()
+    cfg = mcmc.McmcConfig(n_chains=64, n_weeks=3, priors=priors, adapt_after=300, seed=6,
+                          param_set=mcmc.FIT_SEASONAL)
+    m = _dummy_model(cfg, lib, theta0=np.tile([2.5, 3.0, 1.0, 1.0, 3.0, 0.1], (64, 1)))
+    m.run_target(lambda th: np.zeros(len(th)), 4000)
+    th, _, _ = m.history()
+    s = th[400::25].reshape(-1, 6)   # thinned past the chains' autocorrelation
+    for i in range(5):
+        _, a, b = priors[i]
+        dist = stats.truncnorm((0.0 - a) / b, (mcmc.TRUNC_HI - a) / b, loc=a, scale=b)
+        assert s[:, i].min() > 0.0
+        assert stats.kstest(s[:, i], dist.cdf).pvalue > 1e-3, i
+    x = np.array([2.2, 3.4, 0.3, 1.9, 0.5, 0.12])
+    ref = sum(stats.truncnorm((0.0 - a) / b, (mcmc.TRUNC_HI - a) / b, loc=a, scale=b).logpdf(x[i])
+              for i, (_, a, b) in enumerate(priors[:5])) + stats.beta(10, 90).logpdf(0.12)
+    assert np.isclose(m.log_prior(x), ref, rtol=1e-10)
+    assert m.log_prior(np.array([2.2, 3.4, -0.1, 1.9, 0.5, 0.12])) == -np.inf
+    m.close()
+
+
+def test_climate_fit_sends_each_chain_its_climate(lib, oracle_lib):
+    """ARBO_FIT_CLIMATE (MCMC::copyParams, MCMC.cpp:79-86): every simulation
+    of a proposal runs under the base climate with the vector's five
+    coefficients; the other parameters stay the base's.  A recording
+    batch_init in front of the oracle engine sees exactly the chains'
+    vectors, two particles each."""
+    W = 6
+    days = 7 * W - 1
+    cl = configs.seasonal_climate(days + 1)
+    base = configs.base_params(seed_frac_h=0.02, seed_frac_m=0.02)
+    humans = np.full(4, 150, np.uint32)
+    seen = []
+    init_t = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_uint32, C.POINTER(abi.Shape), C.POINTER(abi.Params),
+                         C.POINTER(C.c_uint64), C.c_uint32)
+
+    def recording_init(e, n, shapes, params, seeds, max_days):
+        for k in range(n):
+            c = shapes[k].climate.contents
+            seen.append((c.scale_eip, c.scale_mle, c.scale_cc, c.scale_humidity, c.min_mos_to_human,
+                         c.days, params[k].beta_hm))
+        return oracle_lib.arbo_batch_init(e, n, shapes, params, seeds, max_days)
+
+    cb = init_t(recording_init)
+    api = mcmc.engine_api(oracle_lib)
+    api.batch_init = C.cast(cb, C.c_void_p).value
+    orc = Engine(oracle_lib, threads=2)
+    cfg = mcmc.McmcConfig(n_chains=4, n_weeks=W, priors=configs.climate_priors(), n_particles=2, adapt_after=50,
+                          seed=3, param_set=mcmc.FIT_CLIMATE)
+    m = mcmc.Mcmc(cfg, 2, 2, humans, base, np.full(W, 20, np.uint32), lib=lib, climate=cl)
+    m.run(orc, 2, api=api)
+    th, ll, _ = m.history()
+    assert np.all(np.isfinite(ll))
+    f32 = np.float32
+    starts = seen[:8]   # the starting vectors, two particles each
+    cur = th[0]
+    for k, rec in enumerate(starts):
+        assert rec[5] == cl.temperature.size and rec[6] == f32(base.beta_hm)
+    got = {tuple(r[:5]) for r in seen}
+    for it in range(2):
+        for c in range(4):
+            v = tuple(float(f32(x)) for x in th[it, c, :5])
+            assert v in got, (it, c, v)
+    assert cur.shape == (4, 6)
+    m.close()
+    orc.close()
+    with pytest.raises(ArboError):   # the climate set needs a climate
+        mcmc.Mcmc(cfg, 2, 2, humans, base, np.full(W, 20, np.uint32), lib=lib)
