@@ -7,6 +7,7 @@ import pytest
 
 from arbo_gpu_fit_amd.engine import Engine
 from tests.cases import CASES, CHECK_DAYS
+from tests.random_cases import N_RANDOM, random_case
 
 pytestmark = pytest.mark.gpu
 
@@ -189,5 +190,15 @@ def test_aoi_rings_match_oracle(name, hip_lib, oracle_lib):
         recorded += int((a != 0).sum())
         assert np.array_equal(gpu.history(i), orc.history(i))
     assert recorded > 0
+    gpu.close()
+    orc.close()
+
+
+@pytest.mark.parametrize("k", range(N_RANDOM))
+def test_random_parity(k, hip_lib, oracle_lib):
+    """Seeded random combinations of every model switch (tests/random_cases.py)."""
+    specs, T = random_case(k)
+    print(f"[parity] random case {k}: {len(specs)} sims, {T} days", flush=True)
+    gpu, orc = _run_pair(specs, T, hip_lib, oracle_lib)
     gpu.close()
     orc.close()

@@ -24,7 +24,7 @@ def _reference(oracle_lib, specs, T):
     return out
 
 
-@pytest.mark.parametrize("case", ["ragged", "batch_pair"])
+@pytest.mark.parametrize("case", ["ragged", "batch_pair", "climate"])
 def test_every_tuning_is_bit_identical(case, hip_lib, oracle_lib):
     specs, T = CASES[case]()
     ref = _reference(oracle_lib, specs, T)

@@ -447,3 +447,20 @@ def test_aoi_rings_skip_introductions(oracle_lib):
     assert (eng.history(0) >= 0).sum() > 100
     assert not eng.aoi(0).any()
     eng.close()
+
+
+def test_random_cases_are_valid_and_run(oracle_lib):
+    """The seeded random simulations of the GPU parity sweep are valid
+    inputs, and the oracle's serial and OpenMP runs of them agree."""
+    from tests.random_cases import N_RANDOM, random_case
+    for k in range(N_RANDOM):
+        specs, T = random_case(k)
+        runs = []
+        for threads in (1, 4):
+            eng = Engine(oracle_lib, threads=threads)
+            eng.init(specs, T)
+            eng.step(T)
+            runs.append([eng.series(i)["n_incidence"].copy() for i in range(len(specs))])
+            eng.close()
+        for a, b in zip(*runs):
+            assert np.array_equal(a, b), k
