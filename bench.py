@@ -2,6 +2,7 @@
 MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--sims 256] [--no-cpu] [--autotune-days D]
+    python bench.py --workload plain     # the single 100x100-lattice simulation instead
 
 Workload: the 256-simulation batch (communities log-uniform 400..10,000 squared
 to a lattice, 200..2,000 humans per community, per-simulation transmission,
@@ -12,6 +13,12 @@ days (W + K = 1,825 by default, the whole configured run).  The first warm-up
 days time every candidate launch tuning for D days each and keep the fastest
 (results do not depend on the tuning).  Inputs are synthetic and resident in
 HBM before timing starts.
+
+With --workload plain: one simulation on the 100x100 lattice (1e7 humans,
+Beta(2,5) initial immunity, 3,650 days), the day loop replayed as hipGraphs
+of 8 days (10,000 communities: one wave each, too few to fill the GPU); the
+roofline comes from a second run of the same days with per-kernel HIP
+events.
 
 The JSON line carries:
   roofline       the demography kernel (the dominant launch of the day) against
@@ -78,10 +85,10 @@ def _oracle_rate(lib, specs, days: int, threads: int):
     return humans * days / dt, humans, dt
 
 
-def cpu_baseline(specs, days: int, threads: int, serial_days: int) -> dict:
+def cpu_baseline(specs, days: int, threads: int, serial_days: int, what: str = "") -> dict:
     """The CPU oracle on a bounded sample: the first simulations of the batch
-    for the first `days` days on `threads` OpenMP threads, and simulation 0
-    alone on one thread for `serial_days` days."""
+    (or `what`) for the first `days` days on `threads` OpenMP threads, and
+    simulation 0 alone on one thread for `serial_days` days."""
     from tests.oracle_lib import load_oracle
 
     lib = load_oracle()
@@ -92,8 +99,10 @@ def cpu_baseline(specs, days: int, threads: int, serial_days: int) -> dict:
         "unit": "person-days/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"CPU oracle (g++ -O2, OpenMP {threads} threads): simulations 0..{len(specs) - 1} of the "
-                  f"batch ({humans} humans), days 0..{days - 1}, {dt:.1f} s",
+        "sample": f"CPU oracle (g++ -O2, OpenMP {threads} threads): "
+                  + (f"{what} ({humans} humans)" if what else
+                     f"simulations 0..{len(specs) - 1} of the batch ({humans} humans)")
+                  + f", days 0..{days - 1}, {dt:.1f} s",
         "serial": {
             "value": srate,
             "cores": 1,
@@ -192,8 +201,9 @@ def mcmc_bench(args) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1800)
-    ap.add_argument("--warmup", type=int, default=25)
+    ap.add_argument("--steps", type=int, default=None, help="timed days (batch 1,800; plain 3,600)")
+    ap.add_argument("--warmup", type=int, default=None, help="warm-up days (batch 25; plain 50)")
+    ap.add_argument("--workload", choices=("batch", "plain"), default="batch")
     ap.add_argument("--sims", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sims", type=int, default=8)
@@ -236,13 +246,20 @@ def main():
     if args.mcmc:
         print(json.dumps(mcmc_bench(args)), flush=True)
         return
+    plain = args.workload == "plain"
+    if args.steps is None:
+        args.steps = 3600 if plain else 1800
+    if args.warmup is None:
+        args.warmup = 50 if plain else 25
+    if plain:
+        args.graph = True
 
     from arbo_gpu_fit_amd import abi, configs
     from arbo_gpu_fit_amd.engine import Engine
 
     lib = abi.load_hip_library()
     t0 = time.perf_counter()
-    specs, days_cfg = configs.batch256(args.sims)
+    specs, days_cfg = configs.lattice100() if plain else configs.batch256(args.sims)
     humans = sum(int(s.humans.sum()) for s in specs)
     comms = sum(s.communities for s in specs)
     sero_comms = sum(s.communities * s.params.n_serotypes for s in specs)
@@ -303,9 +320,11 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": f"batched plain simulation: {len(specs)} simulations, {comms} communities "
-                        f"(400-10,000 per simulation), {humans} humans (200-2,000 per community), "
-                        f"{slots} mosquito slots, {days} days",
+            "workload": (f"plain single simulation: 100x100 lattice, {humans} humans (1,000 per community), "
+                         f"{slots} mosquito slots, Beta(2,5) initial immunity, {days} days" if plain else
+                         f"batched plain simulation: {len(specs)} simulations, {comms} communities "
+                         f"(400-10,000 per simulation), {humans} humans (200-2,000 per community), "
+                         f"{slots} mosquito slots, {days} days"),
             "simulations": len(specs),
             "communities": comms,
             "humans": humans,
@@ -320,8 +339,19 @@ def main():
     step_gbs = step_bytes / (elapsed / args.steps) / 1e9
     out["step_roofline"] = {"bytes_per_step": step_bytes, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS}
-    if not args.graph:
-        traffic = profiled_traffic() if args.sims == 256 else None
+    if plain:   # kernel means from a separate pass with per-kernel HIP events, same launch tuning
+        tun = eng.tuning()
+        eng.close()
+        eng = Engine(lib, device=0, flags=abi.F_TIMING)
+        eng.init(specs, days)
+        eng.set_tuning(**tun)
+        eng.step(args.warmup)
+        eng.sync()
+        eng.reset_timing()
+        eng.step(args.steps)   # the same days as the timed run
+        eng.sync()
+    if not args.graph or plain:
+        traffic = profiled_traffic() if (args.sims == 256 and not plain) else None
         ms_a, n_a = eng.kernel_time(0)
         ms_b, n_b = eng.kernel_time(1)
         bytes_a = 4 * humans + 4 * slots + 32 * sero_comms
@@ -339,13 +369,19 @@ def main():
             "mean_launch_ms": mean_a,
             "traffic_source": PROFILE if traffic is not None else None,
         }
+        if plain:
+            out["roofline"]["pass"] = (f"a second run of the same days ({args.warmup}..{days - 1}) with per-kernel "
+                                       "HIP events")
         out["kernels_ms"] = {"k_day_demography": ms_a, "k_day_events": ms_b, "launches": n_a}
     eng.close()
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] CPU oracle baseline on {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(specs[:args.cpu_sims], args.cpu_days, threads, args.cpu_serial_days)
+        if plain:   # a bounded sample: 730 days on the threads, 60 days on one
+            out["cpu_baseline"] = cpu_baseline(specs, 730, threads, 60, what="the 100x100-lattice simulation")
+        else:
+            out["cpu_baseline"] = cpu_baseline(specs[:args.cpu_sims], args.cpu_days, threads, args.cpu_serial_days)
     print(json.dumps(out), flush=True)
 
 
