@@ -20,6 +20,11 @@ constexpr uint32_t kInboxEdges = 9;       // in-edges per destination (Moore-8 +
 constexpr uint32_t kHumanAlign = 4;       // humans per 16-byte vector
 constexpr uint32_t kMosqAlign = 128;      // mosquito slots per 16-byte group of alive-bitmap words
 constexpr uint32_t kAoiRecord = 5000;     // ages per age-of-infection ring
+// Per-community partials of a day, written by the demography kernel and
+// summed per simulation by k_day_reduce: word 0 mosquitoes alive after the
+// births, 1 humans, 2-3 infectious humans (two 16-bit serotype pairs), 4-5
+// E->I onsets (pairs), 8-23 the eight long-distance rates or counts (u64).
+constexpr uint32_t kPartWords = 24;
 constexpr uint32_t kAoiWindow = 3650;     // rings record the days t with max_days - t <= this
 
 struct CommDesc {
@@ -121,6 +126,7 @@ struct EngineView {
     unsigned long long* lam;       // [2 parity][n_sims][8] long-distance events of the day: the rate in
                                    // 2^-24 units (Poisson counts) or the count (stochastic rounding)
     uint32_t* series;
+    uint32_t* part;                // [G][kPartWords] per-community partials of the day (see kPartWords)
     uint32_t* spatial;             // per-community infectious humans [(T+1)][S][C] per simulation, or null
     int32_t* hist;                 // per human 4 infection-history words (one per serotype), or null
     uint8_t* aoi_mark;             // per human: 1 when a mosquito event infected it today (age-of-infection

@@ -64,6 +64,7 @@ struct ArboEngine {
     DevBuf<uint8_t> aoi_mark;
     DevBuf<uint32_t> aoi, aoi_pos;
     DevBuf<unsigned long long> lam;
+    DevBuf<uint32_t> part;   // per-community partials of the day
     DevBuf<float> dens;
     uint64_t series_total = 0;
     EngineView view{};
@@ -128,7 +129,7 @@ void free_batch(ArboEngine* e) {
     dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->active); dev_free(e->hsum); dev_free(e->series); dev_free(e->spatial); dev_free(e->csr_thr); dev_free(e->csr_inbox);
     dev_free(e->in_edge); dev_free(e->in_slot); dev_free(e->tables); dev_free(e->day_ctr);
     dev_free(e->h_next); dev_free(e->m_word); dev_free(e->m_next); dev_free(e->h_birth); dev_free(e->hist); dev_free(e->aoi_mark); dev_free(e->aoi); dev_free(e->aoi_pos);
-    dev_free(e->lam); dev_free(e->dens);
+    dev_free(e->lam); dev_free(e->dens); dev_free(e->part);
     e->ready = false;
 }
 
@@ -412,7 +413,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         (rc = dev_alloc(e, e->aoi_pos, (e->flags & ARBO_F_AOI) ? (size_t)n_sims * 4 : 0)) || (rc = dev_alloc(e, e->m_word, m_tot)) ||
         (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kInboxEdges * 8)) || (rc = dev_alloc(e, e->nld, comm.size() * 8)) ||
         (rc = dev_alloc(e, e->hsum, comm.size() * 4)) || (rc = dev_alloc(e, e->active, comm.size())) ||
-        (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->series, ser_tot)) || (rc = dev_alloc(e, e->spatial, spa_tot)) ||
+        (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->part, comm.size() * kPartWords)) || (rc = dev_alloc(e, e->series, ser_tot)) || (rc = dev_alloc(e, e->spatial, spa_tot)) ||
         (rc = dev_alloc(e, e->csr_thr, csr_thr.size())) || (rc = dev_alloc(e, e->csr_inbox, csr_inbox.size())) || (rc = dev_alloc(e, e->in_edge, in_edge.size())) ||
         (rc = dev_alloc(e, e->in_slot, in_slot.size())) || (rc = dev_alloc(e, e->tables, table_pool.size())) ||
         (rc = dev_alloc(e, e->dens, dens_pool.size())) || (rc = dev_alloc(e, e->day_ctr, 1))) {
@@ -460,6 +461,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.cnt = e->cnt.p;
     v.nld = e->nld.p;
     v.lam = e->lam.p;
+    v.part = e->part.p;
     v.series = e->series.p;
     v.spatial = (e->flags & ARBO_F_SPATIAL) ? e->spatial.p : nullptr;
     v.hist = (e->flags & (ARBO_F_HISTORY | ARBO_F_AOI)) ? e->hist.p : nullptr;

@@ -670,7 +670,9 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
             }
         }
         s_n[lane] = n;
-        if (fx) atomicAdd(&v.lam[((size_t)(t & 1u) * v.n_sims + cd.sim) * 8 + lane], fx);
+        // summed over the simulation by k_day_reduce (no same-address atomics
+        // across the simulation's communities)
+        reinterpret_cast<unsigned long long*>(v.part + (size_t)g * kPartWords + 8)[lane] = fx;
     }
     __syncthreads();
     // Destinations: event i of source segment k takes word i%4 of Philox
@@ -712,18 +714,15 @@ __global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
         }
     }
 
-    // ---- series row t+1 (datacollect.cu:79-189) ----
+    // ---- series row t+1 (datacollect.cu:79-189): this community's part ----
     const SeriesIdx si = series_index(sd.series_off, v.max_days, S);
     const uint32_t row = t + 1;
-    if (lane == 0) {
-        atomicAdd(&v.series[si.n_count + row], cd.n);
-        atomicAdd(&v.series[si.m_count + row], Mc);
-    }
+    if (lane == 0)
+        reinterpret_cast<uint4*>(v.part + (size_t)g * kPartWords)[0] = make_uint4(Mc, cd.n, ih01, ih23);
+    else if (lane == 1)
+        reinterpret_cast<uint2*>(v.part + (size_t)g * kPartWords + 4)[0] = make_uint2(inc01, inc23);
     if (lane < S) {
         const uint32_t ih = unpack_pair(ih01, ih23, lane);
-        const uint32_t ic = unpack_pair(inc01, inc23, lane);
-        if (ih) atomicAdd(&v.series[si.n_inf + (size_t)row * S + lane], ih);
-        if (ic) atomicAdd(&v.series[si.n_inc + (size_t)row * S + lane], ic);
         if (cd.c == 0) v.series[si.n_one + (size_t)row * S + lane] = ih;
         if (HIST && v.spatial) v.spatial[sd.spatial_off + ((uint64_t)row * S + lane) * sd.C + cd.c] = ih;
     }
@@ -1062,8 +1061,55 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t spl
         atomicAdd(&v.nld[(size_t)(sd.comm_off + d) * 8 + k], 1u);
         v.active[sd.comm_off + d] = 1u;
     }
-    // the rate sums of the next day start at zero
-    if (part == 0 && tid < 8) v.lam[((size_t)((t + 1) & 1u) * v.n_sims + sim) * 8 + tid] = 0ull;
+}
+
+// Sums the day's per-community partials of one simulation (one 256-thread
+// workgroup each): the series row t + 1 (datacollect.cu:79-189) and the
+// long-distance totals the long-distance kernel reads.  Integer sums, so the
+// order does not matter.
+__global__ __launch_bounds__(256) void k_day_reduce(EngineView v) {
+    const uint32_t sim = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+    const uint32_t t = *v.day;
+    const SimDesc& sd = v.sims[sim];
+    __shared__ unsigned long long s_fx[8];
+    __shared__ uint32_t s_cnt[4][10];   // per wave: humans, mosquitoes, infectious[4], onsets[4]
+    if (tid < 8) s_fx[tid] = 0ull;
+    __syncthreads();
+    uint32_t nh = 0, nm = 0, inf[4] = {0, 0, 0, 0}, inc[4] = {0, 0, 0, 0};
+    unsigned long long fx[8] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+    for (uint32_t c = tid; c < sd.C; c += 256) {
+        const uint32_t* pw = v.part + (size_t)(sd.comm_off + c) * kPartWords;
+        const uint4 a = reinterpret_cast<const uint4*>(pw)[0];
+        const uint2 b = reinterpret_cast<const uint2*>(pw + 4)[0];
+        nm += a.x;
+        nh += a.y;
+        inf[0] += a.z & 0xFFFFu; inf[1] += a.z >> 16; inf[2] += a.w & 0xFFFFu; inf[3] += a.w >> 16;
+        inc[0] += b.x & 0xFFFFu; inc[1] += b.x >> 16; inc[2] += b.y & 0xFFFFu; inc[3] += b.y >> 16;
+        const unsigned long long* f = reinterpret_cast<const unsigned long long*>(pw + 8);
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) fx[k] += f[k];
+    }
+    const uint32_t cnt[10] = {nh, nm, inf[0], inf[1], inf[2], inf[3], inc[0], inc[1], inc[2], inc[3]};
+#pragma unroll
+    for (uint32_t k = 0; k < 10; ++k) {
+        const uint32_t w = wave_sum(cnt[k]);
+        if (lane == 0) s_cnt[wv][k] = w;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k)
+        if (fx[k]) atomicAdd(&s_fx[k], fx[k]);
+    __syncthreads();
+    const uint32_t S = sd.S, row = t + 1;
+    const SeriesIdx si = series_index(sd.series_off, v.max_days, S);
+    if (tid < 10) {
+        const uint32_t k = tid;
+        const uint32_t x = s_cnt[0][k] + s_cnt[1][k] + s_cnt[2][k] + s_cnt[3][k];
+        if (k == 0) v.series[si.n_count + row] = x;
+        else if (k == 1) v.series[si.m_count + row] = x;
+        else if (k < 6) { if (k - 2 < S) v.series[si.n_inf + (size_t)row * S + (k - 2)] = x; }
+        else if (k - 6 < S) v.series[si.n_inc + (size_t)row * S + (k - 6)] = x;
+    }
+    if (tid >= 32 && tid < 40) v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + (tid - 32)] = s_fx[tid - 32];
 }
 
 // Age-of-infection rings (nepidemic.cu:229-243), one workgroup per
@@ -1240,6 +1286,7 @@ hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st) 
 }
 
 hipError_t launch_longdist(const EngineView& v, const Tuning& tu, hipStream_t st) {
+    hipLaunchKernelGGL(k_day_reduce, dim3(v.n_sims), dim3(256), 0, st, v);
     hipLaunchKernelGGL(k_day_longdist, dim3(v.n_sims * tu.ld_split), dim3(256), 0, st, v, tu.ld_split);
     return hipGetLastError();
 }
