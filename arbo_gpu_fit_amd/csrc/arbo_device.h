@@ -25,6 +25,7 @@ constexpr uint32_t kAoiRecord = 5000;     // ages per age-of-infection ring
 // births, 1 humans, 2-3 infectious humans (two 16-bit serotype pairs), 4-5
 // E->I onsets (pairs), 8-23 the eight long-distance rates or counts (u64).
 constexpr uint32_t kPartWords = 24;
+constexpr uint32_t kReduceSplit = 8;      // workgroups per simulation of k_day_reduce
 constexpr uint32_t kAoiWindow = 3650;     // rings record the days t with max_days - t <= this
 
 struct CommDesc {

@@ -1061,14 +1061,19 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t spl
         atomicAdd(&v.nld[(size_t)(sd.comm_off + d) * 8 + k], 1u);
         v.active[sd.comm_off + d] = 1u;
     }
+    // the long-distance sums of the next day start at zero
+    if (part == 0 && tid < 8) v.lam[((size_t)((t + 1) & 1u) * v.n_sims + sim) * 8 + tid] = 0ull;
 }
 
-// Sums the day's per-community partials of one simulation (one 256-thread
-// workgroup each): the series row t + 1 (datacollect.cu:79-189) and the
-// long-distance totals the long-distance kernel reads.  Integer sums, so the
-// order does not matter.
-__global__ __launch_bounds__(256) void k_day_reduce(EngineView v) {
-    const uint32_t sim = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+// Sums the day's per-community partials of a simulation: `split`
+// workgroups of 256 threads each take a share of its communities and add
+// their sums to the series row t + 1 (datacollect.cu:79-189, zero until
+// then) and to the long-distance totals the long-distance kernel reads
+// (zeroed by the previous day's long-distance kernel), `split` integer
+// atomics per address, so the order does not matter.
+__global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split) {
+    const uint32_t sim = blockIdx.x / split, part = blockIdx.x % split;
+    const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
     const uint32_t t = *v.day;
     const SimDesc& sd = v.sims[sim];
     __shared__ unsigned long long s_fx[8];
@@ -1077,7 +1082,7 @@ __global__ __launch_bounds__(256) void k_day_reduce(EngineView v) {
     __syncthreads();
     uint32_t nh = 0, nm = 0, inf[4] = {0, 0, 0, 0}, inc[4] = {0, 0, 0, 0};
     unsigned long long fx[8] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
-    for (uint32_t c = tid; c < sd.C; c += 256) {
+    for (uint32_t c = part * 256 + tid; c < sd.C; c += split * 256) {
         const uint32_t* pw = v.part + (size_t)(sd.comm_off + c) * kPartWords;
         const uint4 a = reinterpret_cast<const uint4*>(pw)[0];
         const uint2 b = reinterpret_cast<const uint2*>(pw + 4)[0];
@@ -1104,12 +1109,15 @@ __global__ __launch_bounds__(256) void k_day_reduce(EngineView v) {
     if (tid < 10) {
         const uint32_t k = tid;
         const uint32_t x = s_cnt[0][k] + s_cnt[1][k] + s_cnt[2][k] + s_cnt[3][k];
-        if (k == 0) v.series[si.n_count + row] = x;
-        else if (k == 1) v.series[si.m_count + row] = x;
-        else if (k < 6) { if (k - 2 < S) v.series[si.n_inf + (size_t)row * S + (k - 2)] = x; }
-        else if (k - 6 < S) v.series[si.n_inc + (size_t)row * S + (k - 6)] = x;
+        uint32_t* dst = nullptr;
+        if (k == 0) dst = &v.series[si.n_count + row];
+        else if (k == 1) dst = &v.series[si.m_count + row];
+        else if (k < 6) { if (k - 2 < S) dst = &v.series[si.n_inf + (size_t)row * S + (k - 2)]; }
+        else if (k - 6 < S) dst = &v.series[si.n_inc + (size_t)row * S + (k - 6)];
+        if (dst && x) atomicAdd(dst, x);
     }
-    if (tid >= 32 && tid < 40) v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + (tid - 32)] = s_fx[tid - 32];
+    if (tid >= 32 && tid < 40 && s_fx[tid - 32])
+        atomicAdd(&v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + (tid - 32)], s_fx[tid - 32]);
 }
 
 // Age-of-infection rings (nepidemic.cu:229-243), one workgroup per
@@ -1286,7 +1294,7 @@ hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st) 
 }
 
 hipError_t launch_longdist(const EngineView& v, const Tuning& tu, hipStream_t st) {
-    hipLaunchKernelGGL(k_day_reduce, dim3(v.n_sims), dim3(256), 0, st, v);
+    hipLaunchKernelGGL(k_day_reduce, dim3(v.n_sims * kReduceSplit), dim3(256), 0, st, v, kReduceSplit);
     hipLaunchKernelGGL(k_day_longdist, dim3(v.n_sims * tu.ld_split), dim3(256), 0, st, v, tu.ld_split);
     return hipGetLastError();
 }
