@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--cpu-days", type=int, default=365)
     ap.add_argument("--cpu-serial-days", type=int, default=365)
     ap.add_argument("--graph", action="store_true", help="replay the day loop as hipGraphs (no per-kernel timing)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="plain workload: launch every kernel (per-kernel HIP events) instead of graph replay")
     ap.add_argument("--autotune-days", type=int, default=2,
                     help="days per candidate launch tuning, timed within the warm-up (0: default tuning)")
     ap.add_argument("--mcmc", action="store_true", help="MCMC ESS/hour, plain vs surrogate, instead")
@@ -251,7 +253,7 @@ def main():
         args.steps = 3600 if plain else 1800
     if args.warmup is None:
         args.warmup = 50 if plain else 25
-    if plain:
+    if plain and not args.no_graph:
         args.graph = True
 
     from arbo_gpu_fit_amd import abi, configs
@@ -339,7 +341,7 @@ def main():
     step_gbs = step_bytes / (elapsed / args.steps) / 1e9
     out["step_roofline"] = {"bytes_per_step": step_bytes, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS}
-    if plain:   # kernel means from a separate pass with per-kernel HIP events, same launch tuning
+    if plain and args.graph:   # kernel means from a separate pass with per-kernel HIP events, same tuning
         tun = eng.tuning()
         eng.close()
         eng = Engine(lib, device=0, flags=abi.F_TIMING)
@@ -369,7 +371,7 @@ def main():
             "mean_launch_ms": mean_a,
             "traffic_source": PROFILE if traffic is not None else None,
         }
-        if plain:
+        if plain and args.graph:
             out["roofline"]["pass"] = (f"a second run of the same days ({args.warmup}..{days - 1}) with per-kernel "
                                        "HIP events")
         out["kernels_ms"] = {"k_day_demography": ms_a, "k_day_events": ms_b, "launches": n_a}
