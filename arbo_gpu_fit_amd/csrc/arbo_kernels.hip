@@ -1101,8 +1101,12 @@ __global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split
         if (lane == 0) s_cnt[wv][k] = w;
     }
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k)
-        if (fx[k]) atomicAdd(&s_fx[k], fx[k]);
+    for (uint32_t k = 0; k < 8; ++k) {   // butterfly over the wave, then one LDS add per wave
+        unsigned long long x = fx[k];
+#pragma unroll
+        for (uint32_t d = kWave / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
+        if (lane == 0 && x) atomicAdd(&s_fx[k], x);
+    }
     __syncthreads();
     const uint32_t S = sd.S, row = t + 1;
     const SeriesIdx si = series_index(sd.series_off, v.max_days, S);
