@@ -12,6 +12,9 @@
 //   event application onto humans + introductions nepidemic.cu:146-319
 //   time-series counters                          datacollect.cu:10-189
 //   t=0 census                                    demographicintial.cu:86-297, epidemicinitial.cu:27-99
+//   per-community series, infection history,      datacollect.cu:10-23, demographicintial.cu:158-171,
+//   age-of-infection rings                        nepidemic.cu:229-298, ndemographic.cu:96-102
+//   climate-driven days and the logistic target   SimulationParam.cpp:92-137, simulation.cpp:84-132
 // with these deliberate changes, the same in the engine:
 //   - stateless Philox keyed on (seed; entity, day, purpose, index) instead of
 //     curand states picked by SM/warp/lane id;
