@@ -319,3 +319,33 @@ def sim_climate(lib, theta, base) -> abi.Climate:
     if rc:
         raise ArboError(rc, "arbo_mcmc_sim_climate")
     return out
+
+
+PRIOR_NAMES = {PRIOR_UNIFORM: "Uniform", PRIOR_CIRCULAR: "CircularUniform", PRIOR_BETA: "Beta",
+               PRIOR_TRUNC_NORMAL: "TruncatedNormal"}
+
+
+def write_chain_csv(path: str, theta: np.ndarray, loglik: np.ndarray, chain: int, names=PARAM_NAMES):
+    """One chain's accepted states in the layout of MCMC::writeChain
+    (MCMC.cpp:400-440): header "step,logLikelihood,<names>", then one line
+    per iteration with the chain's current log-likelihood and parameters.
+    theta [iterations, chains, 6] and loglik [iterations, chains] as from
+    Mcmc.history()."""
+    th = np.asarray(theta)[:, chain, :]
+    ll = np.asarray(loglik)[:, chain]
+    lines = ["step,logLikelihood," + ",".join(names)]
+    for it in range(th.shape[0]):
+        lines.append(f"{it},{float(ll[it])!r}," + ",".join(repr(float(x)) for x in th[it]))
+    with open(path, "w") as f:
+        f.write("\n".join(lines))
+
+
+def write_priors_csv(path: str, priors, names=PARAM_NAMES):
+    """The priors in the layout of MCMC::writePriors (MCMC.cpp:383-397):
+    "PARAMETER_NAME,PRIOR_DISTRIBUTION,PARAMETERS", the parameters as
+    "(a;b)"."""
+    lines = ["PARAMETER_NAME,PRIOR_DISTRIBUTION,PARAMETERS"]
+    for name, (k, a, b) in zip(names, priors):
+        lines.append(f"{name},{PRIOR_NAMES[k]},({a!r};{b!r})")
+    with open(path, "w") as f:
+        f.write("\n".join(lines))

@@ -256,3 +256,23 @@ def test_climate_fit_sends_each_chain_its_climate(lib, oracle_lib):
     orc.close()
     with pytest.raises(ArboError):   # the climate set needs a climate
         mcmc.Mcmc(cfg, 2, 2, humans, base, np.full(W, 20, np.uint32), lib=lib)
+
+
+def test_chain_and_prior_files(tmp_path, lib):
+    """MCMC.csv and MCMC_priors.csv layouts (MCMC.cpp:383-440), one chain
+    file per chain, read back value for value."""
+    rng = np.random.default_rng(2)
+    th = rng.normal(size=(5, 3, 6))
+    ll = rng.normal(size=(5, 3)) * 100
+    path = tmp_path / "MCMC.csv"
+    mcmc.write_chain_csv(str(path), th, ll, chain=1, names=mcmc.CLIMATE_PARAM_NAMES)
+    lines = path.read_text().split("\n")
+    assert lines[0] == "step,logLikelihood," + ",".join(mcmc.CLIMATE_PARAM_NAMES)
+    back = np.array([[float(x) for x in ln.split(",")] for ln in lines[1:]])
+    assert np.array_equal(back[:, 0], np.arange(5)) and np.array_equal(back[:, 1], ll[:, 1])
+    assert np.array_equal(back[:, 2:], th[:, 1, :])
+    pp = tmp_path / "MCMC_priors.csv"
+    mcmc.write_priors_csv(str(pp), configs.climate_priors(), mcmc.CLIMATE_PARAM_NAMES)
+    rows = pp.read_text().split("\n")
+    assert rows[0] == "PARAMETER_NAME,PRIOR_DISTRIBUTION,PARAMETERS"
+    assert rows[1] == "scale_eip,TruncatedNormal,(2.5;0.5)" and rows[6] == "reporting,Beta,(10.0;90.0)"
