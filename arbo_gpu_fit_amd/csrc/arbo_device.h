@@ -156,8 +156,10 @@ struct Tuning {
     uint32_t xcd;        // bit 0 demography, bit 1 events: the workgroups one XCD receives take a
                          // contiguous range of communities
     uint32_t ld_split;   // long-distance kernel: workgroups per simulation
+    uint32_t wps;        // demography: 0 the compiler's register budget, 8 eight waves per SIMD
 };
 constexpr uint32_t kDemographyShapes[][2] = {{4, 4}, {4, 2}, {8, 4}, {2, 2}};   // (unroll, early_m)
+constexpr uint32_t kOccShapes[][2] = {{4, 2}, {2, 2}};   // shapes also built for 8 waves per SIMD
 constexpr uint32_t kMaxLdSplit = 64;
 constexpr uint32_t kXcds = 8;   // gfx950: workgroups go to the 8 XCDs in turn
 

@@ -74,7 +74,7 @@ struct ArboEngine {
     // timing
     std::vector<hipEvent_t> ev_pool, ev_used;
     double kernel_ms[2] = {0.0, 0.0};
-    Tuning tuning{4, 4, 0, 8};
+    Tuning tuning{4, 4, 0, 8, 0};
     uint64_t kernel_launches[2] = {0, 0};
 };
 
@@ -514,8 +514,10 @@ int arbo_batch_step(ArboEngine* e, uint32_t n_days) {
 
 namespace {
 
-Tuning from_abi(const ArboTuning& t) { return Tuning{t.census_unroll, t.early_mosq, t.xcd_order, t.ld_split}; }
-ArboTuning to_abi(const Tuning& t) { return ArboTuning{t.unroll, t.early_m, t.xcd, t.ld_split}; }
+Tuning from_abi(const ArboTuning& t) {
+    return Tuning{t.census_unroll, t.early_mosq, t.xcd_order, t.ld_split, t.waves_per_simd};
+}
+ArboTuning to_abi(const Tuning& t) { return ArboTuning{t.unroll, t.early_m, t.xcd, t.ld_split, t.wps}; }
 
 int apply_tuning(ArboEngine* e, const Tuning& tu) {
     if (!tuning_supported(tu)) return fail(e, ARBO_E_INVALID, "unsupported tuning");
@@ -534,9 +536,13 @@ int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n) {
     uint32_t k = 0;
     for (const auto& d : kDemographyShapes)
         for (uint32_t x = 0; x <= 3; x += 3) {
-            if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8};
+            if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8, 0};
             ++k;
         }
+    for (const auto& d : kOccShapes) {   // eight waves per SIMD: for batches of few communities
+        if (out && k < cap) out[k] = ArboTuning{d[0], d[1], 0, 8, 8};
+        ++k;
+    }
     *n = k;
     return ARBO_OK;
 }

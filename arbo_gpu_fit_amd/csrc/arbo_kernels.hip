@@ -353,8 +353,10 @@ __device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& c
 // HIST: the infection history or the per-community series may be kept
 // (v.hist or v.spatial not null); a separate instantiation keeps their
 // stores out of the plain day loop.
-template <uint32_t UNROLL, uint32_t EARLY_M, bool XCD, bool HIST>
-__global__ __launch_bounds__(kWave) void k_day_demography(EngineView v) {
+// WPS: 0, or the resident waves per SIMD the register budget must allow.
+template <uint32_t UNROLL, uint32_t EARLY_M, bool XCD, bool HIST, uint32_t WPS>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPS ? WPS : 1)))
+void k_day_demography(EngineView v) {
     static_assert(EARLY_M >= 1 && EARLY_M <= UNROLL, "early mosquito batch within the first unrolled batch");
     const uint32_t g = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, lane = threadIdx.x;
     const uint32_t t = *v.day;
@@ -1255,14 +1257,21 @@ template <bool XCD, bool HIST>
 static hipError_t launch_demography_xh(const EngineView& v, const Tuning& tu, hipStream_t st) {
     const dim3 grid(v.n_comm), block(kWave);
     const size_t lds = (size_t)v.death_m_lds * 4;
-    if (tu.unroll == 4 && tu.early_m == 4)
-        hipLaunchKernelGGL((k_day_demography<4, 4, XCD, HIST>), grid, block, lds, st, v);
+    if (tu.wps == 8) {
+        if (tu.unroll == 4 && tu.early_m == 2)
+            hipLaunchKernelGGL((k_day_demography<4, 2, XCD, HIST, 8>), grid, block, lds, st, v);
+        else if (tu.unroll == 2 && tu.early_m == 2)
+            hipLaunchKernelGGL((k_day_demography<2, 2, XCD, HIST, 8>), grid, block, lds, st, v);
+        else
+            return hipErrorInvalidValue;
+    } else if (tu.unroll == 4 && tu.early_m == 4)
+        hipLaunchKernelGGL((k_day_demography<4, 4, XCD, HIST, 0>), grid, block, lds, st, v);
     else if (tu.unroll == 4 && tu.early_m == 2)
-        hipLaunchKernelGGL((k_day_demography<4, 2, XCD, HIST>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<4, 2, XCD, HIST, 0>), grid, block, lds, st, v);
     else if (tu.unroll == 8 && tu.early_m == 4)
-        hipLaunchKernelGGL((k_day_demography<8, 4, XCD, HIST>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<8, 4, XCD, HIST, 0>), grid, block, lds, st, v);
     else if (tu.unroll == 2 && tu.early_m == 2)
-        hipLaunchKernelGGL((k_day_demography<2, 2, XCD, HIST>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<2, 2, XCD, HIST, 0>), grid, block, lds, st, v);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1276,7 +1285,11 @@ static hipError_t launch_demography_x(const EngineView& v, const Tuning& tu, hip
 
 bool tuning_supported(const Tuning& tu) {
     bool shape = false;
-    for (const auto& d : kDemographyShapes) shape = shape || (d[0] == tu.unroll && d[1] == tu.early_m);
+    if (tu.wps == 8) {
+        for (const auto& d : kOccShapes) shape = shape || (d[0] == tu.unroll && d[1] == tu.early_m);
+    } else if (tu.wps == 0) {
+        for (const auto& d : kDemographyShapes) shape = shape || (d[0] == tu.unroll && d[1] == tu.early_m);
+    }
     return shape && tu.xcd <= 3 && tu.ld_split >= 1 && tu.ld_split <= kMaxLdSplit;
 }
 

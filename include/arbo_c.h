@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define ARBO_ABI_VERSION 4u
+#define ARBO_ABI_VERSION 5u
 #define ARBO_MAX_SEROTYPES 4u
 #define ARBO_MAX_COMMUNITY_HUMANS 8192u
 #define ARBO_MAX_COMMUNITY_SLOTS 8192u
@@ -276,11 +276,14 @@ typedef struct ArboTuning {
     uint32_t xcd_order;       /* bit 0 demography, bit 1 event kernel: the workgroups one XCD receives
                                  take a contiguous range of communities */
     uint32_t ld_split;        /* long-distance kernel: workgroups per simulation, 1..64 */
+    uint32_t waves_per_simd;  /* demography kernel: 0 the compiler's register budget, 8 one that
+                                 keeps 8 waves per SIMD (with census_unroll/early_mosq 4/2 or
+                                 2/2): more resident waves for batches of few communities */
 } ArboTuning;
 
 /* The tunings arbo_autotune tries (out may be NULL to count them). */
 ARBO_API int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n);
-/* Set or read the engine's tuning (default {4, 4, 0, 8}); ARBO_E_INVALID for a
+/* Set or read the engine's tuning (default {4, 4, 0, 8, 0}); ARBO_E_INVALID for a
  * combination the library was not built with. */
 ARBO_API int arbo_set_tuning(ArboEngine* e, const ArboTuning* t);
 ARBO_API int arbo_get_tuning(const ArboEngine* e, ArboTuning* out);

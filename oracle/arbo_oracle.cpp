@@ -110,7 +110,7 @@ struct ArboEngine {
     int threads = 0;
     bool ready = false;
     std::string err;
-    ArboTuning tuning{4, 4, 0, 8};   // launch knobs: accepted and reported, no effect on the CPU
+    ArboTuning tuning{4, 4, 0, 8, 0};   // launch knobs: accepted and reported, no effect on the CPU
     uint32_t flags = 0;
 };
 
@@ -1053,15 +1053,20 @@ int arbo_reset_timing(ArboEngine* e) { return e ? ARBO_OK : ARBO_E_INVALID; }
 // Launch tunings: the same candidate set as the GPU library, so a caller's
 // tuning calls run unchanged; the CPU engine's results never depend on them.
 static const uint32_t kShapes[4][2] = {{4, 4}, {4, 2}, {8, 4}, {2, 2}};
+static const uint32_t kOcc8Shapes[2][2] = {{4, 2}, {2, 2}};   // also offered with 8 waves per SIMD
 
 int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n) {
     if (!n) return ARBO_E_INVALID;
     uint32_t k = 0;
     for (const auto& d : kShapes)
         for (uint32_t x = 0; x <= 3; x += 3) {
-            if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8};
+            if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8, 0};
             ++k;
         }
+    for (const auto& d : kOcc8Shapes) {
+        if (out && k < cap) out[k] = ArboTuning{d[0], d[1], 0, 8, 8};
+        ++k;
+    }
     *n = k;
     return ARBO_OK;
 }
@@ -1069,7 +1074,11 @@ int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n) {
 int arbo_set_tuning(ArboEngine* e, const ArboTuning* t) {
     if (!e || !t) return ARBO_E_INVALID;
     bool shape = false;
-    for (const auto& d : kShapes) shape = shape || (d[0] == t->census_unroll && d[1] == t->early_mosq);
+    if (t->waves_per_simd == 8) {
+        for (const auto& d : kOcc8Shapes) shape = shape || (d[0] == t->census_unroll && d[1] == t->early_mosq);
+    } else if (t->waves_per_simd == 0) {
+        for (const auto& d : kShapes) shape = shape || (d[0] == t->census_unroll && d[1] == t->early_mosq);
+    }
     if (!shape || t->xcd_order > 3 || t->ld_split < 1 || t->ld_split > 64) return fail(e, ARBO_E_INVALID, "unsupported tuning");
     e->tuning = *t;
     return ARBO_OK;
