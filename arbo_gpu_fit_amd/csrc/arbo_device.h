@@ -25,7 +25,8 @@ constexpr uint32_t kAoiRecord = 5000;     // ages per age-of-infection ring
 // births, 1 humans, 2-3 infectious humans (two 16-bit serotype pairs), 4-5
 // E->I onsets (pairs), 8-23 the eight long-distance rates or counts (u64).
 constexpr uint32_t kPartWords = 24;
-constexpr uint32_t kReduceSplit = 8;      // workgroups per simulation of k_day_reduce
+constexpr uint32_t kMaxReduceSplit = 64;  // workgroups per simulation of k_day_reduce: about one
+                                          // community per thread, 1..64, about 2,048 in all
 constexpr uint32_t kAoiWindow = 3650;     // rings record the days t with max_days - t <= this
 
 struct CommDesc {
@@ -146,6 +147,7 @@ struct EngineView {
     uint32_t n_comm;
     uint32_t max_days;
     uint32_t death_m_lds;          // entries of the largest mosquito death table (LDS staging)
+    uint32_t reduce_split;         // workgroups per simulation of k_day_reduce
 };
 
 // Launch knobs of the day kernels.  None of them changes a result: every

@@ -480,6 +480,13 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.n_comm = e->n_comm;
     v.max_days = max_days;
     v.death_m_lds = death_m_lds;
+    {   // about one community per thread of k_day_reduce, at most kMaxReduceSplit workgroups per
+        // simulation and about 2,048 in all (8 per CU)
+        uint32_t max_c = 1;
+        for (const SimHost& sh : e->sims) max_c = std::max(max_c, sh.C);
+        const uint32_t cap = std::max<uint32_t>(1, std::min<uint32_t>(kMaxReduceSplit, 2048 / n_sims));
+        v.reduce_split = std::min<uint32_t>(cap, (max_c + 255) / 256);
+    }
 
     HIP_TRY(e, launch_init(v, e->stream));
     HIP_TRY(e, hipStreamSynchronize(e->stream));

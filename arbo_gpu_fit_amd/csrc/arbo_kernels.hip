@@ -1068,11 +1068,13 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t spl
 }
 
 // Sums the day's per-community partials of a simulation: `split`
-// workgroups of 256 threads each take a share of its communities and add
+// workgroups of 256 threads (about one community per thread, at most 64
+// per simulation and about 2,048 in all) each take a share of its
+// communities and add
 // their sums to the series row t + 1 (datacollect.cu:79-189, zero until
 // then) and to the long-distance totals the long-distance kernel reads
 // (zeroed by the previous day's long-distance kernel), `split` integer
-// atomics per address, so the order does not matter.
+// atomics per word, so the order does not matter.
 __global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split) {
     const uint32_t sim = blockIdx.x / split, part = blockIdx.x % split;
     const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
@@ -1311,7 +1313,7 @@ hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st) 
 }
 
 hipError_t launch_longdist(const EngineView& v, const Tuning& tu, hipStream_t st) {
-    hipLaunchKernelGGL(k_day_reduce, dim3(v.n_sims * kReduceSplit), dim3(256), 0, st, v, kReduceSplit);
+    hipLaunchKernelGGL(k_day_reduce, dim3(v.n_sims * v.reduce_split), dim3(256), 0, st, v, v.reduce_split);
     hipLaunchKernelGGL(k_day_longdist, dim3(v.n_sims * tu.ld_split), dim3(256), 0, st, v, tu.ld_split);
     return hipGetLastError();
 }
