@@ -64,6 +64,9 @@ LAYOUT_PROGRAM = r"""
 #include <stdio.h>
 #include "arbo_c.h"
 #include "arbo_mcmc.h"
+#include "arbo_output.h"
+#include "arbo_data.h"
+#include "arbo_surrogate.h"
 #define F(T, f) printf("%s.%s %zu\n", #T, #f, offsetof(T, f));
 int main(void) {
     printf("ArboEngineDesc %zu\nArboParams %zu\nArboShape %zu\nArboSeries %zu\nArboState %zu\n",
