@@ -7,6 +7,7 @@ the MLP's matrix products on the MFMA units, the chains in the C++ driver;
 this module sequences them and summarises the output."""
 from __future__ import annotations
 
+import dataclasses
 import math
 import sys
 import time
@@ -45,20 +46,36 @@ def prior_sample(n: int, seed: int, priors=None) -> np.ndarray:
     return th
 
 
+def model_spec(lib, theta, seed: int, g: int, humans, base, param_set: int = mcmc.FIT_SEASONAL,
+               climate=None) -> SimSpec:
+    """The simulation of one parameter vector: the seasonal set's five
+    parameters in ArboParams, or the climate set's five coefficients in the
+    climate (MCMC::copyParams, MCMC.cpp:79-86)."""
+    if param_set == mcmc.FIT_CLIMATE:
+        t = [float(x) for x in theta]
+        cl = dataclasses.replace(climate, scale_eip=t[0], scale_mle=t[1], scale_cc=t[2], scale_humidity=t[3],
+                                 min_mos_to_human=t[4])
+        return SimSpec(g, g, humans, base, seed, climate=cl)
+    return SimSpec(g, g, humans, mcmc.sim_params(lib, theta, base), seed, climate=climate)
+
+
 def simulate_weekly(engine: Engine, thetas: np.ndarray, seed: int, batch: int = 256,
-                    n_weeks: int = configs.MCMC_WEEKS) -> np.ndarray:
+                    n_weeks: int = configs.MCMC_WEEKS, param_set: int = mcmc.FIT_SEASONAL, climate=None,
+                    shape=None, lib=None) -> np.ndarray:
     """Weekly incidence [n, n_weeks] of one simulation per vector of the MCMC
-    model (1e6 humans, 32 x 32 communities), `batch` simulations per engine
-    batch; simulation i has Philox seed (i << 32) | seed."""
-    lib = mcmc.bind(engine.lib)
-    g, humans = configs.mcmc_shape()
+    model (1e6 humans, 32 x 32 communities, or shape = (grid, humans)),
+    `batch` simulations per engine batch; simulation i has Philox seed
+    (i << 32) | seed.  lib: the library of the parameter mapping and weekly
+    binning (default: the engine's)."""
+    lib = mcmc.bind(lib if lib is not None else engine.lib)
+    g, humans = shape if shape is not None else configs.mcmc_shape()
     base = configs.base_params()
     days = 7 * n_weeks - 1
     out = np.zeros((len(thetas), n_weeks), np.uint32)
     t0 = time.perf_counter()
     for b0 in range(0, len(thetas), batch):
         th = thetas[b0:b0 + batch]
-        specs = [SimSpec(g, g, humans, mcmc.sim_params(lib, t, base), ((b0 + i) << 32) | (seed & 0xFFFFFFFF))
+        specs = [model_spec(lib, t, ((b0 + i) << 32) | (seed & 0xFFFFFFFF), g, humans, base, param_set, climate)
                  for i, t in enumerate(th)]
         engine.init(specs, days)
         engine.step(days)
@@ -69,7 +86,8 @@ def simulate_weekly(engine: Engine, thetas: np.ndarray, seed: int, batch: int = 
 
 
 def train_surrogate(thetas: np.ndarray, weekly: np.ndarray, epochs: int = 200, batch: int = 512,
-                    lr: float = 1e-3, seed: int = 1, lib=None, stage: int = 10) -> tuple[Surrogate, np.ndarray]:
+                    lr: float = 1e-3, seed: int = 1, lib=None, stage: int = 10,
+                    priors=None) -> tuple[Surrogate, np.ndarray]:
     """The MLP (6 -> 256^4 -> n_weeks, GELU) fitted to the weekly incidence
     by the Poisson loss (Adam), its output bias started at the data's
     log-mean per week.  The learning rate follows a cosine decay from `lr` to
@@ -77,7 +95,7 @@ def train_surrogate(thetas: np.ndarray, weekly: np.ndarray, epochs: int = 200, b
     between stages)."""
     widths = list(SURROGATE_WIDTHS[:-1]) + [weekly.shape[1]]
     sur = Surrogate(widths, max_batch=max(batch, 4096), seed=seed, lib=lib)
-    x = features(thetas, configs.mcmc_priors()).astype(np.float32)
+    x = features(thetas, priors or configs.mcmc_priors()).astype(np.float32)
     y = weekly.astype(np.float32)
     p = sur.get_params()
     p[-weekly.shape[1]:] = np.log(y.mean(axis=0) + 1e-2)
@@ -91,10 +109,12 @@ def train_surrogate(thetas: np.ndarray, weekly: np.ndarray, epochs: int = 200, b
     return sur, np.concatenate(hist)
 
 
-def summarise(m: mcmc.Mcmc, wall: float, burn: int) -> dict:
+def summarise(m: mcmc.Mcmc, wall: float, burn: int, names=mcmc.PARAM_NAMES) -> dict:
     th, ll, acc = m.history()
     post = th[burn:].copy()
-    post[:, :, 4] = circular_unwrap(post[:, :, 4], 0.0, 2 * math.pi)
+    for i, (k, a, b) in enumerate(m.cfg.priors):   # circular parameters unwrapped before ESS and R-hat
+        if k == mcmc.PRIOR_CIRCULAR:
+            post[:, :, i] = circular_unwrap(post[:, :, i], a, b)
     e = [ess(post[:, :, i]) for i in range(mcmc.D)]
     r = [rhat(post[:, :, i]) for i in range(mcmc.D)]
     n = th.shape[0]
@@ -105,33 +125,41 @@ def summarise(m: mcmc.Mcmc, wall: float, burn: int) -> dict:
         "wall_s": wall,
         "s_per_iteration": wall / n,
         "ess_min": min(e),
-        "ess": dict(zip(mcmc.PARAM_NAMES, e)),
+        "ess": dict(zip(names, e)),
         "ess_per_hour": min(e) / (sampling_wall / 3600.0),
-        "rhat": dict(zip(mcmc.PARAM_NAMES, r)),
+        "rhat": dict(zip(names, r)),
         "rhat_max": max(r),
         "acceptance": float(acc[burn:].mean()),
-        "posterior_mean": dict(zip(mcmc.PARAM_NAMES, th[burn:].reshape(-1, mcmc.D).mean(axis=0).tolist())),
-        "posterior_sd": dict(zip(mcmc.PARAM_NAMES, th[burn:].reshape(-1, mcmc.D).std(axis=0).tolist())),
+        "posterior_mean": dict(zip(names, th[burn:].reshape(-1, mcmc.D).mean(axis=0).tolist())),
+        "posterior_sd": dict(zip(names, th[burn:].reshape(-1, mcmc.D).std(axis=0).tolist())),
         "counters": m.counters(),
     }
 
 
 def run_fit(engine: Engine, data: np.ndarray, iters: int, chains: int = 64, seed: int = 1, sur: Surrogate | None = None,
             sub_steps: int = 10, theta0=None, burn: int | None = None, chunk: int = 10,
-            da_mode: int = mcmc.DA_LANGEVIN) -> tuple[mcmc.Mcmc, dict]:
+            da_mode: int = mcmc.DA_LANGEVIN, param_set: int = mcmc.FIT_SEASONAL, climate=None,
+            shape=None, api=None, lib=None) -> tuple[mcmc.Mcmc, dict]:
     """A plain (sur None) or delayed-acceptance fit of `data` (first stage
     `da_mode`); returns the driver and its summary over the iterations after
-    `burn`."""
-    g, humans = configs.mcmc_shape()
-    cfg = mcmc.McmcConfig(n_chains=chains, n_weeks=len(data), priors=configs.mcmc_priors(),
-                          adapt_after=100 * mcmc.D, nb_size=configs.MCMC_NB_SIZE, seed=seed)
-    m = mcmc.Mcmc(cfg, g, g, humans, configs.base_params(), data, theta0=theta0, lib=engine.lib)
+    `burn`.  param_set FIT_CLIMATE fits the reference's own parameters
+    (climate coefficients, configs.climate_priors()) under `climate`.  api:
+    mcmc.engine_api of the engine's library when it is not the driver's
+    library `lib` (default: the engine's) (plain fits)."""
+    g, humans = shape if shape is not None else configs.mcmc_shape()
+    fit_climate = param_set == mcmc.FIT_CLIMATE
+    priors = configs.climate_priors() if fit_climate else configs.mcmc_priors()
+    names = mcmc.CLIMATE_PARAM_NAMES if fit_climate else mcmc.PARAM_NAMES
+    cfg = mcmc.McmcConfig(n_chains=chains, n_weeks=len(data), priors=priors,
+                          adapt_after=100 * mcmc.D, nb_size=configs.MCMC_NB_SIZE, seed=seed, param_set=param_set)
+    m = mcmc.Mcmc(cfg, g, g, humans, configs.base_params(), data, theta0=theta0,
+                  lib=lib if lib is not None else engine.lib, climate=climate)
     t0 = time.perf_counter()
     done = 0
     while done < iters:
         n = min(chunk, iters - done)
         if sur is None:
-            m.run(engine, n)
+            m.run(engine, n, api=api)
         else:
             m.run_da(engine, sur, n, sub_steps=sub_steps, mode=da_mode)
         done += n
@@ -139,7 +167,7 @@ def run_fit(engine: Engine, data: np.ndarray, iters: int, chains: int = 64, seed
         log(f"[fit] {kind} {done}/{iters} iterations, "
             f"{time.perf_counter() - t0:.1f} s")
     wall = time.perf_counter() - t0
-    out = summarise(m, wall, iters // 4 if burn is None else burn)
+    out = summarise(m, wall, iters // 4 if burn is None else burn, names)
     if sur is not None:
         out["delayed_acceptance"] = m.da_counters()
     return m, out

@@ -276,3 +276,38 @@ def test_chain_and_prior_files(tmp_path, lib):
     rows = pp.read_text().split("\n")
     assert rows[0] == "PARAMETER_NAME,PRIOR_DISTRIBUTION,PARAMETERS"
     assert rows[1] == "scale_eip,TruncatedNormal,(2.5;0.5)" and rows[6] == "reporting,Beta,(10.0;90.0)"
+
+
+def test_fit_workflow_with_the_reference_parameter_set(lib, oracle_lib):
+    """fit.model_spec puts a climate-set vector's five coefficients into the
+    simulation's climate (MCMC::copyParams, MCMC.cpp:79-86): the training
+    simulations equal direct runs under those coefficients, differ between
+    vectors, and a plain fit over them summarises under the reference's
+    parameter names."""
+    from arbo_gpu_fit_amd import fit
+    from arbo_gpu_fit_amd.engine import ClimateSpec
+    W = 5
+    cl = configs.seasonal_climate(7 * W, biting_rate=1.0)
+    humans = np.full(4, 250, np.uint32)
+    shape = (2, humans)
+    th = fit.prior_sample(3, seed=4, priors=configs.climate_priors())
+    assert np.all(th[:, :5] > 0)
+    orc = Engine(oracle_lib, threads=2)
+    weekly = fit.simulate_weekly(orc, th, seed=9, n_weeks=W, param_set=mcmc.FIT_CLIMATE, climate=cl, shape=shape,
+                                 lib=lib)
+    for i in range(3):
+        c = ClimateSpec(cl.temperature, cl.humidity, cl.precipitation, cl.biting_rate, *(float(x) for x in th[i, :5]),
+                        cl.mos_net_growth)
+        spec = SimSpec(2, 2, humans, configs.base_params(), (i << 32) | 9, climate=c)
+        e = Engine(oracle_lib, threads=2)
+        e.init([spec], 7 * W - 1)
+        e.step(7 * W - 1)
+        assert np.array_equal(weekly[i], mcmc.weekly_incidence(lib, e.series(0)["n_incidence"], W)), i
+        e.close()
+    assert len({tuple(w) for w in weekly}) > 1
+    m, out = fit.run_fit(orc, weekly[0], iters=2, chains=4, seed=3, param_set=mcmc.FIT_CLIMATE, climate=cl,
+                         shape=shape, api=mcmc.engine_api(oracle_lib), lib=lib, burn=0)
+    assert set(out["ess"]) == set(mcmc.CLIMATE_PARAM_NAMES)
+    assert out["counters"]["proposals"] == 8
+    m.close()
+    orc.close()
