@@ -68,3 +68,26 @@ def test_unsupported_tuning_is_rejected(hip_lib):
     with pytest.raises(RuntimeError):
         gpu.set_tuning(ld_split=0)
     gpu.close()
+
+
+def test_every_tuning_keeps_the_outputs(hip_lib, oracle_lib):
+    """The instantiations that keep the infection history, the per-community
+    series and the age-of-infection rings, under every launch tuning."""
+    specs, T = CASES["reference_params"]()
+    flags = abi.F_SPATIAL | abi.F_AOI
+    orc = Engine(oracle_lib, flags=flags, threads=4)
+    orc.init(specs, T)
+    orc.step(T)
+    ref = (orc.spatial(0), orc.history(0), orc.aoi(0), orc.state(0))
+    orc.close()
+    gpu = Engine(hip_lib, flags=flags)
+    for cand in gpu.tuning_candidates():
+        print(f"[tuning] outputs kept, tuning={cand}", flush=True)
+        gpu.init(specs, T)
+        gpu.set_tuning(**cand)
+        gpu.step(T)
+        assert np.array_equal(gpu.spatial(0), ref[0]), cand
+        assert np.array_equal(gpu.history(0), ref[1]), cand
+        assert np.array_equal(gpu.aoi(0), ref[2]), cand
+        _compare_state(gpu.state(0), ref[3], f"{cand}")
+    gpu.close()
