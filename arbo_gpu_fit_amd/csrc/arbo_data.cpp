@@ -19,6 +19,11 @@ std::string unquote(const std::string& s) {
     return s;
 }
 
+// Drops the carriage return of a line saved with CRLF endings.
+void chomp(std::string& line) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+}
+
 // Reads the named columns of a csv file: out[k][row] for names[k].  Entry j
 // of a data line belongs to header column j mod (header columns).
 int read_columns(const char* path, const std::vector<std::string>& names, std::vector<std::vector<float>>& out) {
@@ -27,6 +32,7 @@ int read_columns(const char* path, const std::vector<std::string>& names, std::v
     if (!f.is_open()) return ARBO_E_INVALID;
     std::string line, cell;
     if (!std::getline(f, line)) return ARBO_E_INVALID;
+    chomp(line);
     std::vector<int> col(names.size(), -1);
     int ncol = 0;
     {
@@ -42,6 +48,7 @@ int read_columns(const char* path, const std::vector<std::string>& names, std::v
         if (c < 0) return ARBO_E_INVALID;
     out.assign(names.size(), {});
     while (std::getline(f, line)) {
+        chomp(line);
         std::istringstream in(line);
         int j = 0;
         while (std::getline(in, cell, ',')) {
@@ -93,7 +100,7 @@ ARBO_API int arbo_community_sizes(uint32_t W, uint32_t H, uint32_t nb, uint32_t 
                                   float hetero, uint64_t seed, uint32_t* out) {
     if (!out || W < 1 || H < 1 || (nb != 4 && nb != 8) || !std::isfinite(hetero)) return ARBO_E_INVALID;
     const uint64_t C = (uint64_t)W * H;
-    if (C > (1u << 24) || total < C) return ARBO_E_INVALID;
+    if (C > (1u << 24) || total < C || total > 0xffffffffull) return ARBO_E_INVALID;
     // cumulative relative sizes degree^hetero, normalised (network.cpp:138-159), as 24-bit thresholds
     std::vector<double> cum(C);
     double acc = 0.0;

@@ -24,7 +24,10 @@ extern "C" {
 #endif
 
 /* climate_smooth.csv: columns "T" (deg C), "H" (relative humidity), "P"
- * (precipitation); NULL arrays are skipped.  The arrays feed ArboClimate. */
+ * (precipitation); NULL arrays are skipped.  The arrays feed ArboClimate.
+ * Both readers take columns by header name, strip double quotes and accept
+ * LF or CRLF line endings; a cell that does not start with a number is an
+ * error. */
 ARBO_API int arbo_read_climate_csv(const char* path, float* temperature, float* humidity, float* precipitation,
                                    uint32_t cap, uint32_t* rows);
 
@@ -39,7 +42,8 @@ ARBO_API int arbo_read_dengue_csv(const char* path, float* incidence, uint32_t c
  * one human per community); each of the other total - C individuals draws
  * its community independently, a 24-bit Philox uniform keyed by (seed,
  * individual) against the cumulative 24-bit thresholds.  out[c], c =
- * y * grid_w + x.  Errors: total < C, bad lattice, hetero not finite. */
+ * y * grid_w + x.  Errors: total < C or above 2^32 - 1, bad lattice, hetero
+ * not finite. */
 ARBO_API int arbo_community_sizes(uint32_t grid_w, uint32_t grid_h, uint32_t neighbourhood, uint32_t include_self,
                                   uint64_t total, float hetero, uint64_t seed, uint32_t* out);
 

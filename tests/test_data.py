@@ -48,6 +48,26 @@ def test_dengue_file(tmp_path):
         read_dengue_csv(str(path))
 
 
+def test_crlf_files(tmp_path):
+    """A file saved with CRLF endings reads as the LF one, the last column
+    of the header included."""
+    path = tmp_path / "climate_crlf.csv"
+    path.write_bytes(b'"date","T","H","P"\r\n"d0",20.5,0.75,3\r\n"d1",21.25,0.5,0\r\n')
+    cl = read_climate_csv(str(path))
+    assert cl.temperature.tolist() == [20.5, 21.25]
+    assert cl.humidity.tolist() == [0.75, 0.5] and cl.precipitation.tolist() == [3.0, 0.0]
+    path = tmp_path / "dengue_crlf.csv"
+    path.write_bytes(b'"week","incidence"\r\n1,4\r\n2,9\r\n')
+    assert read_dengue_csv(str(path)).tolist() == [4.0, 9.0]
+
+
+def test_community_sizes_reject_bad_totals():
+    with pytest.raises(ArboError):
+        community_sizes(4, 4, 15, 1.0, seed=1)          # fewer individuals than communities
+    with pytest.raises(ArboError):
+        community_sizes(4, 4, 1 << 32, 1.0, seed=1)     # counts are 32-bit
+
+
 @pytest.mark.parametrize("hetero,nb", [(0.0, 4), (2.0, 4), (1.5, 8)])
 def test_community_sizes_follow_degree_power(hetero, nb):
     """One individual per community, then a multinomial with p_c proportional
