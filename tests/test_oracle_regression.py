@@ -1,7 +1,8 @@
 """Drift guard of the oracle (CPU): SHA-256 digests of every parity case's
 final census and series, and of the kept outputs (per-community series,
 infection history, age-of-infection rings) of the reference's parameter
-set, against the digests committed in tests/golden/oracle_digests.json
+set, and of a short MCMC run over the oracle engine, against the digests
+committed in tests/golden/oracle_digests.json
 (made by tests/golden/make_oracle_digests.py).
 
 The digests pin nothing about correctness: the closed forms, known-answer
@@ -56,6 +57,20 @@ def kept_output_digests(lib) -> dict:
                 "aoi": _digest({"v": e.aoi(0)})}
 
 
+def mcmc_digests(oracle_lib) -> dict:
+    """Four iterations of six chains of the configured fit over the oracle
+    engine: proposals, log-likelihoods and decisions."""
+    from arbo_gpu_fit_amd import mcmc
+    from tests.test_mcmc import _tiny_driver
+    lib = mcmc.bind(abi.load_hip_library())
+    m, orc = _tiny_driver(lib, oracle_lib)
+    m.run(orc, 4, api=mcmc.engine_api(oracle_lib))
+    th, ll, acc = m.history()
+    m.close()
+    orc.close()
+    return {"chains": _digest({"theta": th, "loglik": ll, "accepted": acc})}
+
+
 def _golden() -> dict:
     with open(GOLDEN) as f:
         return json.load(f)
@@ -68,6 +83,10 @@ def test_case_outputs_unchanged(name, oracle_lib):
 
 def test_kept_outputs_unchanged(oracle_lib):
     assert kept_output_digests(oracle_lib) == _golden()["kept_outputs"]
+
+
+def test_mcmc_chains_unchanged(oracle_lib):
+    assert mcmc_digests(oracle_lib) == _golden()["mcmc"]
 
 
 def test_digest_sees_one_changed_word():
