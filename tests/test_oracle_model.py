@@ -176,12 +176,15 @@ def test_resume_and_batch_invariance(oracle_lib):
         e.close()
 
 
-@pytest.mark.parametrize("field,value,code", [
+INVALID_PARAMS = [
     ("n_serotypes", 5, abi.E_INVALID), ("n_serotypes", 0, abi.E_INVALID), ("neighbourhood", 6, abi.E_INVALID),
     ("beta_hm", -1.0, abi.E_INVALID), ("beta_mh", float("nan"), abi.E_INVALID),
     ("long_distance", 1.5, abi.E_INVALID), ("season_amp", 2.0, abi.E_INVALID), ("iip_max", 1, abi.E_INVALID),
     ("m_scale", 0.0, abi.E_INVALID), ("seed_frac_h", -0.1, abi.E_INVALID), ("immunity_mode", 3, abi.E_INVALID),
-])
+]
+
+
+@pytest.mark.parametrize("field,value,code", INVALID_PARAMS)
 def test_invalid_parameters(oracle_lib, field, value, code):
     spec = small_spec()
     setattr(spec.params, field, value)
