@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 MAX_SEROTYPES = 4
 MAX_COMMUNITY_HUMANS = 8192
 MAX_COMMUNITY_SLOTS = 8192

@@ -218,42 +218,47 @@ def mcmc_spec(theta: dict | None = None, seed: int = 3, **kw) -> SimSpec:
 
 
 # ---------------------------------------------------------------------------
-# The batched MCMC fit: 6 parameters (beta_hm, beta_mh, log10 long-distance
-# fraction, seasonal amplitude, seasonal phase, reporting rate) of a model of
-# 1e6 humans in 1,000 requested communities (32 x 32 after squaring), 260
-# weekly counts (7 * 260 - 1 = 1,819 days, main.cpp:34), negative-binomial
-# observations of size r = 10 (variance mu + mu^2 / 10; the reference's
-# dispersion parameter is 1 / r, MCMC.cpp:242-243), synthetic data from the
-# model at MCMC_TRUTH.
+# The batched MCMC fit: 7 parameters (beta_hm, beta_mh, log10 long-distance
+# fraction, seasonal amplitude, seasonal phase, reporting rate, NB
+# dispersion) of a model of 1e6 humans in 1,000 requested communities (32 x
+# 32 after squaring), 260 weekly counts (7 * 260 - 1 = 1,819 days,
+# main.cpp:34), negative-binomial observations with dispersion d = 0.1, i.e.
+# size 1 / d = 10 (variance mu + mu^2 / 10; MCMC.cpp:240-243), held fixed, so
+# six parameters are fitted; synthetic data from the model at MCMC_TRUTH.
 
 MCMC_WEEKS = 260
-MCMC_NB_SIZE = 10.0
-MCMC_TRUTH = np.array([0.3, 0.3, -2.0, 0.5, 1.0, 0.1])
+MCMC_DISPERSION = 0.1
+MCMC_TRUTH = np.array([0.3, 0.3, -2.0, 0.5, 1.0, 0.1, MCMC_DISPERSION])
+PRIOR_FIXED = 4
 
 
 def mcmc_priors():
     """(kind, a, b) per parameter: the batch's parameter ranges
     (beta ~ U[0.05, 0.6], long-distance ~ LogU[1e-3, 1e-1], amplitude ~
-    U[0, 0.9], phase ~ U[0, 2 pi) on the circle) and the reference's
-    observationRate prior Beta(10, 90) for the reporting rate (MCMC.h:47-48)."""
+    U[0, 0.9], phase ~ U[0, 2 pi) on the circle), the reference's
+    observationRate prior Beta(10, 90) for the reporting rate (MCMC.h:47-48)
+    and the dispersion fixed at MCMC_DISPERSION."""
     return [(0, 0.05, 0.6), (0, 0.05, 0.6), (0, -3.0, -1.0), (0, 0.0, 0.9), (1, 0.0, 2.0 * math.pi),
-            (2, 10.0, 90.0)]
+            (2, 10.0, 90.0), (PRIOR_FIXED, MCMC_DISPERSION, 0.0)]
 
 
 def climate_priors():
     """The reference's fitted set (MCMC.h:37-50): TruncatedNormal priors of
     scaleEIP (2.5, 0.5), scaleMLE (3, 0.5), scaleCC (1, 0.4), scaleHumidity
-    (1, 0.4), minMosToHuman (3, 1) and Beta(10, 90) for the observation
-    (reporting) rate; kind 3 is the normal truncated to (0, 10000]."""
-    return [(3, 2.5, 0.5), (3, 3.0, 0.5), (3, 1.0, 0.4), (3, 1.0, 0.4), (3, 3.0, 1.0), (2, 10.0, 90.0)]
+    (1, 0.4), minMosToHuman (3, 1), Beta(10, 90) for the observation
+    (reporting) rate and Beta(20, 80) for the NB dispersion; kind 3 is the
+    normal truncated to (0, 10000]."""
+    return [(3, 2.5, 0.5), (3, 3.0, 0.5), (3, 1.0, 0.4), (3, 1.0, 0.4), (3, 3.0, 1.0), (2, 10.0, 90.0),
+            (2, 20.0, 80.0)]
 
 
 def mcmc_data(engine, theta=MCMC_TRUTH, n_weeks: int = MCMC_WEEKS, seed: int = 3, obs_seed: int = 7,
-              total_humans: int = 1_000_000, communities: int = 1000, nb_size: float = MCMC_NB_SIZE):
+              total_humans: int = 1_000_000, communities: int = 1000):
     """Synthetic weekly counts: one simulation of the model at `theta` (Philox
     seed `seed`) on `engine`, its weekly incidence (all serotypes) observed
-    through NB(mean reporting * incidence, size nb_size) with numpy's PCG64
-    seeded by `obs_seed`.  Returns (observed, simulated weekly incidence)."""
+    through NB(mean reporting * incidence, size 1 / dispersion) with numpy's
+    PCG64 seeded by `obs_seed`.  Returns (observed, simulated weekly
+    incidence)."""
     from . import abi as _abi
     from .mcmc import sim_params, weekly_incidence
     lib = _abi.load_hip_library()
@@ -265,5 +270,6 @@ def mcmc_data(engine, theta=MCMC_TRUTH, n_weeks: int = MCMC_WEEKS, seed: int = 3
     wk = weekly_incidence(lib, engine.series(0)["n_incidence"], n_weeks)
     rng = np.random.Generator(np.random.PCG64(obs_seed))
     mu = float(theta[5]) * wk.astype(np.float64)
+    nb_size = 1.0 / float(theta[6])
     obs = rng.negative_binomial(nb_size, nb_size / (nb_size + mu)).astype(np.uint32)
     return obs, wk

@@ -27,25 +27,31 @@
 #include "arbo_c.h"
 #include "arbo_mcmc.h"
 #include "arbo_rng.h"
+#include "arbo_tables.h"
 
 using namespace arbo;
 
 namespace {
 
 constexpr uint32_t D = ARBO_MCMC_PARAMS;
+constexpr uint32_t kModel = 5;                      // parameters 0..4 set the simulation
+constexpr uint32_t kRho = 5, kDisp = 6;             // reporting rate, NB dispersion
+constexpr uint32_t kSurIn = ARBO_MCMC_SUR_INPUTS;   // surrogate inputs of parameters 0..4
 constexpr double kTwoPi = 6.283185307179586;
 constexpr double kTruncHi = 10000.0;   // upper end of the truncated-normal priors (MCMC.h:37-46)
 
 // Philox purposes of the driver (key: the driver's seed).
-enum : uint32_t { Q_MIX = 1, Q_NORMAL = 2, Q_ACCEPT = 3, Q_SIM_SEED = 4 };
+enum : uint32_t { Q_MIX = 1, Q_NORMAL = 2, Q_ACCEPT = 3, Q_SIM_SEED = 4, Q_BLOCK = 8 };
 constexpr uint32_t kInitIter = 0xFFFFFFFFu;
 
 double prior_mean(const ArboPrior& p) {
     if (p.kind == ARBO_PRIOR_BETA) return p.a / (p.a + p.b);
+    if (p.kind == ARBO_PRIOR_FIXED) return p.a;
     return 0.5 * (p.a + p.b);
 }
 
 double prior_sd(const ArboPrior& p) {
+    if (p.kind == ARBO_PRIOR_FIXED) return 0.0;
     if (p.kind == ARBO_PRIOR_BETA) {
         const double s = p.a + p.b;
         return std::sqrt(p.a * p.b / (s * s * (s + 1.0)));
@@ -71,6 +77,8 @@ double prior_logpdf(const ArboPrior& p, double x) {
                                        std::erfc(p.a / (p.b * std::sqrt(2.0))));
             return -0.5 * z * z - std::log(p.b) - 0.5 * std::log(2.0 * M_PI) - std::log(mass);
         }
+        case ARBO_PRIOR_FIXED:   // a point mass: the parameter is not fitted
+            return x == p.a ? 0.0 : -INFINITY;
     }
     return -INFINITY;
 }
@@ -80,14 +88,28 @@ bool valid_prior(const ArboPrior& p) {
     if (p.kind == ARBO_PRIOR_BETA) return p.a > 0.0 && p.b > 0.0;
     if (p.kind == ARBO_PRIOR_TRUNC_NORMAL) return p.b > 0.0 && p.a > -10.0 * p.b && p.a < kTruncHi;
     if (p.kind == ARBO_PRIOR_UNIFORM || p.kind == ARBO_PRIOR_CIRCULAR) return p.b > p.a;
+    if (p.kind == ARBO_PRIOR_FIXED) return true;
     return false;
 }
 
 // Inputs the parameters 0..4 take: 1 each, 2 for a circular one.
 uint32_t n_inputs(const ArboPrior* priors) {
     uint32_t n = 0;
-    for (uint32_t i = 0; i + 1 < D; ++i) n += priors[i].kind == ARBO_PRIOR_CIRCULAR ? 2u : 1u;
+    for (uint32_t i = 0; i < kModel; ++i) n += priors[i].kind == ARBO_PRIOR_CIRCULAR ? 2u : 1u;
     return n;
+}
+
+// Digamma by the recurrence psi(x) = psi(x + 1) - 1/x up to x >= 6, then
+// the asymptotic series (Abramowitz & Stegun 6.3.18); x > 0.
+double digamma(double x) {
+    double acc = 0.0;
+    while (x < 6.0) {
+        acc -= 1.0 / x;
+        x += 1.0;
+    }
+    const double f = 1.0 / (x * x);
+    return acc + std::log(x) - 0.5 / x -
+           f * (1.0 / 12.0 - f * (1.0 / 120.0 - f * (1.0 / 252.0 - f * (1.0 / 240.0 - f / 132.0))));
 }
 
 // Cholesky factor (lower) of a d x d symmetric positive-definite matrix;
@@ -124,8 +146,10 @@ struct ArboMcmc {
     std::vector<uint32_t> data;
     Key key{0, 0};
     uint32_t C = 0;
-    // chain state
+    // chain state: parameters, likelihood estimate, log prior, and the Philox
+    // seeds of the chain's particles (the pseudo-marginal auxiliary state)
     std::vector<double> theta, loglik, logprior;
+    std::vector<uint64_t> useed;
     bool started = false;
     uint32_t iter = 0;
     // pooled adaptation (Welford over every chain state after every iteration)
@@ -133,11 +157,12 @@ struct ArboMcmc {
     double mean[D] = {};
     double m2[D * D] = {};
     double sd0[D] = {};
-    // history
+    // history: states after each iteration, and each iteration's candidates
     std::vector<double> h_theta, h_loglik;
     std::vector<uint8_t> h_acc;
+    std::vector<double> h_cand, h_cand_lp;
     // counters
-    uint64_t n_prop = 0, n_support = 0, n_sims = 0, n_acc = 0;
+    uint64_t n_prop = 0, n_support = 0, n_sims = 0, n_acc = 0, n_unsim = 0;
     double eval_s = 0.0;
     // delayed acceptance
     uint64_t da_steps = 0, da_acc = 0;
@@ -155,9 +180,10 @@ int fail(ArboMcmc* m, int code, const std::string& msg) {
 
 double u01(uint32_t a, uint32_t b) { return u53(a, b); }
 
-// Six standard normals of (chain, iteration) by Box-Muller on 53-bit uniforms.
+// D standard normals of (chain, iteration) by Box-Muller on 53-bit
+// uniforms; z holds D rounded up to even.
 void normals(const ArboMcmc* m, uint32_t c, uint32_t it, double* z) {
-    for (uint32_t k = 0; k < D / 2; ++k) {
+    for (uint32_t k = 0; k < (D + 1) / 2; ++k) {
         const U4 r = draw(m->key, c, it, Q_NORMAL, k);
         const double u1 = 1.0 - u01(r.x, r.y);   // (0, 1]
         const double u2 = u01(r.z, r.w);
@@ -185,9 +211,15 @@ void pool_add(ArboMcmc* m, const double* th) {
 }
 
 // Proposal of chain c at iteration it (see file header).
+// Parameters with a fixed prior keep their value.
+void pin_fixed(const ArboMcmc* m, double* th) {
+    for (uint32_t i = 0; i < D; ++i)
+        if (m->desc.priors[i].kind == ARBO_PRIOR_FIXED) th[i] = m->desc.priors[i].a;
+}
+
 void propose(const ArboMcmc* m, uint32_t c, uint32_t it, const double* L, bool adapted, double* out) {
     const double* th = &m->theta[(size_t)c * D];
-    double z[D];
+    double z[D + 1];
     normals(m, c, it, z);
     const U4 r = draw(m->key, c, it, Q_MIX, 0);
     const bool fixed = !adapted || u01(r.x, r.y) < m->desc.mix_beta;
@@ -210,6 +242,7 @@ void propose(const ArboMcmc* m, uint32_t c, uint32_t it, const double* L, bool a
             if (out[i] >= p.b) out[i] = p.a;
         }
     }
+    pin_fixed(m, out);
 }
 
 uint64_t sim_seed(const ArboMcmc* m, uint32_t c, uint32_t it, uint32_t particle) {
@@ -217,8 +250,27 @@ uint64_t sim_seed(const ArboMcmc* m, uint32_t c, uint32_t it, uint32_t particle)
     return ((uint64_t)r.y << 32) | r.x;
 }
 
-double nb_week(uint32_t k, double mu, double r) {
-    if (!(mu > 0.0)) return k == 0 ? 0.0 : -INFINITY;
+// Seeds of chain c's particles for its proposal of iteration it: the
+// starting values use the chain's initial seeds; with n_blocks = 0 every
+// particle is fresh; with G blocks the particles p = b (mod G) of one
+// uniform block b are fresh and the rest keep the chain's seeds.
+void particle_seeds(const ArboMcmc* m, uint32_t c, uint32_t it, uint64_t* out) {
+    const uint32_t P = m->desc.n_particles, G = m->desc.n_blocks;
+    const uint64_t* cur = &m->useed[(size_t)c * P];
+    if (it == kInitIter) {
+        std::memcpy(out, cur, sizeof(uint64_t) * P);
+        return;
+    }
+    const uint32_t b = G ? mulhi(draw(m->key, c, it, Q_BLOCK, 0).x, G) : 0u;
+    for (uint32_t p = 0; p < P; ++p) out[p] = (G == 0 || p % G == b) ? sim_seed(m, c, it, p) : cur[p];
+}
+
+// NB log pmf of count k, mean mu, size r (MCMC.cpp:216-243 with r = 1 /
+// dispersion).  A zero mean gives -inf for k > 0; for k = 0 it gives 0, or
+// -inf under ARBO_MCMC_F_ZERO_WEEK_FLOOR (the reference computes 0 log 0 =
+// NaN there and its fmax(-250, NaN) keeps the floor).
+double nb_week(uint32_t k, double mu, double r, bool zero_floor = false) {
+    if (!(mu > 0.0)) return (k == 0 && !zero_floor) ? 0.0 : -INFINITY;
     // lgamma(k + r) - lgamma(k + 1) - lgamma(r) + k log(mu/(r+mu)) + r log(r/(r+mu))
     const double lrm = std::log(r + mu);
     return std::lgamma((double)k + r) - std::lgamma((double)k + 1.0) - std::lgamma(r) +
@@ -226,10 +278,10 @@ double nb_week(uint32_t k, double mu, double r) {
 }
 
 double nb_loglik(const uint32_t* sim_weekly, const uint32_t* data, uint32_t W, uint32_t skip, double rho,
-                 double r, double floor_v) {
+                 double r, double floor_v, bool zero_floor = false) {
     double s = 0.0;
     for (uint32_t w = skip; w < W; ++w) {
-        const double v = nb_week(data[w], rho * (double)sim_weekly[w], r);
+        const double v = nb_week(data[w], rho * (double)sim_weekly[w], r, zero_floor);
         s += v > floor_v ? v : floor_v;
     }
     return s;
@@ -245,29 +297,55 @@ void weekly(const uint32_t* inc, uint32_t rows, uint32_t S, uint32_t W, uint32_t
 }
 
 // log-likelihoods of n vectors from simulations on the engine; particles of a
-// vector are combined by log-mean-exp (MCMC.cpp:252-282)
+// vector are combined by log-mean-exp (MCMC.cpp:252-282).  Vectors the
+// engine's own checks would reject (arbo::validate, the test batch_init
+// applies) are not simulated: they get -inf, so a proposal the engine cannot
+// represent is rejected instead of failing the whole batch.
 int eval_engine(ArboMcmc* m, ArboEngine* eng, const ArboEngineApi& api, uint32_t n, const double* th,
                 const uint32_t* chains, uint32_t it, double* ll) {
     if (n == 0) return ARBO_OK;
     const uint32_t P = m->desc.n_particles, W = m->desc.n_weeks;
     const uint32_t T = 7 * W - 1, rows = T + 1;
     const uint32_t S = m->base.n_serotypes;
-    const uint32_t sims = n * P;
+    const bool fit_climate = m->desc.param_set == ARBO_FIT_CLIMATE;
+    std::vector<ArboClimate> climates(fit_climate ? n : 0);   // each vector's climate coefficients
+    std::vector<ArboParams> vparams(n);
+    std::vector<uint32_t> sim_of(n, UINT32_MAX);               // batch index of each vector's first particle
+    uint32_t nv = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (fit_climate) {
+            arbo_mcmc_sim_climate(th + (size_t)i * D, &m->climate, &climates[i]);
+            vparams[i] = m->base;
+        } else {
+            arbo_mcmc_sim_params(th + (size_t)i * D, &m->base, &vparams[i]);
+        }
+        const ArboClimate* cl = fit_climate ? &climates[i] : (m->has_climate ? &m->climate : nullptr);
+        const ArboShape sh{m->grid_w, m->grid_h, m->humans.data(), m->has_slots ? m->slots.data() : nullptr, cl,
+                           m->share.empty() ? nullptr : m->share.data()};
+        // the observation model needs a reporting rate >= 0 and a dispersion > 0
+        const bool observable = th[(size_t)i * D + kRho] >= 0.0 && th[(size_t)i * D + kDisp] > 0.0;
+        if (!observable || !validate(sh, vparams[i], T).empty()) {
+            ll[i] = -INFINITY;
+            m->n_unsim++;
+            continue;
+        }
+        sim_of[i] = nv * P;
+        ++nv;
+    }
+    if (nv == 0) return ARBO_OK;
+    const uint32_t sims = nv * P;
     std::vector<ArboShape> shapes(sims);
     std::vector<ArboParams> params(sims);
     std::vector<uint64_t> seeds(sims);
-    const bool fit_climate = m->desc.param_set == ARBO_FIT_CLIMATE;
-    std::vector<ArboClimate> climates(fit_climate ? n : 0);   // each vector's climate coefficients
     for (uint32_t i = 0; i < n; ++i) {
-        if (fit_climate) arbo_mcmc_sim_climate(th + (size_t)i * D, &m->climate, &climates[i]);
+        if (sim_of[i] == UINT32_MAX) continue;
         const ArboClimate* cl = fit_climate ? &climates[i] : (m->has_climate ? &m->climate : nullptr);
+        particle_seeds(m, chains[i], it, &seeds[sim_of[i]]);
         for (uint32_t p = 0; p < P; ++p) {
-            const uint32_t k = i * P + p;
+            const uint32_t k = sim_of[i] + p;
             shapes[k] = ArboShape{m->grid_w, m->grid_h, m->humans.data(), m->has_slots ? m->slots.data() : nullptr,
                                   cl, m->share.empty() ? nullptr : m->share.data()};
-            if (fit_climate) params[k] = m->base;
-            else arbo_mcmc_sim_params(th + (size_t)i * D, &m->base, &params[k]);
-            seeds[k] = sim_seed(m, chains[i], it, p);
+            params[k] = vparams[i];
         }
     }
     int rc = api.batch_init(eng, sims, shapes.data(), params.data(), seeds.data(), T);
@@ -276,15 +354,18 @@ int eval_engine(ArboMcmc* m, ArboEngine* eng, const ArboEngineApi& api, uint32_t
     std::vector<uint32_t> nc(rows), mc(rows), inf((size_t)rows * S), inc((size_t)rows * S), one((size_t)rows * S);
     std::vector<uint32_t> wk(W);
     std::vector<double> part(P);
+    const bool zero_floor = (m->desc.flags & ARBO_MCMC_F_ZERO_WEEK_FLOOR) != 0;
     for (uint32_t i = 0; i < n; ++i) {
-        const double rho = th[(size_t)i * D + 5];
+        if (sim_of[i] == UINT32_MAX) continue;
+        const double rho = th[(size_t)i * D + kRho];
+        const double r = 1.0 / th[(size_t)i * D + kDisp];
         for (uint32_t p = 0; p < P; ++p) {
             ArboSeries ser{nc.data(), mc.data(), inf.data(), inc.data(), one.data()};
-            rc = api.read_series(eng, i * P + p, &ser);
+            rc = api.read_series(eng, sim_of[i] + p, &ser);
             if (rc != ARBO_OK) return fail(m, rc, std::string("engine: ") + (api.last_error ? api.last_error(eng) : ""));
             weekly(inc.data(), rows, S, W, wk.data());
-            part[p] = nb_loglik(wk.data(), m->data.data(), W, m->desc.skip_weeks, rho, m->desc.nb_size,
-                                m->desc.week_floor);
+            part[p] = nb_loglik(wk.data(), m->data.data(), W, m->desc.skip_weeks, rho, r, m->desc.week_floor,
+                                zero_floor);
         }
         double mx = part[0];
         for (uint32_t p = 1; p < P; ++p) mx = part[p] > mx ? part[p] : mx;
@@ -294,6 +375,28 @@ int eval_engine(ArboMcmc* m, ArboEngine* eng, const ArboEngineApi& api, uint32_t
     }
     m->n_sims += sims;
     return ARBO_OK;
+}
+
+// A chain's move to its candidate: parameters, likelihood, prior and, for
+// the pseudo-marginal state, the seeds its likelihood came from.
+void accept_move(ArboMcmc* m, uint32_t c, uint32_t it, const double* th, double ll, double lp) {
+    std::memcpy(&m->theta[(size_t)c * D], th, sizeof(double) * D);
+    m->loglik[c] = ll;
+    m->logprior[c] = lp;
+    const uint32_t P = m->desc.n_particles;
+    std::vector<uint64_t> s(P);
+    particle_seeds(m, c, it, s.data());
+    std::memcpy(&m->useed[(size_t)c * P], s.data(), sizeof(uint64_t) * P);
+}
+
+// Appends the iteration's states and candidates to the history.
+void record_iteration(ArboMcmc* m, const std::vector<uint8_t>& acc, const std::vector<double>& cand,
+                      const std::vector<double>& cand_lp) {
+    m->h_theta.insert(m->h_theta.end(), m->theta.begin(), m->theta.end());
+    m->h_loglik.insert(m->h_loglik.end(), m->loglik.begin(), m->loglik.end());
+    m->h_acc.insert(m->h_acc.end(), acc.begin(), acc.end());
+    m->h_cand.insert(m->h_cand.end(), cand.begin(), cand.end());
+    m->h_cand_lp.insert(m->h_cand_lp.end(), cand_lp.begin(), cand_lp.end());
 }
 
 // Cholesky factor of the adapted proposal covariance 2.38^2/d Sigma_n (+ a
@@ -326,7 +429,7 @@ int run_iterations(ArboMcmc* m, uint32_t n_iters, Eval eval) {
         }
         m->started = true;
     }
-    std::vector<double> prop((size_t)C * D), sub((size_t)C * D), sub_ll(C);
+    std::vector<double> prop((size_t)C * D), sub((size_t)C * D), sub_ll(C), cand_lp(C);
     std::vector<uint32_t> sub_chain(C);
     std::vector<uint8_t> acc(C);
     for (uint32_t step = 0; step < n_iters; ++step) {
@@ -337,6 +440,7 @@ int run_iterations(ArboMcmc* m, uint32_t n_iters, Eval eval) {
         for (uint32_t c = 0; c < C; ++c) {
             propose(m, c, it, L, adapted, &prop[(size_t)c * D]);
             acc[c] = 0;
+            cand_lp[c] = -INFINITY;
             if (std::isfinite(log_prior(m, &prop[(size_t)c * D]))) {  // checkParams (MCMC.cpp:164-186)
                 std::memcpy(&sub[(size_t)ns * D], &prop[(size_t)c * D], sizeof(double) * D);
                 sub_chain[ns++] = c;
@@ -352,21 +456,18 @@ int run_iterations(ArboMcmc* m, uint32_t n_iters, Eval eval) {
             const uint32_t c = sub_chain[k];
             const double* th = &sub[(size_t)k * D];
             const double lp = log_prior(m, th);
+            cand_lp[c] = sub_ll[k] + lp;
             const double log_a = (sub_ll[k] + lp) - (m->loglik[c] + m->logprior[c]);
             const U4 r = draw(m->key, c, it, Q_ACCEPT, 0);
             const double u = 1.0 - u01(r.x, r.y);   // (0, 1]
             if (std::log(u) < log_a) {               // MCMC.cpp:306-344
-                std::memcpy(&m->theta[(size_t)c * D], th, sizeof(double) * D);
-                m->loglik[c] = sub_ll[k];
-                m->logprior[c] = lp;
+                accept_move(m, c, it, th, sub_ll[k], lp);
                 acc[c] = 1;
                 m->n_acc++;
             }
         }
         for (uint32_t c = 0; c < C; ++c) pool_add(m, &m->theta[(size_t)c * D]);
-        m->h_theta.insert(m->h_theta.end(), m->theta.begin(), m->theta.end());
-        m->h_loglik.insert(m->h_loglik.end(), m->loglik.begin(), m->loglik.end());
-        m->h_acc.insert(m->h_acc.end(), acc.begin(), acc.end());
+        record_iteration(m, acc, prop, cand_lp);
         m->iter++;
     }
     return ARBO_OK;
@@ -406,12 +507,18 @@ ARBO_API int arbo_mcmc_create(const ArboMcmcDesc* desc, const ArboShape* shape, 
     if (desc->abi_version != ARBO_ABI_VERSION) return ARBO_E_INVALID;
     if (desc->n_chains == 0 || desc->n_chains > ARBO_MCMC_MAX_CHAINS || desc->n_particles == 0 ||
         desc->n_weeks < 2 || desc->n_weeks > (1u << 20) || desc->skip_weeks >= desc->n_weeks ||
-        !(desc->nb_size > 0.0) || !(desc->mix_beta >= 0.0 && desc->mix_beta <= 1.0) || !(desc->init_scale > 0.0))
+        !(desc->mix_beta >= 0.0 && desc->mix_beta <= 1.0) || !(desc->init_scale > 0.0))
         return ARBO_E_INVALID;
     if ((uint64_t)desc->n_chains * desc->n_particles > (1u << 20)) return ARBO_E_INVALID;
+    if (desc->n_blocks > desc->n_particles * 1024u || (desc->flags & ~ARBO_MCMC_F_ZERO_WEEK_FLOOR) ||
+        desc->reserved0 != 0)
+        return ARBO_E_INVALID;
     for (uint32_t i = 0; i < D; ++i)
         if (!valid_prior(desc->priors[i])) return ARBO_E_INVALID;
-    if (n_inputs(desc->priors) > D || desc->priors[D - 1].kind == ARBO_PRIOR_CIRCULAR) return ARBO_E_INVALID;
+    // at most six surrogate inputs; the reporting rate and the dispersion are no circular parameters
+    if (n_inputs(desc->priors) > kSurIn || desc->priors[kRho].kind == ARBO_PRIOR_CIRCULAR ||
+        desc->priors[kDisp].kind == ARBO_PRIOR_CIRCULAR)
+        return ARBO_E_INVALID;
     if (desc->param_set > ARBO_FIT_CLIMATE || (desc->param_set == ARBO_FIT_CLIMATE && !shape->climate))
         return ARBO_E_INVALID;
     ArboMcmc* m = new (std::nothrow) ArboMcmc();
@@ -448,6 +555,10 @@ ARBO_API int arbo_mcmc_create(const ArboMcmcDesc* desc, const ArboShape* shape, 
     m->theta.resize((size_t)m->C * D);
     m->loglik.assign(m->C, -INFINITY);
     m->logprior.assign(m->C, 0.0);
+    m->useed.resize((size_t)m->C * desc->n_particles);
+    for (uint32_t c = 0; c < m->C; ++c)
+        for (uint32_t p = 0; p < desc->n_particles; ++p)
+            m->useed[(size_t)c * desc->n_particles + p] = sim_seed(m, c, kInitIter, p);
     for (uint32_t i = 0; i < D; ++i) m->sd0[i] = desc->init_scale * prior_sd(desc->priors[i]) / std::sqrt((double)D);
     for (uint32_t c = 0; c < m->C; ++c)
         for (uint32_t i = 0; i < D; ++i)
@@ -496,6 +607,22 @@ ARBO_API int arbo_mcmc_read(const ArboMcmc* m, uint32_t first, uint32_t count, d
     if (theta) std::memcpy(theta, m->h_theta.data() + (size_t)first * C * D, sizeof(double) * count * C * D);
     if (loglik) std::memcpy(loglik, m->h_loglik.data() + (size_t)first * C, sizeof(double) * count * C);
     if (accepted) std::memcpy(accepted, m->h_acc.data() + (size_t)first * C, (size_t)count * C);
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_mcmc_read_candidates(const ArboMcmc* m, uint32_t first, uint32_t count, double* theta,
+                                       double* logpost) {
+    if (!m) return ARBO_E_INVALID;
+    if ((uint64_t)first + count > m->iter) return ARBO_E_INVALID;
+    const size_t C = m->C;
+    if (theta) std::memcpy(theta, m->h_cand.data() + (size_t)first * C * D, sizeof(double) * count * C * D);
+    if (logpost) std::memcpy(logpost, m->h_cand_lp.data() + (size_t)first * C, sizeof(double) * count * C);
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_mcmc_unsimulable(const ArboMcmc* m, uint64_t* out) {
+    if (!m || !out) return ARBO_E_INVALID;
+    *out = m->n_unsim;
     return ARBO_OK;
 }
 
@@ -556,10 +683,10 @@ void input_scale(const ArboPrior& p, double& c, double& s) {
 }
 
 // Surrogate inputs: parameters 0..4 by their priors (a circular one as cos,
-// sin), zeros after; at most six inputs (checked at creation).
+// sin), zeros after; at most kSurIn inputs (checked at creation).
 void features_of(const ArboMcmc* m, const double* th, float* x) {
     uint32_t j = 0;
-    for (uint32_t i = 0; i + 1 < D; ++i) {
+    for (uint32_t i = 0; i < kModel; ++i) {
         const ArboPrior& p = m->desc.priors[i];
         if (p.kind == ARBO_PRIOR_CIRCULAR) {
             x[j++] = (float)std::cos(th[i]);
@@ -570,29 +697,35 @@ void features_of(const ArboMcmc* m, const double* th, float* x) {
             x[j++] = (float)((th[i] - c) / s);
         }
     }
-    while (j < D) x[j++] = 0.0f;
+    while (j < kSurIn) x[j++] = 0.0f;
 }
 
 
-// log prior gradient (zero for the uniform and circular priors)
+// log prior gradient (zero for the uniform, circular and fixed priors)
 double prior_grad(const ArboPrior& p, double x) {
     if (p.kind == ARBO_PRIOR_BETA) return (p.a - 1.0) / x - (p.b - 1.0) / (1.0 - x);
     if (p.kind == ARBO_PRIOR_TRUNC_NORMAL) return -(x - p.a) / (p.b * p.b);
     return 0.0;
 }
 
-// Surrogate log posterior and gradient of n vectors.
+// Surrogate log posterior and gradient of n vectors: the prior times the
+// NB likelihood (size 1 / dispersion) of the data given the reporting rate
+// times the surrogate's mean incidence.  A fixed parameter's gradient is 0.
 int sur_logpost(ArboMcmc* m, ArboSurrogate* sur, uint32_t n, const double* th, double* lp, double* grad) {
     if (n == 0) return ARBO_OK;
     const uint32_t W = m->desc.n_weeks;
-    const double r = m->desc.nb_size;
-    std::vector<float> x((size_t)n * D), f((size_t)n * W), dout((size_t)n * W, 0.0f), dx((size_t)n * D);
-    std::vector<double> drho(n, 0.0);
-    for (uint32_t i = 0; i < n; ++i) features_of(m, th + (size_t)i * D, &x[(size_t)i * D]);
+    std::vector<float> x((size_t)n * kSurIn), f((size_t)n * W), dout((size_t)n * W, 0.0f), dx((size_t)n * kSurIn);
+    std::vector<double> drho(n, 0.0), dr(n, 0.0);
+    for (uint32_t i = 0; i < n; ++i) features_of(m, th + (size_t)i * D, &x[(size_t)i * kSurIn]);
     int rc = arbo_sur_forward(sur, n, x.data(), f.data());
     if (rc != ARBO_OK) return fail(m, rc, std::string("surrogate: ") + arbo_sur_last_error(sur));
     for (uint32_t i = 0; i < n; ++i) {
-        const double rho = th[(size_t)i * D + 5];
+        const double rho = th[(size_t)i * D + kRho];
+        const double r = 1.0 / th[(size_t)i * D + kDisp];
+        if (!(rho >= 0.0) || !(th[(size_t)i * D + kDisp] > 0.0)) {   // no observation model
+            lp[i] = -INFINITY;
+            continue;
+        }
         double ll = 0.0;
         for (uint32_t w = m->desc.skip_weeks; w < W; ++w) {
             const double mu = rho * std::exp((double)f[(size_t)i * W + w]);
@@ -603,6 +736,9 @@ int sur_logpost(ArboMcmc* m, ArboSurrogate* sur, uint32_t n, const double* th, d
                 const double d = (double)k - ((double)k + r) * mu / (r + mu);   // d ll / d log mu
                 dout[(size_t)i * W + w] = (float)d;
                 drho[i] += d / rho;
+                if (grad)   // d ll / d r
+                    dr[i] += digamma((double)k + r) - digamma(r) + std::log(r) + 1.0 - std::log(r + mu) -
+                             ((double)k + r) / (r + mu);
             } else {
                 ll += m->desc.week_floor;
             }
@@ -615,9 +751,9 @@ int sur_logpost(ArboMcmc* m, ArboSurrogate* sur, uint32_t n, const double* th, d
         for (uint32_t i = 0; i < n; ++i) {
             const double* t = th + (size_t)i * D;
             double* g = grad + (size_t)i * D;
-            const float* d = &dx[(size_t)i * D];
+            const float* d = &dx[(size_t)i * kSurIn];
             uint32_t j = 0;   // the inputs of features_of, in order
-            for (uint32_t k = 0; k + 1 < D; ++k) {
+            for (uint32_t k = 0; k < kModel; ++k) {
                 const ArboPrior& p = m->desc.priors[k];
                 if (p.kind == ARBO_PRIOR_CIRCULAR) {
                     g[k] = -std::sin(t[k]) * (double)d[j] + std::cos(t[k]) * (double)d[j + 1];
@@ -629,7 +765,11 @@ int sur_logpost(ArboMcmc* m, ArboSurrogate* sur, uint32_t n, const double* th, d
                 }
                 g[k] += prior_grad(p, t[k]);
             }
-            g[5] = drho[i] + prior_grad(m->desc.priors[5], t[5]);
+            g[kRho] = drho[i] + prior_grad(m->desc.priors[kRho], t[kRho]);
+            // r = 1 / d: d ll / d d = -(d ll / d r) / d^2
+            g[kDisp] = -dr[i] / (t[kDisp] * t[kDisp]) + prior_grad(m->desc.priors[kDisp], t[kDisp]);
+            for (uint32_t k = 0; k < D; ++k)
+                if (m->desc.priors[k].kind == ARBO_PRIOR_FIXED) g[k] = 0.0;
         }
     }
     return ARBO_OK;
@@ -679,7 +819,7 @@ extern "C" {
 
 ARBO_API int arbo_mcmc_features(const ArboMcmc* m, uint32_t n, const double* theta, float* x) {
     if (!m || !theta || !x) return ARBO_E_INVALID;
-    for (uint32_t i = 0; i < n; ++i) features_of(m, theta + (size_t)i * D, x + (size_t)i * D);
+    for (uint32_t i = 0; i < n; ++i) features_of(m, theta + (size_t)i * D, x + (size_t)i * kSurIn);
     return ARBO_OK;
 }
 
@@ -687,7 +827,7 @@ ARBO_API int arbo_mcmc_surrogate_logpost(ArboMcmc* m, ArboSurrogate* sur, uint32
                                          double* out, double* grad) {
     if (!m || !sur || !theta || !out) return ARBO_E_INVALID;
     uint32_t in = 0, outw = 0;
-    if (arbo_sur_shape(sur, &in, &outw) != ARBO_OK || in != D || outw != m->desc.n_weeks)
+    if (arbo_sur_shape(sur, &in, &outw) != ARBO_OK || in != kSurIn || outw != m->desc.n_weeks)
         return fail(m, ARBO_E_INVALID, "surrogate shape does not match (6 inputs, n_weeks outputs)");
     return sur_logpost(m, sur, n, theta, out, grad);
 }
@@ -703,7 +843,7 @@ int check_da(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da) {
                                          !(da->target_accept > 0.0 && da->target_accept < 1.0)))
         return fail(m, ARBO_E_INVALID, "bad delayed-acceptance descriptor");
     uint32_t in = 0, outw = 0;
-    if (arbo_sur_shape(sur, &in, &outw) != ARBO_OK || in != D || outw != m->desc.n_weeks)
+    if (arbo_sur_shape(sur, &in, &outw) != ARBO_OK || in != kSurIn || outw != m->desc.n_weeks)
         return fail(m, ARBO_E_INVALID, "surrogate shape does not match (6 inputs, n_weeks outputs)");
     return ARBO_OK;
 }
@@ -723,7 +863,7 @@ int run_screen_impl(ArboMcmc* m, ArboSurrogate* sur, uint32_t n_iters, Ibm ibm) 
         if (rc != ARBO_OK) return rc;
     }
     std::vector<double> prop((size_t)C * D), sub((size_t)C * D), sub_slp(C), pass((size_t)C * D), pass_slp(C),
-        pass_ll(C);
+        pass_ll(C), cand_lp(C);
     std::vector<uint32_t> sub_chain(C), pass_chain(C);
     std::vector<uint8_t> acc(C);
     for (uint32_t step = 0; step < n_iters; ++step) {
@@ -734,7 +874,9 @@ int run_screen_impl(ArboMcmc* m, ArboSurrogate* sur, uint32_t n_iters, Ibm ibm) 
         for (uint32_t c = 0; c < C; ++c) {
             propose(m, c, it, L, adapted, &prop[(size_t)c * D]);
             acc[c] = 0;
+            cand_lp[c] = -INFINITY;
             if (std::isfinite(log_prior(m, &prop[(size_t)c * D]))) {   // checkParams (MCMC.cpp:164-186)
+                cand_lp[c] = NAN;   // until the simulator judges it
                 std::memcpy(&sub[(size_t)ns * D], &prop[(size_t)c * D], sizeof(double) * D);
                 sub_chain[ns++] = c;
             }
@@ -767,20 +909,17 @@ int run_screen_impl(ArboMcmc* m, ArboSurrogate* sur, uint32_t n_iters, Ibm ibm) 
             const double lp_new = log_prior(m, tp);
             // L(theta') L_s(theta) / (L(theta) L_s(theta')); the surrogate terms carry the prior
             const double log_a = (pass_ll[k] + lp_new - pass_slp[k]) - (m->loglik[c] + m->logprior[c] - s_lp[c]);
+            cand_lp[c] = pass_ll[k] + lp_new;
             const U4 r = draw(m->key, c, it, Q_ACCEPT, 0);
             if (std::log(1.0 - u01(r.x, r.y)) < log_a) {
-                std::memcpy(&m->theta[(size_t)c * D], tp, sizeof(double) * D);
-                m->loglik[c] = pass_ll[k];
-                m->logprior[c] = lp_new;
+                accept_move(m, c, it, tp, pass_ll[k], lp_new);
                 s_lp[c] = pass_slp[k];
                 acc[c] = 1;
                 m->n_acc++;
             }
         }
         for (uint32_t c = 0; c < C; ++c) pool_add(m, &m->theta[(size_t)c * D]);
-        m->h_theta.insert(m->h_theta.end(), m->theta.begin(), m->theta.end());
-        m->h_loglik.insert(m->h_loglik.end(), m->loglik.begin(), m->loglik.end());
-        m->h_acc.insert(m->h_acc.end(), acc.begin(), acc.end());
+        record_iteration(m, acc, prop, cand_lp);
         m->iter++;
     }
     return ARBO_OK;
@@ -824,7 +963,7 @@ int run_da_impl(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da, uint32_t 
         if (rc != ARBO_OK) return rc;
     }
     std::vector<double> cur((size_t)C * D), cur_lp(C), cur_g((size_t)C * D), prop((size_t)C * D), p_lp(C),
-        p_g((size_t)C * D), sub((size_t)C * D), sub_ll(C);
+        p_g((size_t)C * D), sub((size_t)C * D), sub_ll(C), cand_lp(C);
     std::vector<uint32_t> idx(C), sub_chain(C);
     std::vector<uint8_t> moved(C), acc(C);
     for (uint32_t step = 0; step < n_iters; ++step) {
@@ -845,6 +984,13 @@ int run_da_impl(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da, uint32_t 
             std::memset(L, 0, sizeof L);
             for (uint32_t i = 0; i < D; ++i) L[i * D + i] = m->sd0[i] * std::sqrt((double)D);
         }
+        // fixed parameters: an identity row and column (their gradient is 0 and
+        // every proposal is pinned back, so they never move)
+        for (uint32_t i = 0; i < D; ++i) {
+            if (m->desc.priors[i].kind != ARBO_PRIOR_FIXED) continue;
+            for (uint32_t k = 0; k < D; ++k) L[i * D + k] = L[k * D + i] = 0.0;
+            L[i * D + i] = 1.0;
+        }
         cur = m->theta;
         cur_lp = s_lp;
         cur_g = s_g;
@@ -856,7 +1002,7 @@ int run_da_impl(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da, uint32_t 
             for (uint32_t c = 0; c < C; ++c) {
                 const double* th = &cur[(size_t)c * D];
                 const double* g = &cur_g[(size_t)c * D];
-                double mg[D], z[D], mean[D];
+                double mg[D], z[D + 1], mean[D];
                 // M g = L (L^T g)
                 double ltg[D];
                 for (uint32_t i = 0; i < D; ++i) {
@@ -869,7 +1015,7 @@ int run_da_impl(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da, uint32_t 
                     for (uint32_t k = 0; k <= i; ++k) s += L[i * D + k] * ltg[k];
                     mg[i] = s;
                 }
-                for (uint32_t k = 0; k < D / 2; ++k) {
+                for (uint32_t k = 0; k < (D + 1) / 2; ++k) {
                     const U4 r = draw(m->key, c, sj, Q_MALA_NORMAL, k);
                     const double u1 = 1.0 - u01(r.x, r.y), u2 = u01(r.z, r.w);
                     const double rad = std::sqrt(-2.0 * std::log(u1));
@@ -884,6 +1030,7 @@ int run_da_impl(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da, uint32_t 
                     out[i] = mean[i] + eps * s;
                 }
                 wrap(m, out);
+                pin_fixed(m, out);
                 if (std::isfinite(log_prior(m, out))) {
                     std::memcpy(&sub[(size_t)nv * D], out, sizeof(double) * D);
                     idx[nv++] = c;
@@ -938,6 +1085,7 @@ int run_da_impl(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da, uint32_t 
         uint32_t ns = 0;
         for (uint32_t c = 0; c < C; ++c) {
             acc[c] = 0;
+            cand_lp[c] = NAN;   // a chain that did not move has no candidate for the simulator
             if (!moved[c]) continue;
             std::memcpy(&sub[(size_t)ns * D], &cur[(size_t)c * D], sizeof(double) * D);
             sub_chain[ns++] = c;
@@ -952,11 +1100,10 @@ int run_da_impl(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da, uint32_t 
             const double lp_new = log_prior(m, tp);
             // L(theta') L_s(theta) / (L(theta) L_s(theta')); the surrogate terms carry the prior
             const double log_a = (sub_ll[k] + lp_new - cur_lp[c]) - (m->loglik[c] + m->logprior[c] - s_lp[c]);
+            cand_lp[c] = sub_ll[k] + lp_new;
             const U4 r = draw(m->key, c, it, Q_ACCEPT, 0);
             if (std::log(1.0 - u01(r.x, r.y)) < log_a) {
-                std::memcpy(&m->theta[(size_t)c * D], tp, sizeof(double) * D);
-                m->loglik[c] = sub_ll[k];
-                m->logprior[c] = lp_new;
+                accept_move(m, c, it, tp, sub_ll[k], lp_new);
                 s_lp[c] = cur_lp[c];
                 std::memcpy(&s_g[(size_t)c * D], &cur_g[(size_t)c * D], sizeof(double) * D);
                 acc[c] = 1;
@@ -964,9 +1111,7 @@ int run_da_impl(ArboMcmc* m, ArboSurrogate* sur, const ArboDaDesc* da, uint32_t 
             }
         }
         for (uint32_t c = 0; c < C; ++c) pool_add(m, &m->theta[(size_t)c * D]);
-        m->h_theta.insert(m->h_theta.end(), m->theta.begin(), m->theta.end());
-        m->h_loglik.insert(m->h_loglik.end(), m->loglik.begin(), m->loglik.end());
-        m->h_acc.insert(m->h_acc.end(), acc.begin(), acc.end());
+        record_iteration(m, acc, cur, cand_lp);
         m->iter++;
     }
     return ARBO_OK;

@@ -133,6 +133,14 @@ std::string validate(const ArboShape& sh, const ArboParams& p, uint32_t max_days
     if (!(p.recovery >= 0.0f && p.recovery < 1000.0f) || !(p.eip >= 0.0f && p.eip < 1000.0f)) return "recovery/eip out of range";
     if (p.iip_max < p.iip_min || p.iip_max > 1000) return "bad incubation range";
     if (!(p.m_scale > 0.0f) || !(p.m_shape > 0.0f) || !(p.h_scale > 0.0f) || !(p.h_shape > 0.0f)) return "bad lifespan parameters";
+    // Lifetimes are integer ages up to the tables' last entries (a human death
+    // day fits 16 bits of age, a mosquito one 11), drawn by 24-bit uniforms:
+    // the lifetimes past the table (which would take its last age) must be
+    // below that resolution, F(last age) >= 1 - 2^-24 in 24-bit fixed point.
+    if (fixed24(human_cdf(p, (double)(kHumanAgeTable - 1))) < kOne24 - 1)
+        return "human lifespan beyond the 65,535-day lifetime table";
+    if (fixed24(mosq_cdf(p, (double)(kMosqAgeTable - 1))) < kOne24 - 1)
+        return "mosquito lifespan beyond the 2,047-day lifetime table";
     if (!(p.m_ratio >= 0.0f)) return "m_ratio must be >= 0";
     if (!(p.external_infection >= 0.0f)) return "external_infection must be >= 0";
     if (!(p.seed_frac_h >= 0.0f && p.seed_frac_h <= 1.0f) || !(p.seed_frac_m >= 0.0f && p.seed_frac_m <= 1.0f)) return "seed fractions must be in [0,1]";
@@ -151,6 +159,17 @@ std::string validate(const ArboShape& sh, const ArboParams& p, uint32_t max_days
     if (sh.climate) {
         const std::string why = validate_climate(*sh.climate, max_days);
         if (!why.empty()) return why;
+        // Births on day t search the running maximum of F(a; scale(t + a)),
+        // which is at least F(a; largest scale of the run): that must reach 1
+        // within the table for every birth day's row to.
+        double smax = 0.0;
+        for (uint32_t r = 1; r <= max_days; ++r) {
+            double d[4];
+            climate_day(*sh.climate, r, d);
+            smax = std::max(smax, d[3]);
+        }
+        if (fixed24(weibull_cdf((double)(kMosqAgeTable - 1), smax, (double)p.m_shape)) < kOne24 - 1)
+            return "climate gives a mosquito lifespan beyond the 2,047-day lifetime table";
         for (float m : climate_target(*sh.climate, (double)nh, max_days)) {
             if (!std::isfinite(m) || !(m >= 0.0f)) return "climate gives an invalid mosquito target";
             clim_peak = std::max(clim_peak, (double)m / (double)nh);

@@ -32,7 +32,9 @@ def prior_sample(n: int, seed: int, priors=None) -> np.ndarray:
     rng = np.random.Generator(np.random.PCG64(seed))
     th = np.zeros((n, mcmc.D))
     for i, (k, a, b) in enumerate(priors):
-        if k == mcmc.PRIOR_BETA:
+        if k == mcmc.PRIOR_FIXED:
+            th[:, i] = a
+        elif k == mcmc.PRIOR_BETA:
             th[:, i] = rng.beta(a, b, n)
         elif k == mcmc.PRIOR_TRUNC_NORMAL:   # N(a, b^2) on (0, TRUNC_HI] by rejection
             v = rng.normal(a, b, 4 * n + 64)
@@ -110,13 +112,17 @@ def train_surrogate(thetas: np.ndarray, weekly: np.ndarray, epochs: int = 200, b
 
 
 def summarise(m: mcmc.Mcmc, wall: float, burn: int, names=mcmc.PARAM_NAMES) -> dict:
+    """ESS, R-hat, acceptance and posterior moments over the iterations after
+    `burn`, over the fitted parameters (fixed ones left out)."""
     th, ll, acc = m.history()
     post = th[burn:].copy()
     for i, (k, a, b) in enumerate(m.cfg.priors):   # circular parameters unwrapped before ESS and R-hat
         if k == mcmc.PRIOR_CIRCULAR:
             post[:, :, i] = circular_unwrap(post[:, :, i], a, b)
-    e = [ess(post[:, :, i]) for i in range(mcmc.D)]
-    r = [rhat(post[:, :, i]) for i in range(mcmc.D)]
+    fitted = [i for i, (k, _, _) in enumerate(m.cfg.priors) if k != mcmc.PRIOR_FIXED]
+    names = [names[i] for i in fitted]
+    e = [ess(post[:, :, i]) for i in fitted]
+    r = [rhat(post[:, :, i]) for i in fitted]
     n = th.shape[0]
     sampling_wall = wall * (n - burn) / n
     return {
@@ -130,8 +136,8 @@ def summarise(m: mcmc.Mcmc, wall: float, burn: int, names=mcmc.PARAM_NAMES) -> d
         "rhat": dict(zip(names, r)),
         "rhat_max": max(r),
         "acceptance": float(acc[burn:].mean()),
-        "posterior_mean": dict(zip(names, th[burn:].reshape(-1, mcmc.D).mean(axis=0).tolist())),
-        "posterior_sd": dict(zip(names, th[burn:].reshape(-1, mcmc.D).std(axis=0).tolist())),
+        "posterior_mean": dict(zip(names, th[burn:].reshape(-1, mcmc.D)[:, fitted].mean(axis=0).tolist())),
+        "posterior_sd": dict(zip(names, th[burn:].reshape(-1, mcmc.D)[:, fitted].std(axis=0).tolist())),
         "counters": m.counters(),
     }
 
@@ -139,19 +145,24 @@ def summarise(m: mcmc.Mcmc, wall: float, burn: int, names=mcmc.PARAM_NAMES) -> d
 def run_fit(engine: Engine, data: np.ndarray, iters: int, chains: int = 64, seed: int = 1, sur: Surrogate | None = None,
             sub_steps: int = 10, theta0=None, burn: int | None = None, chunk: int = 10,
             da_mode: int = mcmc.DA_LANGEVIN, param_set: int = mcmc.FIT_SEASONAL, climate=None,
-            shape=None, api=None, lib=None) -> tuple[mcmc.Mcmc, dict]:
+            shape=None, api=None, lib=None, n_particles: int = 1, n_blocks: int = 0,
+            priors=None) -> tuple[mcmc.Mcmc, dict]:
     """A plain (sur None) or delayed-acceptance fit of `data` (first stage
     `da_mode`); returns the driver and its summary over the iterations after
     `burn`.  param_set FIT_CLIMATE fits the reference's own parameters
     (climate coefficients, configs.climate_priors()) under `climate`.  api:
     mcmc.engine_api of the engine's library when it is not the driver's
-    library `lib` (default: the engine's) (plain fits)."""
+    library `lib` (default: the engine's) (plain fits).  n_particles
+    simulations per proposal (the reference's C_MAXSIMRUN is 30,
+    constant.h:8) and n_blocks pseudo-marginal seed blocks (0: fresh seeds
+    for every proposal)."""
     g, humans = shape if shape is not None else configs.mcmc_shape()
     fit_climate = param_set == mcmc.FIT_CLIMATE
-    priors = configs.climate_priors() if fit_climate else configs.mcmc_priors()
+    if priors is None:
+        priors = configs.climate_priors() if fit_climate else configs.mcmc_priors()
     names = mcmc.CLIMATE_PARAM_NAMES if fit_climate else mcmc.PARAM_NAMES
-    cfg = mcmc.McmcConfig(n_chains=chains, n_weeks=len(data), priors=priors,
-                          adapt_after=100 * mcmc.D, nb_size=configs.MCMC_NB_SIZE, seed=seed, param_set=param_set)
+    cfg = mcmc.McmcConfig(n_chains=chains, n_weeks=len(data), priors=priors, adapt_after=100 * mcmc.D, seed=seed,
+                          param_set=param_set, n_particles=n_particles, n_blocks=n_blocks)
     m = mcmc.Mcmc(cfg, g, g, humans, configs.base_params(), data, theta0=theta0,
                   lib=lib if lib is not None else engine.lib, climate=climate)
     t0 = time.perf_counter()

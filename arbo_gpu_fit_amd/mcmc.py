@@ -14,10 +14,16 @@ import numpy as np
 from . import abi
 from .engine import ArboError, Engine
 
-D = 6
-PARAM_NAMES = ("beta_hm", "beta_mh", "log10_long_distance", "season_amp", "season_phase", "reporting")
-CLIMATE_PARAM_NAMES = ("scale_eip", "scale_mle", "scale_cc", "scale_humidity", "min_mos_to_human", "reporting")
-PRIOR_UNIFORM, PRIOR_CIRCULAR, PRIOR_BETA, PRIOR_TRUNC_NORMAL = 0, 1, 2, 3
+D = 7
+SUR_INPUTS = 6   # surrogate inputs of parameters 0-4
+PARAM_NAMES = ("beta_hm", "beta_mh", "log10_long_distance", "season_amp", "season_phase", "reporting", "dispersion")
+CLIMATE_PARAM_NAMES = ("scale_eip", "scale_mle", "scale_cc", "scale_humidity", "min_mos_to_human", "reporting",
+                       "dispersion")
+# the reference's names of the climate set (MCMC.h:37-50)
+REFERENCE_NAMES = ("scaleEIP", "scaleMLE", "scaleCC", "scaleHumidity", "minMosToHuman", "observationRate",
+                   "dispersion")
+PRIOR_UNIFORM, PRIOR_CIRCULAR, PRIOR_BETA, PRIOR_TRUNC_NORMAL, PRIOR_FIXED = 0, 1, 2, 3, 4
+F_ZERO_WEEK_FLOOR = 1   # zero-mean weeks score the floor even for a zero count (the reference's fmax(-250, NaN))
 TRUNC_HI = 10000.0   # upper end of the truncated-normal priors
 FIT_SEASONAL, FIT_CLIMATE = 0, 1
 
@@ -34,8 +40,9 @@ class Prior(C.Structure):
 class McmcDesc(C.Structure):
     _fields_ = [
         ("abi_version", _u32), ("n_chains", _u32), ("n_particles", _u32), ("n_weeks", _u32),
-        ("skip_weeks", _u32), ("adapt_after", _u32), ("param_set", _u32), ("reserved0", _u32),
-        ("nb_size", C.c_double), ("week_floor", C.c_double), ("mix_beta", C.c_double), ("init_scale", C.c_double),
+        ("skip_weeks", _u32), ("adapt_after", _u32), ("param_set", _u32), ("flags", _u32),
+        ("n_blocks", _u32), ("reserved0", _u32),
+        ("week_floor", C.c_double), ("mix_beta", C.c_double), ("init_scale", C.c_double),
         ("seed", C.c_uint64), ("priors", Prior * D),
     ]
 
@@ -69,6 +76,8 @@ PROTOTYPES = {
     "arbo_mcmc_da_counters": (C.c_int, [McmcPtr, _u64p, _u64p, _f64p, _f64p]),
     "arbo_mcmc_iterations": (C.c_int, [McmcPtr, _u32p]),
     "arbo_mcmc_read": (C.c_int, [McmcPtr, _u32, _u32, _f64p, _f64p, C.POINTER(C.c_uint8)]),
+    "arbo_mcmc_read_candidates": (C.c_int, [McmcPtr, _u32, _u32, _f64p, _f64p]),
+    "arbo_mcmc_unsimulable": (C.c_int, [McmcPtr, _u64p]),
     "arbo_mcmc_counters": (C.c_int, [McmcPtr, _u64p, _u64p, _u64p, _u64p, _f64p]),
     "arbo_mcmc_sim_params": (C.c_int, [_f64p, C.POINTER(abi.Params), C.POINTER(abi.Params)]),
     "arbo_mcmc_sim_climate": (C.c_int, [_f64p, C.POINTER(abi.Climate), C.POINTER(abi.Climate)]),
@@ -122,18 +131,23 @@ def engine_api(lib: C.CDLL) -> EngineApi:
 
 @dataclass
 class McmcConfig:
+    """ArboMcmcDesc (include/arbo_mcmc.h).  n_particles: simulations per
+    proposal (the reference's C_MAXSIMRUN is 30, constant.h:8); n_blocks: 0
+    fresh particle seeds for every proposal, G >= 1 block pseudo-marginal
+    (each proposal refreshes one of G blocks of the chain's seeds)."""
     n_chains: int
     n_weeks: int
-    priors: list            # [(kind, a, b)] * 6
+    priors: list            # [(kind, a, b)] * 7
     n_particles: int = 1
     skip_weeks: int = 1
     adapt_after: int = 600
-    nb_size: float = 10.0
     week_floor: float = -250.0
     mix_beta: float = 0.05
     init_scale: float = 0.1
     seed: int = 1
     param_set: int = FIT_SEASONAL
+    n_blocks: int = 0
+    flags: int = 0
 
     def desc(self) -> McmcDesc:
         d = McmcDesc()
@@ -144,7 +158,8 @@ class McmcConfig:
         d.skip_weeks = self.skip_weeks
         d.adapt_after = self.adapt_after
         d.param_set = self.param_set
-        d.nb_size = self.nb_size
+        d.flags = self.flags
+        d.n_blocks = self.n_blocks
         d.week_floor = self.week_floor
         d.mix_beta = self.mix_beta
         d.init_scale = self.init_scale
@@ -197,7 +212,7 @@ class Mcmc:
         self._check(self.lib.arbo_mcmc_run(self._h, engine._h, C.byref(api) if api is not None else None, n_iters))
 
     def run_target(self, loglik, n_iters: int):
-        """n_iters iterations with loglik(theta[n, 6]) -> [n] from Python."""
+        """n_iters iterations with loglik(theta[n, 7]) -> [n] from Python."""
         def cb(ctx, n, th, out):
             try:
                 t = np.ctypeslib.as_array(th, shape=(n * D,)).reshape(n, D).copy()
@@ -245,7 +260,7 @@ class Mcmc:
 
     def features(self, theta) -> np.ndarray:
         t = _f64(np.atleast_2d(theta))
-        x = np.zeros(t.shape, np.float32)
+        x = np.zeros((t.shape[0], SUR_INPUTS), np.float32)
         self._check(self.lib.arbo_mcmc_features(self._h, t.shape[0], t.ctypes.data_as(_f64p),
                                                 x.ctypes.data_as(C.POINTER(C.c_float))))
         return x
@@ -265,7 +280,7 @@ class Mcmc:
         return n.value
 
     def history(self, first: int = 0, count: int | None = None):
-        """(theta [iters, chains, 6], loglik [iters, chains], accepted [iters, chains])."""
+        """(theta [iters, chains, 7], loglik [iters, chains], accepted [iters, chains])."""
         n = self.iterations - first if count is None else count
         C_ = self.cfg.n_chains
         th = np.zeros((n, C_, D))
@@ -275,13 +290,28 @@ class Mcmc:
                                             acc.ctypes.data_as(C.POINTER(C.c_uint8))))
         return th, ll, acc
 
+    def candidates(self, first: int = 0, count: int | None = None):
+        """(theta [iters, chains, 7], logpost [iters, chains]): each
+        iteration's candidates and their log-likelihood plus log prior (-inf
+        outside the support or unsimulable, NaN when a delayed-acceptance
+        first stage stopped it)."""
+        n = self.iterations - first if count is None else count
+        C_ = self.cfg.n_chains
+        th = np.zeros((n, C_, D))
+        lp = np.zeros((n, C_))
+        self._check(self.lib.arbo_mcmc_read_candidates(self._h, first, n, th.ctypes.data_as(_f64p),
+                                                       lp.ctypes.data_as(_f64p)))
+        return th, lp
+
     def counters(self) -> dict:
         p, s, sims, a = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
         sec = C.c_double()
         self._check(self.lib.arbo_mcmc_counters(self._h, C.byref(p), C.byref(s), C.byref(sims), C.byref(a),
                                                 C.byref(sec)))
+        u = C.c_uint64()
+        self._check(self.lib.arbo_mcmc_unsimulable(self._h, C.byref(u)))
         return {"proposals": p.value, "in_support": s.value, "simulations": sims.value, "accepted": a.value,
-                "eval_seconds": sec.value}
+                "unsimulable": u.value, "eval_seconds": sec.value}
 
     def log_prior(self, theta) -> float:
         t = _f64(theta)
@@ -322,20 +352,92 @@ def sim_climate(lib, theta, base) -> abi.Climate:
 
 
 PRIOR_NAMES = {PRIOR_UNIFORM: "Uniform", PRIOR_CIRCULAR: "CircularUniform", PRIOR_BETA: "Beta",
-               PRIOR_TRUNC_NORMAL: "TruncatedNormal"}
+               PRIOR_TRUNC_NORMAL: "TruncatedNormal", PRIOR_FIXED: "Fixed"}
 
 
-def write_chain_csv(path: str, theta: np.ndarray, loglik: np.ndarray, chain: int, names=PARAM_NAMES):
+def _num(x: float) -> str:
+    """A number as C++'s default ostream formatting prints it (6
+    significant digits, %g), which the reference's writers use."""
+    x = float(x)
+    if np.isnan(x):
+        return "nan" if not np.signbit(x) else "-nan"
+    if np.isinf(x):
+        return "inf" if x > 0 else "-inf"
+    return f"{x:g}"
+
+
+def write_chain_csv(path: str, theta: np.ndarray, logpost: np.ndarray, chain: int, names=PARAM_NAMES):
     """One chain's accepted states in the layout of MCMC::writeChain
     (MCMC.cpp:400-440): header "step,logLikelihood,<names>", then one line
-    per iteration with the chain's current log-likelihood and parameters.
-    theta [iterations, chains, 6] and loglik [iterations, chains] as from
-    Mcmc.history()."""
+    per iteration with the chain's current log posterior (the reference's
+    prevAcceptedLogLikelihood includes the log prior, MCMC.cpp:284-300) and
+    parameters.  theta [iterations, chains, 7] and logpost [iterations,
+    chains] (history log-likelihood plus log prior)."""
     th = np.asarray(theta)[:, chain, :]
-    ll = np.asarray(loglik)[:, chain]
+    lp = np.asarray(logpost)[:, chain]
     lines = ["step,logLikelihood," + ",".join(names)]
     for it in range(th.shape[0]):
-        lines.append(f"{it},{float(ll[it])!r}," + ",".join(repr(float(x)) for x in th[it]))
+        lines.append(f"{it},{_num(lp[it])}," + ",".join(_num(x) for x in th[it]))
+    with open(path, "w") as f:
+        f.write("\n".join(lines))
+
+
+def write_candidates_csv(path: str, theta: np.ndarray, logpost: np.ndarray, accepted: np.ndarray, chain: int,
+                         names=PARAM_NAMES):
+    """One chain's candidates in the layout of MCMC::writeCandidates
+    (MCMC.cpp:443-484): header "step,logLikelihood,<names>,accepted", then
+    per iteration the candidate's log posterior (-inf outside the prior
+    support, where the reference repeats its last value; nan when a
+    delayed-acceptance first stage stopped it), its parameters and 1 if the
+    chain accepted it.  theta and logpost as from Mcmc.candidates(),
+    accepted from Mcmc.history()."""
+    th = np.asarray(theta)[:, chain, :]
+    lp = np.asarray(logpost)[:, chain]
+    acc = np.asarray(accepted)[:, chain]
+    lines = ["step,logLikelihood," + ",".join(names) + ",accepted"]
+    for it in range(th.shape[0]):
+        lines.append(f"{it},{_num(lp[it])}," + ",".join(_num(x) for x in th[it]) + f",{int(acc[it])}")
+    with open(path, "w") as f:
+        f.write("\n".join(lines))
+
+
+# Columns of parameterData.csv (Parameter::writeParameters, parameter.cpp:8-47).
+PARAMETER_DATA_COLUMNS = (
+    "MCMCStep", "nSize", "minMosToHuman", "subPopTotal", "commSize", "bitingRate", "mnBitingSuccess",
+    "nmBitingSuccess", "EIPRange", "EIP", "IIP", "Recovery", "noDistance", "EIRate", "longDistance", "initialSP",
+    "mDemoScale", "mDemoShape", "nDemoScale", "nDemoShape", "nInfantScale", "nInfantShape", "nInfToDemoLoc",
+    "netPower", "netM", "netSeed", "nHeteroPower", "mHeteroPower")
+
+
+def parameter_data_row(step: int, params: abi.Params, humans, climate=None, biting_rate: float = 0.25) -> list:
+    """One row of parameterData.csv for the simulations of a proposal.  The
+    engine's parameters map onto the reference's SimulationParam fields:
+    bitingRate x success = beta (nmBitingSuccess from beta_hm,
+    mnBitingSuccess from beta_mh, at the climate's or the given biting
+    rate), EIP = eip, IIP = the incubation midpoint, the lifespans, noDistance,
+    externalInfection, longDistance, initialSeroPrev = sero_prev; minMosToHuman
+    is the climate's or m_ratio.  Fields with no counterpart in this engine
+    print the reference's defaults (SimulationParam.cpp:11-47): exIncPeriodRange
+    2, netPower 1, netM 1, netSeed 1, nHeteroPopSize 0, mHeteroPopSize 0."""
+    hum = np.asarray(humans)
+    n = int(hum.sum())
+    C_ = int(hum.size)
+    br = float(climate.biting_rate) if climate is not None else biting_rate
+    mmh = float(climate.min_mos_to_human) if climate is not None else float(params.m_ratio)
+    return [step, n, mmh, C_, n / C_, br, params.beta_mh / br, params.beta_hm / br, 2.0, params.eip,
+            0.5 * (params.iip_min + params.iip_max), params.recovery, params.no_distance,
+            params.external_infection, params.long_distance, params.sero_prev, params.m_scale, params.m_shape,
+            params.h_scale, params.h_shape, params.h_infant_scale, params.h_infant_shape, params.h_loc,
+            1.0, 1.0, 1.0, 0.0, 0.0]
+
+
+def write_parameter_data_csv(path: str, rows: list):
+    """parameterData.csv in the layout of Parameter::writeParameters
+    (parameter.cpp:8-47): the header, then one line per MCMC step whose
+    candidate was simulated (rows from parameter_data_row)."""
+    lines = [",".join(PARAMETER_DATA_COLUMNS)]
+    for r in rows:
+        lines.append(",".join([str(int(r[0]))] + [_num(x) for x in r[1:]]))
     with open(path, "w") as f:
         f.write("\n".join(lines))
 

@@ -152,7 +152,7 @@ def mcmc_bench(args) -> dict:
     # surrogate itself, so every surrogate move is kept
     g, humans = configs.mcmc_shape()
     cfg = mcmc.McmcConfig(n_chains=args.chains, n_weeks=len(data), priors=configs.mcmc_priors(),
-                          adapt_after=100 * mcmc.D, nb_size=configs.MCMC_NB_SIZE, seed=99)
+                          adapt_after=100 * mcmc.D, seed=99)
     pilot = mcmc.Mcmc(cfg, g, g, humans, configs.base_params(), data, lib=lib)
 
     def sur_loglik(th):

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define ARBO_ABI_VERSION 5u
+#define ARBO_ABI_VERSION 6u
 #define ARBO_MAX_SEROTYPES 4u
 #define ARBO_MAX_COMMUNITY_HUMANS 8192u
 #define ARBO_MAX_COMMUNITY_SLOTS 8192u

@@ -356,7 +356,7 @@ void build_network(Sim& s) {
     }
 }
 
-int validate(ArboEngine* e, const ArboShape& sh, const ArboParams& p) {
+int validate(ArboEngine* e, const ArboShape& sh, const ArboParams& p, uint32_t max_days) {
     if (p.n_serotypes < 1 || p.n_serotypes > ARBO_MAX_SEROTYPES) return fail(e, ARBO_E_INVALID, "n_serotypes must be 1..4");
     if (sh.grid_w < 1 || sh.grid_h < 1 || !sh.humans) return fail(e, ARBO_E_INVALID, "empty lattice");
     if ((uint64_t)sh.grid_w * sh.grid_h > (1u << 24)) return fail(e, ARBO_E_INVALID, "too many communities");
@@ -367,6 +367,23 @@ int validate(ArboEngine* e, const ArboShape& sh, const ArboParams& p) {
     if (!(p.recovery >= 0.0f && p.recovery < 1000.0f) || !(p.eip >= 0.0f && p.eip < 1000.0f)) return fail(e, ARBO_E_INVALID, "recovery/eip out of range");
     if (p.iip_max < p.iip_min || p.iip_max > 1000) return fail(e, ARBO_E_INVALID, "bad incubation range");
     if (!(p.m_scale > 0.0f) || !(p.m_shape > 0.0f) || !(p.h_scale > 0.0f) || !(p.h_shape > 0.0f)) return fail(e, ARBO_E_INVALID, "bad lifespan parameters");
+    // integer lifetimes end at the tables' last ages: the tail past them must be
+    // below the 24-bit resolution of the lifetime draws, F >= 1 - 2^-24 there
+    if (to24(cdf_h(p, (double)(HUMAN_AGE_TABLE - 1))) < ONE24 - 1)
+        return fail(e, ARBO_E_INVALID, "human lifespan beyond the 65,535-day lifetime table");
+    if (to24(cdf_m(p, (double)(MOSQ_AGE_TABLE - 1))) < ONE24 - 1)
+        return fail(e, ARBO_E_INVALID, "mosquito lifespan beyond the 2,047-day lifetime table");
+    if (sh.climate && !clim_invalid(*sh.climate, max_days)) {
+        // every birth day's first-passage row lies above F(a; the largest daily scale)
+        double smax = 0.0;
+        for (uint32_t r = 1; r <= max_days; ++r) {
+            double d[4];
+            clim_day(*sh.climate, r, d);
+            smax = std::max(smax, d[3]);
+        }
+        if (to24(1.0 - exp_d(-pw((double)(MOSQ_AGE_TABLE - 1) / smax, (double)p.m_shape))) < ONE24 - 1)
+            return fail(e, ARBO_E_INVALID, "climate gives a mosquito lifespan beyond the 2,047-day lifetime table");
+    }
     if (!(p.m_ratio >= 0.0f)) return fail(e, ARBO_E_INVALID, "m_ratio must be >= 0");
     if (!(p.external_infection >= 0.0f)) return fail(e, ARBO_E_INVALID, "external_infection must be >= 0");
     if (!(p.seed_frac_h >= 0.0f && p.seed_frac_h <= 1.0f) || !(p.seed_frac_m >= 0.0f && p.seed_frac_m <= 1.0f)) return fail(e, ARBO_E_INVALID, "seed fractions must be in [0,1]");
@@ -838,7 +855,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes,
     if (max_days == 0 || max_days > (1u << 23)) return fail(e, ARBO_E_INVALID, "max_days must be 1..2^23");
     e->ready = false;
     for (uint32_t i = 0; i < n_sims; ++i) {
-        int rc = validate(e, shapes[i], params[i]);
+        int rc = validate(e, shapes[i], params[i], max_days);
         if (rc) return rc;
     }
     e->sims.clear();

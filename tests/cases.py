@@ -79,7 +79,7 @@ def case_climate():
     ca = configs.seasonal_climate(T + 1, biting_rate=0.5)
     b = configs.base_params(seed_frac_h=0.01, seed_frac_m=0.01)
     cb = configs.seasonal_climate(T + 1, t_amp=6.0, h_amp=0.2, p_amp=0.6, phase=1.0, biting_rate=0.8,
-                                  mos_net_growth=0.3, scale_mle=2.0)
+                                  mos_net_growth=0.3, scale_mle=3.0)
     cb.humidity = np.where(np.arange(T + 1) % 3 == 0, cb.humidity, cb.humidity * 0.5).astype(np.float32)
     return [SimSpec(5, 5, np.full(25, 300, np.uint32), a, 31, climate=ca),
             SimSpec(4, 4, np.full(16, 500, np.uint32), b, 32, climate=cb)], T

@@ -25,7 +25,8 @@ pytestmark = pytest.mark.gpu
 def test_delayed_acceptance_keeps_the_target(hip_lib, mode):
     lib = mcmc.bind(hip_lib)
     W = 4
-    priors = [(0, 0.0, 1.0), (0, 0.0, 1.0), (0, -3.0, -1.0), (0, 0.0, 1.0), (1, 0.0, 2 * math.pi), (0, 0.01, 1.0)]
+    priors = [(0, 0.0, 1.0), (0, 0.0, 1.0), (0, -3.0, -1.0), (0, 0.0, 1.0), (1, 0.0, 2 * math.pi), (0, 0.01, 1.0),
+              (mcmc.PRIOR_FIXED, 0.1, 0.0)]
     mean = np.array([0.5, 0.4, -2.0, 0.6, math.pi, 0.5])
     sd = np.array([0.06, 0.05, 0.15, 0.07, 0.3, 0.08])
     rng = np.random.default_rng(3)
@@ -36,7 +37,7 @@ def test_delayed_acceptance_keeps_the_target(hip_lib, mode):
     prec = np.linalg.inv(cov)
 
     def target(th):
-        d = th - mean
+        d = th[:, :6] - mean
         return -0.5 * np.einsum("ni,ij,nj->n", d, prec, d)
 
     print(f"[da] exactness mode={mode}", flush=True)
@@ -47,10 +48,11 @@ def test_delayed_acceptance_keeps_the_target(hip_lib, mode):
     sur.set_params(p)
     cfg = mcmc.McmcConfig(n_chains=64, n_weeks=W, priors=priors, adapt_after=300, seed=8)
     m = mcmc.Mcmc(cfg, 1, 1, [10], configs.base_params(), np.array([3, 9, 7, 12], np.uint32),
-                  theta0=np.tile(mean, (64, 1)), lib=lib)
+                  theta0=np.tile(np.append(mean, 0.1), (64, 1)), lib=lib)
     m.run_da_target(target, sur, 2500, sub_steps=3, step0=0.5, mode=mode)
     th, _, acc = m.history()
-    post = th[300:]
+    assert np.all(th[:, :, 6] == 0.1)
+    post = th[300:, :, :6]
     samples = post.reshape(-1, 6)
     n_eff = np.array([ess(post[:, :, i]) for i in range(6)])
     se = sd / np.sqrt(n_eff)
@@ -113,7 +115,7 @@ def test_trained_surrogate_saves_second_stage_evaluations(hip_lib):
     from scipy import stats
 
     lib = mcmc.bind(hip_lib)
-    W, r, skip = 26, 10.0, 1
+    W, r, skip = 26, 1.0 / configs.MCMC_DISPERSION, 1
     priors = configs.mcmc_priors()
     truth = configs.MCMC_TRUTH
     rng = np.random.default_rng(21)
@@ -145,17 +147,18 @@ def test_trained_surrogate_saves_second_stage_evaluations(hip_lib):
     assert rel < 0.05
 
     n_it, burn, chains = 800, 200, 32
-    theta0 = np.tile(truth, (chains, 1)) + rng.normal(scale=0.002, size=(chains, 6))
+    theta0 = np.tile(truth, (chains, 1))
+    theta0[:, :6] += rng.normal(scale=0.002, size=(chains, 6))
     out = {}
     for name in ("plain", "screened"):
-        cfg = mcmc.McmcConfig(n_chains=chains, n_weeks=W, priors=priors, adapt_after=1000, nb_size=r,
-                              skip_weeks=skip, seed=17)
+        cfg = mcmc.McmcConfig(n_chains=chains, n_weeks=W, priors=priors, adapt_after=1000, skip_weeks=skip,
+                              seed=17)
         m = mcmc.Mcmc(cfg, 1, 1, [10], configs.base_params(), data, theta0=theta0, lib=lib)
         if name == "plain":
             m.run_target(loglik_for(name), n_it)
         else:
             m.run_da_target(loglik_for(name), sur, n_it, mode=mcmc.DA_SCREEN)
-        th = m.history()[0][burn:]
+        th = m.history()[0][burn:, :, :6]
         th[:, :, 4] = np.mod(th[:, :, 4] - truth[4] + math.pi, 2 * math.pi) + truth[4] - math.pi
         e = np.array([ess(th[:, :, i]) for i in range(6)])
         out[name] = (th.reshape(-1, 6).mean(axis=0), th.reshape(-1, 6).std(axis=0), e, counts[name])
