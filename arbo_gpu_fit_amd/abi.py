@@ -110,6 +110,7 @@ PROTOTYPES = {
     "arbo_climate_day": (C.c_int, [C.POINTER(Climate), _u32, C.POINTER(C.c_double)]),
     "arbo_kernel_time": (C.c_int, [EnginePtr, _u32, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "arbo_reset_timing": (C.c_int, [EnginePtr]),
+    "arbo_scan_counts": (C.c_int, [EnginePtr, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "arbo_tuning_candidates": (C.c_int, [C.POINTER(Tuning), _u32, _u32p]),
     "arbo_set_tuning": (C.c_int, [EnginePtr, C.POINTER(Tuning)]),
     "arbo_get_tuning": (C.c_int, [EnginePtr, C.POINTER(Tuning)]),

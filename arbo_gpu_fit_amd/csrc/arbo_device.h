@@ -23,7 +23,9 @@ constexpr uint32_t kAoiRecord = 5000;     // ages per age-of-infection ring
 // Per-community partials of a day, written by the demography kernel and
 // summed per simulation by k_day_reduce: word 0 mosquitoes alive after the
 // births, 1 humans, 2-3 infectious humans (two 16-bit serotype pairs), 4-5
-// E->I onsets (pairs), 8-23 the eight long-distance rates or counts (u64).
+// E->I onsets (pairs), 6-7 the 16-byte census vectors of humans and of
+// mosquitoes the day's scan read, 8-23 the eight long-distance rates or
+// counts (u64).
 constexpr uint32_t kPartWords = 24;
 constexpr uint32_t kMaxReduceSplit = 64;  // workgroups per simulation of k_day_reduce: about one
                                           // community per thread, 1..64, about 2,048 in all
@@ -129,6 +131,8 @@ struct EngineView {
                                    // 2^-24 units (Poisson counts) or the count (stochastic rounding)
     uint32_t* series;
     uint32_t* part;                // [G][kPartWords] per-community partials of the day (see kPartWords)
+    unsigned long long* scanned;   // [n_sims][2] census vectors read by the demography kernel (humans,
+                                   // mosquitoes), summed over days by k_day_reduce
     uint32_t* spatial;             // per-community infectious humans [(T+1)][S][C] per simulation, or null
     int32_t* hist;                 // per human 4 infection-history words (one per serotype), or null
     uint8_t* aoi_mark;             // per human: 1 when a mosquito event infected it today (age-of-infection

@@ -63,7 +63,7 @@ struct ArboEngine {
     DevBuf<int32_t> h_birth, hist;
     DevBuf<uint8_t> aoi_mark;
     DevBuf<uint32_t> aoi, aoi_pos;
-    DevBuf<unsigned long long> lam;
+    DevBuf<unsigned long long> lam, scanned;
     DevBuf<uint32_t> part;   // per-community partials of the day
     DevBuf<float> dens;
     uint64_t series_total = 0;
@@ -129,7 +129,7 @@ void free_batch(ArboEngine* e) {
     dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->active); dev_free(e->hsum); dev_free(e->series); dev_free(e->spatial); dev_free(e->csr_thr); dev_free(e->csr_inbox);
     dev_free(e->in_edge); dev_free(e->in_slot); dev_free(e->tables); dev_free(e->day_ctr);
     dev_free(e->h_next); dev_free(e->m_word); dev_free(e->m_next); dev_free(e->h_birth); dev_free(e->hist); dev_free(e->aoi_mark); dev_free(e->aoi); dev_free(e->aoi_pos);
-    dev_free(e->lam); dev_free(e->dens); dev_free(e->part);
+    dev_free(e->lam); dev_free(e->scanned); dev_free(e->dens); dev_free(e->part);
     e->ready = false;
 }
 
@@ -413,7 +413,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         (rc = dev_alloc(e, e->aoi_pos, (e->flags & ARBO_F_AOI) ? (size_t)n_sims * 4 : 0)) || (rc = dev_alloc(e, e->m_word, m_tot)) ||
         (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kInboxEdges * 8)) || (rc = dev_alloc(e, e->nld, comm.size() * 8)) ||
         (rc = dev_alloc(e, e->hsum, comm.size() * 4)) || (rc = dev_alloc(e, e->active, comm.size())) ||
-        (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->part, comm.size() * kPartWords)) || (rc = dev_alloc(e, e->series, ser_tot)) || (rc = dev_alloc(e, e->spatial, spa_tot)) ||
+        (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->scanned, (size_t)2 * n_sims)) || (rc = dev_alloc(e, e->part, comm.size() * kPartWords)) || (rc = dev_alloc(e, e->series, ser_tot)) || (rc = dev_alloc(e, e->spatial, spa_tot)) ||
         (rc = dev_alloc(e, e->csr_thr, csr_thr.size())) || (rc = dev_alloc(e, e->csr_inbox, csr_inbox.size())) || (rc = dev_alloc(e, e->in_edge, in_edge.size())) ||
         (rc = dev_alloc(e, e->in_slot, in_slot.size())) || (rc = dev_alloc(e, e->tables, table_pool.size())) ||
         (rc = dev_alloc(e, e->dens, dens_pool.size())) || (rc = dev_alloc(e, e->day_ctr, 1))) {
@@ -431,6 +431,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     HIP_TRY(e, launch_fill(e->series.p, ser_tot, 0u, e->stream));
     if (spa_tot) HIP_TRY(e, launch_fill(e->spatial.p, spa_tot, 0u, e->stream));
     HIP_TRY(e, launch_fill(reinterpret_cast<uint32_t*>(e->lam.p), (uint64_t)2 * n_sims * 8 * 2, 0u, e->stream));
+    HIP_TRY(e, launch_fill(reinterpret_cast<uint32_t*>(e->scanned.p), (uint64_t)2 * n_sims * 2, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->day_ctr.p, 1, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->nld.p, comm.size() * 8, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->hsum.p, comm.size() * 4, 0u, e->stream));   // every community scans its humans on day 0
@@ -462,6 +463,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.nld = e->nld.p;
     v.lam = e->lam.p;
     v.part = e->part.p;
+    v.scanned = e->scanned.p;
     v.series = e->series.p;
     v.spatial = (e->flags & ARBO_F_SPATIAL) ? e->spatial.p : nullptr;
     v.hist = (e->flags & (ARBO_F_HISTORY | ARBO_F_AOI)) ? e->hist.p : nullptr;
@@ -790,6 +792,27 @@ int arbo_reset_timing(ArboEngine* e) {
     if (rc) return rc;
     e->kernel_ms[0] = e->kernel_ms[1] = 0.0;
     e->kernel_launches[0] = e->kernel_launches[1] = 0;
+    if (e->ready) {
+        HIP_TRY(e, hipSetDevice(e->device));
+        HIP_TRY(e, launch_fill(reinterpret_cast<uint32_t*>(e->scanned.p), (uint64_t)e->scanned.n * 2, 0u, e->stream));
+    }
+    return ARBO_OK;
+}
+
+int arbo_scan_counts(ArboEngine* e, uint64_t* human_vectors, uint64_t* mosquito_vectors) {
+    if (!e) return ARBO_E_INVALID;
+    if (!e->ready) return fail(e, ARBO_E_STATE, "arbo_batch_init has not been called");
+    HIP_TRY(e, hipSetDevice(e->device));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    std::vector<unsigned long long> v(e->scanned.n);
+    HIP_TRY(e, hipMemcpy(v.data(), e->scanned.p, v.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    uint64_t h = 0, m = 0;
+    for (size_t i = 0; i + 1 < v.size(); i += 2) {
+        h += v[i];
+        m += v[i + 1];
+    }
+    if (human_vectors) *human_vectors = h;
+    if (mosquito_vectors) *mosquito_vectors = m;
     return ARBO_OK;
 }
 

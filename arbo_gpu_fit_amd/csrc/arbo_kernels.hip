@@ -721,8 +721,9 @@ void k_day_demography(EngineView v) {
     const uint32_t row = t + 1;
     if (lane == 0)
         reinterpret_cast<uint4*>(v.part + (size_t)g * kPartWords)[0] = make_uint4(Mc, cd.n, ih01, ih23);
-    else if (lane == 1)
-        reinterpret_cast<uint2*>(v.part + (size_t)g * kPartWords + 4)[0] = make_uint2(inc01, inc23);
+    else if (lane == 1)   // with the census vectors this scan read (the exact skips leave the rest)
+        reinterpret_cast<uint4*>(v.part + (size_t)g * kPartWords + 4)[0] =
+            make_uint4(inc01, inc23, scan_h ? nvec : 0u, mlim);
     if (lane < S) {
         const uint32_t ih = unpack_pair(ih01, ih23, lane);
         if (cd.c == 0) v.series[si.n_one + (size_t)row * S + lane] = ih;
@@ -1082,14 +1083,19 @@ __global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split
     const SimDesc& sd = v.sims[sim];
     __shared__ unsigned long long s_fx[8];
     __shared__ uint32_t s_cnt[4][10];   // per wave: humans, mosquitoes, infectious[4], onsets[4]
+    __shared__ unsigned long long s_scan[2];
     if (tid < 8) s_fx[tid] = 0ull;
+    if (tid < 2) s_scan[tid] = 0ull;
     __syncthreads();
     uint32_t nh = 0, nm = 0, inf[4] = {0, 0, 0, 0}, inc[4] = {0, 0, 0, 0};
+    uint32_t sv_h = 0, sv_m = 0;   // census vectors scanned (fits 32 bits: at most 64 communities per thread)
     unsigned long long fx[8] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
     for (uint32_t c = part * 256 + tid; c < sd.C; c += split * 256) {
         const uint32_t* pw = v.part + (size_t)(sd.comm_off + c) * kPartWords;
         const uint4 a = reinterpret_cast<const uint4*>(pw)[0];
-        const uint2 b = reinterpret_cast<const uint2*>(pw + 4)[0];
+        const uint4 b = reinterpret_cast<const uint4*>(pw + 4)[0];
+        sv_h += b.z;
+        sv_m += b.w;
         nm += a.x;
         nh += a.y;
         inf[0] += a.z & 0xFFFFu; inf[1] += a.z >> 16; inf[2] += a.w & 0xFFFFu; inf[3] += a.w >> 16;
@@ -1111,6 +1117,13 @@ __global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split
         for (uint32_t d = kWave / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, kWave);
         if (lane == 0 && x) atomicAdd(&s_fx[k], x);
     }
+    {
+        const unsigned long long h = wave_sum(sv_h), m = wave_sum(sv_m);
+        if (lane == 0) {
+            atomicAdd(&s_scan[0], h);
+            atomicAdd(&s_scan[1], m);
+        }
+    }
     __syncthreads();
     const uint32_t S = sd.S, row = t + 1;
     const SeriesIdx si = series_index(sd.series_off, v.max_days, S);
@@ -1126,6 +1139,7 @@ __global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split
     }
     if (tid >= 32 && tid < 40 && s_fx[tid - 32])
         atomicAdd(&v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + (tid - 32)], s_fx[tid - 32]);
+    if (tid >= 64 && tid < 66 && s_scan[tid - 64]) atomicAdd(&v.scanned[2 * (size_t)sim + (tid - 64)], s_scan[tid - 64]);
 }
 
 // Age-of-infection rings (nepidemic.cu:229-243), one workgroup per

@@ -251,6 +251,13 @@ class Engine:
     def reset_timing(self):
         self._check(self.lib.arbo_reset_timing(self._h))
 
+    def scan_counts(self) -> tuple[int, int]:
+        """16-byte census vectors of humans and of mosquitoes the demography
+        kernel read since init or the last reset_timing()."""
+        h, m = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.arbo_scan_counts(self._h, C.byref(h), C.byref(m)))
+        return h.value, m.value
+
     def tuning_candidates(self) -> list[dict]:
         n = C.c_uint32()
         self._check(self.lib.arbo_tuning_candidates(None, 0, C.byref(n)))

@@ -1,42 +1,48 @@
-"""Benchmark: person-days per second of the batched dengue-IBM day loop on one
-MI355X.
+"""Benchmark: person-days per second of the dengue-IBM day loop on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--sims 256] [--no-cpu] [--autotune-days D]
-    python bench.py --workload plain     # the single 100x100-lattice simulation instead
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--sims 256] [--no-cpu] [--no-plain]
+    python bench.py --workload plain     # only the single 100x100-lattice simulation
+    python bench.py --mcmc               # MCMC ESS/hour, plain vs surrogate delayed acceptance
 
-Workload: the 256-simulation batch (communities log-uniform 400..10,000 squared
-to a lattice, 200..2,000 humans per community, per-simulation transmission,
-long-distance and seasonal parameters, 1,825 days).  One step = one simulated
-day of every simulation of the batch: the demography kernel and the
-event-application kernel over every community.  W warm-up days, then K timed
-days (W + K = 1,825 by default, the whole configured run).  The first warm-up
-days time every candidate launch tuning for D days each and keep the fastest
-(results do not depend on the tuning).  Inputs are synthetic and resident in
-HBM before timing starts.
+Headline workload: the 256-simulation batch (communities log-uniform
+400..10,000 squared to a lattice, 200..2,000 humans per community,
+per-simulation transmission, long-distance and seasonal parameters, 1,825 days
+configured).  One step = one simulated day of every simulation of the batch:
+the demography, reduce, long-distance and event-application kernels over
+every community.  W warm-up days, then K timed days (W + K <= 1,825; by
+default 25 and 1,800).  With --autotune-days D and at least 10 D warm-up days
+the warm-up first times every launch tuning for D days each and keeps the
+fastest (results do not depend on the tuning).  Inputs are synthetic and
+resident in HBM before timing starts; `value` is person-days per second over
+the timed days, whole job.
 
-With --workload plain: one simulation on the 100x100 lattice (1e7 humans,
-Beta(2,5) initial immunity, 3,650 days), the day loop replayed as hipGraphs
-of 8 days (10,000 communities: one wave each, too few to fill the GPU); the
-roofline comes from a second run of the same days with per-kernel HIP
-events.
+The same line carries a `plain` record: the single simulation of the
+100x100 lattice (1e7 humans, Beta(2,5) initial immunity, 3,650 days), its day
+loop replayed as hipGraphs of 8 days, timed in the same run (50 warm-up days,
+3,600 timed), with its own roofline and CPU baseline.
 
-The JSON line carries:
-  roofline       the demography kernel (the dominant launch of the day) against
-                 HBM bandwidth.  Algorithmic bytes per launch = 4 B per human
-                 (the packed census word, read) + 4 B per mosquito slot (the
-                 packed 16-bit word, read and written back) + 32 B per
-                 (community, serotype) of counters; `achieved` divides them by
-                 the kernel's mean launch time, measured with HIP events on the
-                 engine's stream.  `traffic` is the kernel's measured HBM bytes
-                 per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated
-                 against a plain copy), read from the committed profile summary
-                 of this same command;
-  step_roofline  the whole day step priced at 7 B per human and per mosquito
-                 slot + 32 B per (community, serotype), the per-person-day
-                 figure of a census that stores an age, over the measured
-                 ms_per_step;
-  cpu_baseline   the CPU oracle (OpenMP, and serial beside it) on a bounded
-                 sample of the same workload, on this machine's host cores.
+roofline (per workload): the demography kernel, the dominant launch of the
+day, against HBM bandwidth.
+  * Algorithmic bytes of a launch are the census and per-community bytes the
+    kernel's algorithm must move, counted from what it scanned
+    (arbo_scan_counts): 16 B per human census vector read (4 packed words;
+    a community's humans are skipped, exactly, on days before its earliest
+    death or state change) + 32 B per mosquito vector (8 packed slots, read
+    and written back; slots above the highest alive one are skipped) + 4 B
+    per 32 mosquito slots of alive bitmap written + per community 64 B of
+    descriptor, 32 B of summary read and written, 96 B of partials and 40 B
+    per CSR entry (its row read, its inbox counts written).  `achieved`
+    divides them by the kernel's mean launch time, measured with HIP events
+    on the engine's stream.
+  * `traffic` is the kernel's HBM bytes per launch from rocprofv3's
+    FETCH_SIZE and WRITE_SIZE passes over this same command (calibrated
+    against a plain copy), read from the committed profile; `traffic_frac`
+    is that rate over the peak.  `full_scan_bytes` prices a scan of every
+    member every day (4 B per human, 4 B per slot, 32 B per community and
+    serotype) for comparison.
+cpu_baseline: the CPU oracle built -O3 -march=native on this host, OpenMP
+(and serial beside it), resumed from the GPU's census at the first timed
+day and run over the timed days (at most 60) for a sample of simulations.
 """
 from __future__ import annotations
 
@@ -52,63 +58,234 @@ if ROOT not in sys.path:
 
 METRIC = "person-days/sec on 1 MI355X (plain, batched); MCMC ESS/hour plain vs surrogate"
 HBM_PEAK_GBS = 8000.0
-PROFILE = os.path.join("profiles", "k_day_demography_batch256.md")
-
-
-def profiled_traffic() -> float | None:
-    """HBM bytes per launch of the demography kernel from the committed
-    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated)."""
-    try:
-        with open(os.path.join(ROOT, PROFILE)) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":")[1])
-    except OSError:
-        pass
-    return None
+PROFILES = {"batch": os.path.join("profiles", "k_day_demography_batch256.md"),
+            "plain": os.path.join("profiles", "k_day_demography_plain100.md")}
 
 
 def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def _oracle_rate(lib, specs, days: int, threads: int):
+def profiled_traffic(workload: str, warmup: int, steps: int) -> float | None:
+    """HBM bytes per launch of the demography kernel from the committed
+    rocprofv3 summary of this workload and day window (FETCH_SIZE and
+    WRITE_SIZE passes, copy-calibrated): a line
+    `traffic workload=<w> warmup=<W> steps=<K> bytes_per_launch=<B>`."""
+    try:
+        with open(os.path.join(ROOT, PROFILES[workload])) as f:
+            for line in f:
+                if not line.startswith("traffic "):
+                    continue
+                kv = dict(x.split("=", 1) for x in line.split()[1:])
+                if kv.get("workload") == workload and int(kv["warmup"]) == warmup and int(kv["steps"]) == steps:
+                    return float(kv["bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
+
+
+def row_lengths(spec) -> list[int]:
+    """CSR entries of every community of a simulation: its lattice
+    neighbours (Moore or von Neumann) plus itself when it is a dispersal
+    target or keeps a staying fraction."""
+    p = spec.params
+    own = 1 if (p.include_self or p.no_distance > 0.0) else 0
+    out = []
+    for y in range(spec.grid_h):
+        ny = 1 + (y > 0) + (y < spec.grid_h - 1)
+        for x in range(spec.grid_w):
+            nx = 1 + (x > 0) + (x < spec.grid_w - 1)
+            n = nx * ny - 1 if p.neighbourhood == 8 else (nx - 1) + (ny - 1)
+            out.append(max(1, n + own))
+    return out
+
+
+def host_cpu() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(specs, states, day: int, days: int, threads: int, serial_days: int, what: str) -> dict:
+    """The CPU oracle (-O3 -march=native, built here) resumed at `day` from
+    the GPU's census of the simulations `specs` (states[i]) and run `days`
+    days on `threads` OpenMP threads; simulation 0 alone on one thread for
+    `serial_days` days."""
     from arbo_gpu_fit_amd.engine import Engine
+    from tests.oracle_lib import load_native_oracle
 
-    eng = Engine(lib, threads=threads)
-    eng.init(specs, days)
-    t0 = time.perf_counter()
-    eng.step(days)
-    dt = time.perf_counter() - t0
-    eng.close()
-    humans = sum(int(s.humans.sum()) for s in specs)
-    return humans * days / dt, humans, dt
+    lib, flags = load_native_oracle()
+    T = day + max(days, serial_days)
 
+    def timed(sp, st, n, thr):
+        eng = Engine(lib, threads=thr)
+        eng.init(sp, T)
+        eng.set_day(day)
+        for i, s in enumerate(st):
+            eng.write_state(i, s)
+        t0 = time.perf_counter()
+        eng.step(n)
+        dt = time.perf_counter() - t0
+        eng.close()
+        humans = sum(int(s.humans.sum()) for s in sp)
+        return humans * n / dt, humans, dt
 
-def cpu_baseline(specs, days: int, threads: int, serial_days: int, what: str = "") -> dict:
-    """The CPU oracle on a bounded sample: the first simulations of the batch
-    (or `what`) for the first `days` days on `threads` OpenMP threads, and
-    simulation 0 alone on one thread for `serial_days` days."""
-    from tests.oracle_lib import load_oracle
-
-    lib = load_oracle()
-    rate, humans, dt = _oracle_rate(lib, specs, days, threads)
-    srate, shumans, sdt = _oracle_rate(lib, specs[:1], serial_days, 1)
+    rate, humans, dt = timed(specs, states, days, threads)
+    srate, shumans, sdt = timed(specs[:1], states[:1], serial_days, 1)
+    model = host_cpu()
     return {
         "value": rate,
         "unit": "person-days/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"CPU oracle (g++ -O2, OpenMP {threads} threads): "
-                  + (f"{what} ({humans} humans)" if what else
-                     f"simulations 0..{len(specs) - 1} of the batch ({humans} humans)")
-                  + f", days 0..{days - 1}, {dt:.1f} s",
+        "cpu": model,
+        "build": flags,
+        "sample": f"CPU oracle ({flags}, OpenMP {threads} threads, {model}): {what} ({humans} humans) resumed "
+                  f"from the GPU's census at day {day}, days {day}..{day + days - 1}, {dt:.1f} s",
         "serial": {
             "value": srate,
             "cores": 1,
-            "sample": f"simulation 0 ({shumans} humans), days 0..{serial_days - 1}, one thread, {sdt:.1f} s",
+            "sample": f"the first of them ({shumans} humans), days {day}..{day + serial_days - 1}, one thread, "
+                      f"{sdt:.1f} s",
         },
     }
+
+
+def sim_bench(workload: str, args, lib) -> dict:
+    """Person-days/s of one workload ("batch" or "plain") with its roofline
+    and CPU baseline."""
+    from arbo_gpu_fit_amd import abi, configs
+    from arbo_gpu_fit_amd.engine import Engine
+
+    plain = workload == "plain"
+    steps = args.plain_steps if plain else args.steps
+    warmup = args.plain_warmup if plain else args.warmup
+    t0 = time.perf_counter()
+    specs, days_cfg = configs.lattice100() if plain else configs.batch256(args.sims)
+    humans = sum(int(s.humans.sum()) for s in specs)
+    comms = sum(s.communities for s in specs)
+    sero_comms = sum(s.communities * s.params.n_serotypes for s in specs)
+    log(f"[bench] {workload}: {len(specs)} simulations, {comms} communities, {humans} humans "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    days = warmup + steps
+    if days > days_cfg:
+        raise SystemExit(f"{workload}: warm-up + steps = {days} exceeds the configured {days_cfg} days")
+    graph = plain or args.graph
+    eng = Engine(lib, device=0, flags=abi.F_GRAPH if graph else abi.F_TIMING)
+    t0 = time.perf_counter()
+    eng.init(specs, days)
+    slots = sum(eng.sizes(i)[2] for i in range(len(specs)))
+    log(f"[bench] {workload}: init {time.perf_counter() - t0:.1f} s; {slots} mosquito slots")
+
+    done = 0
+    n_cand = len(eng.tuning_candidates())
+    tune_ms = None
+    if args.autotune_days and n_cand * args.autotune_days <= warmup:
+        chosen, tune_ms = eng.autotune(args.autotune_days)
+        done = n_cand * args.autotune_days
+        log(f"[bench] {workload}: autotuned over {n_cand} launch tunings x {args.autotune_days} days: {chosen}")
+    if warmup > done:
+        eng.step(warmup - done)
+    eng.sync()
+    # the CPU baseline resumes from the GPU's census at the first timed day
+    cpu_sims = list(range(min(len(specs), args.cpu_sims if not plain else 1)))
+    states = [eng.state(i) for i in cpu_sims] if not args.no_cpu else []
+    eng.reset_timing()
+    log(f"[bench] {workload}: warm-up {warmup} days done")
+
+    t_start = time.perf_counter()
+    done = 0
+    last = t_start
+    while done < steps:
+        n = min(100, steps - done)
+        eng.step(n)
+        done += n
+        if time.perf_counter() - last > 20.0:
+            eng.sync()
+            last = time.perf_counter()
+            log(f"[bench] {workload}: {done}/{steps} days, {last - t_start:.1f} s")
+    eng.sync()
+    elapsed = time.perf_counter() - t_start
+    tuning = eng.tuning()
+    if graph:   # kernel means and scan counts from a second pass over the same days with per-kernel events
+        eng.close()
+        eng = Engine(lib, device=0, flags=abi.F_TIMING)
+        eng.init(specs, days)
+        eng.set_tuning(**tuning)
+        eng.step(warmup)
+        eng.sync()
+        eng.reset_timing()
+        eng.step(steps)
+        eng.sync()
+    ms_a, n_a = eng.kernel_time(0)
+    ms_b, _ = eng.kernel_time(1)
+    hv, mv = eng.scan_counts()
+    eng.close()
+
+    mean_a = ms_a / max(n_a, 1)
+    bitmap = 4 * ((slots + 31) // 32)
+    per_comm = sum(64 + 32 + 96 + 40 * rl for s in specs for rl in row_lengths(s))
+    alg = (16 * hv + 32 * mv) / max(n_a, 1) + bitmap + per_comm
+    achieved = alg / (mean_a * 1e-3) / 1e9
+    traffic = profiled_traffic(workload, warmup, steps)
+    full = 4 * humans + 4 * slots + 32 * sero_comms
+    value = humans * steps / elapsed
+    out = {
+        "value": value,
+        "unit": "person-days/s",
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": elapsed * 1e3 / steps,
+        "config": {
+            "workload": (f"plain single simulation: 100x100 lattice, {humans} humans (1,000 per community), "
+                         f"{slots} mosquito slots, Beta(2,5) initial immunity, days {warmup}..{days - 1} timed"
+                         if plain else
+                         f"batched plain simulation: {len(specs)} simulations, {comms} communities "
+                         f"(400-10,000 per simulation), {humans} humans (200-2,000 per community), "
+                         f"{slots} mosquito slots, days {warmup}..{days - 1} timed"),
+            "simulations": len(specs),
+            "communities": comms,
+            "humans": humans,
+            "mosquito_slots": slots,
+            "days": days,
+            "graph": graph,
+            "tuning": tuning,
+            "autotuned": tune_ms is not None,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_day_demography",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "traffic_frac": (traffic / (mean_a * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+            "traffic_source": PROFILES[workload] if traffic else None,
+            "bytes_per_launch": alg,
+            "census_vectors_per_launch": {"human": hv / max(n_a, 1), "mosquito": mv / max(n_a, 1)},
+            "full_scan_bytes": full,
+            "mean_launch_ms": mean_a,
+            "launches": n_a,
+        },
+        "kernels_ms": {"k_day_demography": ms_a, "k_day_events_and_rest": ms_b, "launches": n_a},
+    }
+    if graph:
+        out["roofline"]["pass"] = f"a second run of the same days ({warmup}..{days - 1}) with per-kernel HIP events"
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cdays = min(steps, 60)
+        log(f"[bench] {workload}: CPU oracle baseline on {threads} threads")
+        what = ("the 100x100-lattice simulation" if plain else
+                f"simulations 0..{len(cpu_sims) - 1} of the batch")
+        out["cpu_baseline"] = cpu_baseline([specs[i] for i in cpu_sims], states, warmup, cdays, threads,
+                                           min(cdays, 10 if plain else 20), what)
+    return out
 
 
 def mcmc_bench(args) -> dict:
@@ -153,7 +330,7 @@ def mcmc_bench(args) -> dict:
     g, humans = configs.mcmc_shape()
     cfg = mcmc.McmcConfig(n_chains=args.chains, n_weeks=len(data), priors=configs.mcmc_priors(),
                           adapt_after=100 * mcmc.D, seed=99)
-    pilot = mcmc.Mcmc(cfg, g, g, humans, configs.base_params(), data, lib=lib)
+    pilot = mcmc.Mcmc(cfg, g, g, humans, configs.base_params(), data, lib=lib)  # surrogate only
 
     def sur_loglik(th):
         lp, _ = pilot.surrogate_logpost(sur, th)
@@ -167,6 +344,7 @@ def mcmc_bench(args) -> dict:
                                  ("surrogate", sur, mcmc.DA_LANGEVIN, args.iters),
                                  ("screened", sur, mcmc.DA_SCREEN, args.screen_iters)):
         _, runs[name] = fit.run_fit(eng, data, iters, chains=args.chains, seed=args.mcmc_seed, sur=s,
+                                    n_particles=args.particles, n_blocks=args.blocks,
                                     sub_steps=args.sub_steps, theta0=theta0, da_mode=mode)
         log(f"[mcmc] {name}: ESS min {runs[name]['ess_min']:.1f}, {runs[name]['ess_per_hour']:.0f}/h, "
             f"R-hat max {runs[name]['rhat_max']:.3f}")
@@ -201,17 +379,18 @@ def mcmc_bench(args) -> dict:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed days (batch 1,800; plain 3,600)")
-    ap.add_argument("--warmup", type=int, default=None, help="warm-up days (batch 25; plain 50)")
-    ap.add_argument("--workload", choices=("batch", "plain"), default="batch")
+    ap.add_argument("--steps", type=int, default=1800, help="timed days of the batch")
+    ap.add_argument("--warmup", type=int, default=25, help="warm-up days of the batch")
+    ap.add_argument("--plain-steps", type=int, default=3600, help="timed days of the plain workload")
+    ap.add_argument("--plain-warmup", type=int, default=50, help="warm-up days of the plain workload")
+    ap.add_argument("--workload", choices=("batch", "plain"), default="batch",
+                    help="plain: only the 100x100-lattice simulation as the headline")
+    ap.add_argument("--no-plain", action="store_true", help="leave out the plain record")
     ap.add_argument("--sims", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sims", type=int, default=8)
-    ap.add_argument("--cpu-days", type=int, default=365)
-    ap.add_argument("--cpu-serial-days", type=int, default=365)
-    ap.add_argument("--graph", action="store_true", help="replay the day loop as hipGraphs (no per-kernel timing)")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="plain workload: launch every kernel (per-kernel HIP events) instead of graph replay")
+    ap.add_argument("--cpu-sims", type=int, default=0,
+                    help="simulations of the batch in the CPU sample (0: 64 for windows up to 60 days, else 8)")
+    ap.add_argument("--graph", action="store_true", help="batch: replay the day loop as hipGraphs")
     ap.add_argument("--autotune-days", type=int, default=2,
                     help="days per candidate launch tuning, timed within the warm-up (0: default tuning)")
     ap.add_argument("--mcmc", action="store_true", help="MCMC ESS/hour, plain vs surrogate, instead")
@@ -219,6 +398,8 @@ def main():
     ap.add_argument("--iters", type=int, default=160)
     ap.add_argument("--screen-iters", type=int, default=480, help="iterations of the screened delayed acceptance")
     ap.add_argument("--mcmc-seed", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=1, help="simulations per proposal")
+    ap.add_argument("--blocks", type=int, default=0, help="pseudo-marginal seed blocks (0: fresh seeds)")
     ap.add_argument("--train-runs", type=int, default=50000)
     ap.add_argument("--epochs", type=int, default=200)
     ap.add_argument("--sub-steps", type=int, default=10)
@@ -248,142 +429,34 @@ def main():
     if args.mcmc:
         print(json.dumps(mcmc_bench(args)), flush=True)
         return
-    plain = args.workload == "plain"
-    if args.steps is None:
-        args.steps = 3600 if plain else 1800
-    if args.warmup is None:
-        args.warmup = 50 if plain else 25
-    if plain and not args.no_graph:
-        args.graph = True
+    if args.cpu_sims == 0:
+        args.cpu_sims = 64 if args.steps <= 60 else 8
 
-    from arbo_gpu_fit_amd import abi, configs
-    from arbo_gpu_fit_amd.engine import Engine
+    from arbo_gpu_fit_amd import abi
 
     lib = abi.load_hip_library()
-    t0 = time.perf_counter()
-    specs, days_cfg = configs.lattice100() if plain else configs.batch256(args.sims)
-    humans = sum(int(s.humans.sum()) for s in specs)
-    comms = sum(s.communities for s in specs)
-    sero_comms = sum(s.communities * s.params.n_serotypes for s in specs)
-    log(f"[bench] generated {len(specs)} simulations, {comms} communities, {humans} humans "
-        f"in {time.perf_counter() - t0:.1f} s")
-    days = args.warmup + args.steps
-    flags = abi.F_GRAPH if args.graph else abi.F_TIMING
-    eng = Engine(lib, device=0, flags=flags)
-    t0 = time.perf_counter()
-    eng.init(specs, days)
-    slots = sum(eng.sizes(i)[2] for i in range(len(specs)))
-    log(f"[bench] init {time.perf_counter() - t0:.1f} s; {slots} mosquito slots")
-
-    done = 0
-    chunk = 100
-    n_cand = len(eng.tuning_candidates())
-    tune_ms = None
-    if args.autotune_days and n_cand * args.autotune_days <= args.warmup:
-        chosen, tune_ms = eng.autotune(args.autotune_days)
-        done = n_cand * args.autotune_days
-        log(f"[bench] autotuned over {n_cand} launch tunings x {args.autotune_days} days: {chosen}, "
-            f"{tune_ms:.3f} ms/day")
-    while done < args.warmup:
-        n = min(chunk, args.warmup - done)
-        eng.step(n)
-        done += n
-    eng.sync()
-    eng.reset_timing()
-    log(f"[bench] warm-up {args.warmup} days done")
-
-    t_start = time.perf_counter()
-    done = 0
-    last = t_start
-    while done < args.steps:
-        n = min(chunk, args.steps - done)
-        eng.step(n)
-        done += n
-        if time.perf_counter() - last > 20.0:
-            eng.sync()
-            last = time.perf_counter()
-            log(f"[bench] {done}/{args.steps} days, {last - t_start:.1f} s")
-    eng.sync()
-    elapsed = time.perf_counter() - t_start
-
-    person_days = humans * args.steps
-    value = person_days / elapsed
+    head = sim_bench(args.workload, args, lib)
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "person-days/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
+        "steps": head["steps"],
+        "warmup": head["warmup"],
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {
-            "workload": (f"plain single simulation: 100x100 lattice, {humans} humans (1,000 per community), "
-                         f"{slots} mosquito slots, Beta(2,5) initial immunity, {days} days" if plain else
-                         f"batched plain simulation: {len(specs)} simulations, {comms} communities "
-                         f"(400-10,000 per simulation), {humans} humans (200-2,000 per community), "
-                         f"{slots} mosquito slots, {days} days"),
-            "simulations": len(specs),
-            "communities": comms,
-            "humans": humans,
-            "mosquito_slots": slots,
-            "days": days,
-            "graph": bool(args.graph),
-            "tuning": eng.tuning(),
-            "autotuned": tune_ms is not None,
-        },
+        "config": head["config"],
+        "roofline": head["roofline"],
+        "kernels_ms": head["kernels_ms"],
     }
-    step_bytes = 7 * (humans + slots) + 32 * sero_comms
-    step_gbs = step_bytes / (elapsed / args.steps) / 1e9
-    out["step_roofline"] = {"bytes_per_step": step_bytes, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS}
-    if plain and args.graph:   # kernel means from a separate pass with per-kernel HIP events, same tuning
-        tun = eng.tuning()
-        eng.close()
-        eng = Engine(lib, device=0, flags=abi.F_TIMING)
-        eng.init(specs, days)
-        eng.set_tuning(**tun)
-        eng.step(args.warmup)
-        eng.sync()
-        eng.reset_timing()
-        eng.step(args.steps)   # the same days as the timed run
-        eng.sync()
-    if not args.graph or plain:
-        traffic = profiled_traffic() if (args.sims == 256 and not plain) else None
-        ms_a, n_a = eng.kernel_time(0)
-        ms_b, n_b = eng.kernel_time(1)
-        bytes_a = 4 * humans + 4 * slots + 32 * sero_comms
-        mean_a = ms_a / max(n_a, 1)
-        achieved = bytes_a / (mean_a * 1e-3) / 1e9
-        out["roofline"] = {
-            "bound": "hbm",
-            "kernel": "k_day_demography",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "bytes_per_launch": bytes_a,
-            "mean_launch_ms": mean_a,
-            "traffic_source": PROFILE if traffic is not None else None,
-        }
-        if plain and args.graph:
-            out["roofline"]["pass"] = (f"a second run of the same days ({args.warmup}..{days - 1}) with per-kernel "
-                                       "HIP events")
-        out["kernels_ms"] = {"k_day_demography": ms_a, "k_day_events": ms_b, "launches": n_a}
-    eng.close()
-
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"[bench] CPU oracle baseline on {threads} threads")
-        if plain:   # a bounded sample: 730 days on the threads, 60 days on one
-            out["cpu_baseline"] = cpu_baseline(specs, 730, threads, 60, what="the 100x100-lattice simulation")
-        else:
-            out["cpu_baseline"] = cpu_baseline(specs[:args.cpu_sims], args.cpu_days, threads, args.cpu_serial_days)
+    if "cpu_baseline" in head:
+        out["cpu_baseline"] = head["cpu_baseline"]
+    if args.workload == "batch" and not args.no_plain:
+        out["plain"] = sim_bench("plain", args, lib)
     print(json.dumps(out), flush=True)
 
 

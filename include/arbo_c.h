@@ -265,6 +265,17 @@ ARBO_API int arbo_climate_day(const ArboClimate* climate, uint32_t row, double* 
 ARBO_API int arbo_kernel_time(ArboEngine* e, uint32_t kind, double* ms, uint64_t* launches);
 ARBO_API int arbo_reset_timing(ArboEngine* e);
 
+/* Census traffic of the day loop since arbo_batch_init or the last
+ * arbo_reset_timing: the 16-byte vectors of packed human words (4 humans)
+ * and of packed mosquito words (8 slots) that the demography kernel read.
+ * The kernel skips exactly what cannot change on a day (a community's
+ * humans before its earliest death or state change, mosquito slots above
+ * its highest alive one), so these count the census bytes the algorithm
+ * needs; every mosquito vector read is also written back.  The oracle,
+ * which scans every member every day, reports the full census.  No
+ * counterpart in the reference (test and measurement only). */
+ARBO_API int arbo_scan_counts(ArboEngine* e, uint64_t* human_vectors, uint64_t* mosquito_vectors);
+
 /* Launch knobs of the day kernels.  No knob changes a result: every random
  * draw is keyed by model entities (simulation, community, individual, day,
  * event), never by workgroup or lane, so runs under any tuning are

@@ -112,6 +112,8 @@ struct ArboEngine {
     std::string err;
     ArboTuning tuning{4, 4, 0, 8, 0};   // launch knobs: accepted and reported, no effect on the CPU
     uint32_t flags = 0;
+    uint64_t scan_h = 0, scan_m = 0;    // census vectors of the days stepped (every member, every day)
+    uint64_t full_h = 0, full_m = 0;    // one full scan: 4-human and 8-slot vectors of every community
 };
 
 namespace {
@@ -943,6 +945,12 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes,
     }
     e->day = 0;
     e->max_days = max_days;
+    e->scan_h = e->scan_m = e->full_h = e->full_m = 0;
+    for (const Sim& s : e->sims)
+        for (uint32_t c = 0; c < s.C; ++c) {
+            e->full_h += (s.N[c] + 3) / 4;
+            e->full_m += (s.cap[c] + 7) / 8;
+        }
     e->ready = true;
     return ARBO_OK;
 }
@@ -954,6 +962,8 @@ int arbo_batch_step(ArboEngine* e, uint32_t n_days) {
     for (uint32_t k = 0; k < n_days; ++k) {
         day_step(e, e->day);
         e->day++;
+        e->scan_h += e->full_h;
+        e->scan_m += e->full_m;
     }
     return ARBO_OK;
 }
@@ -1065,7 +1075,20 @@ int arbo_kernel_time(ArboEngine* e, uint32_t, double* ms, uint64_t* launches) {
     return ARBO_OK;
 }
 
-int arbo_reset_timing(ArboEngine* e) { return e ? ARBO_OK : ARBO_E_INVALID; }
+int arbo_reset_timing(ArboEngine* e) {
+    if (!e) return ARBO_E_INVALID;
+    e->scan_h = e->scan_m = 0;
+    return ARBO_OK;
+}
+
+// The oracle scans every member every day: the full census per day.
+int arbo_scan_counts(ArboEngine* e, uint64_t* human_vectors, uint64_t* mosquito_vectors) {
+    if (!e) return ARBO_E_INVALID;
+    if (!e->ready) return fail(e, ARBO_E_STATE, "arbo_batch_init has not been called");
+    if (human_vectors) *human_vectors = e->scan_h;
+    if (mosquito_vectors) *mosquito_vectors = e->scan_m;
+    return ARBO_OK;
+}
 
 // Launch tunings: the same candidate set as the GPU library, so a caller's
 // tuning calls run unchanged; the CPU engine's results never depend on them.

@@ -42,3 +42,15 @@ def load_oracle():
     lib.oracle_dens.restype = C.c_int
     _lib = lib
     return lib
+
+
+def load_native_oracle():
+    """The oracle built -O3 -march=native on this machine (make native, into
+    oracle/_build_native), for timing the CPU baseline on the host that runs
+    it; (library, build flags).  Falls back to the portable build when the
+    native one cannot be built."""
+    path = os.path.join(ROOT, "oracle", "_build_native", "libarbo_oracle.so")
+    res = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "native"], capture_output=True, text=True)
+    if res.returncode != 0 or not os.path.exists(path):
+        return load_oracle(), "g++ -O3 -march=x86-64-v2"
+    return abi.bind(C.CDLL(path)), "g++ -O3 -march=native"
