@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -56,7 +57,8 @@ enum : uint32_t {
     EPI_STORE = 0,       // outf = acc
     EPI_BIAS = 1,        // outf = acc + bias
     EPI_BIAS_GELU = 2,   // outf = z = acc + bias; outh = bf16(gelu(z))
-    EPI_GELU_GRAD = 3    // outf = acc * gelu'(zprev); outh = bf16(outf)
+    EPI_GELU_GRAD = 3,   // outf = acc * gelu'(zprev); outh = bf16(outf)
+    EPI_GELU_ONLY = 4    // outh = bf16(gelu(acc + bias)) (forward passes that keep no z)
 };
 
 struct GemmArgs {
@@ -112,6 +114,9 @@ __global__ __launch_bounds__(64) void k_gemm(GemmArgs g) {
                 g.outh[(size_t)i * g.ldh + j] = f2bf(gelu(z));
                 break;
             }
+            case EPI_GELU_ONLY:
+                g.outh[(size_t)i * g.ldh + j] = f2bf(gelu(v + g.bias[j]));
+                break;
             default: {
                 const float d = v * gelu_grad(g.zprev[(size_t)i * g.ldz + j]);
                 g.outf[(size_t)i * g.ldf + j] = d;
@@ -203,6 +208,33 @@ struct Buf {
     size_t n = 0;
 };
 
+// Activations of arbo_sur_forward_noalloc: two ping-pong buffers of
+// kScratchElems bfloat16 values (4 MiB in all) that every GPU the library
+// runs on holds until the process ends.  Host code never uses the symbol
+// itself (on the host it only names the variable): it takes each device's
+// address once, from hipGetSymbolAddress with that device current, when a
+// surrogate is created there.
+constexpr size_t kScratchElems = size_t(1) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) uint16_t scratch[2 * kScratchElems];
+
+std::mutex g_scratch_mu;
+uint16_t* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch array's address on `device`, which must be current.
+hipError_t scratch_on(int device, uint16_t** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (st != hipSuccess) return st;
+        g_scratch_ptr[device] = static_cast<uint16_t*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
 }  // namespace
 }  // namespace arbo
 
@@ -226,6 +258,7 @@ struct ArboSurrogate {
     Buf<double> epoch_sum;          // the loss summed over one training epoch
     Buf<float> data_x, data_y, by;
     Buf<uint32_t> idx;
+    uint16_t* scratch = nullptr;    // this device's no-allocation activation buffers
     uint64_t adam_t = 0;
     std::string err;
 };
@@ -268,11 +301,13 @@ void release(Buf<T>& b) {
 
 uint32_t blocks(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
-hipError_t gemm(ArboSurrogate* s, const GemmArgs& g) {
+hipError_t gemm(hipStream_t st, const GemmArgs& g) {
     dim3 grid(blocks(g.N, kTile), blocks(g.M, kTile));
-    hipLaunchKernelGGL(k_gemm, grid, dim3(64), 0, s->stream, g);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(64), 0, st, g);
     return hipGetLastError();
 }
+
+hipError_t gemm(ArboSurrogate* s, const GemmArgs& g) { return gemm(s->stream, g); }
 
 GemmArgs args(uint32_t M, uint32_t N, uint32_t K, uint32_t epi) {
     GemmArgs g{};
@@ -417,7 +452,9 @@ ARBO_API int arbo_sur_create(int32_t device, uint32_t n_widths, const uint32_t* 
     s->w.assign(widths, widths + n_widths);
     s->L = n_widths - 1;
     s->max_batch = max_batch;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        scratch_on(device, &s->scratch) != hipSuccess) {
+        if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
         return ARBO_E_DEVICE;
     }
@@ -519,6 +556,42 @@ ARBO_API int arbo_sur_forward(ArboSurrogate* s, uint32_t n, const float* x, floa
     if (int rc = forward(s, n)) return rc;
     SUR_TRY(s, hipMemcpyAsync(out, s->Z[s->L - 1].p, (size_t)n * s->w[s->L] * 4, hipMemcpyDeviceToHost, s->stream));
     SUR_TRY(s, hipStreamSynchronize(s->stream));
+    return ARBO_OK;
+}
+
+ARBO_API int arbo_sur_forward_noalloc(ArboSurrogate* s, uint32_t n, const float* x, float* out, void* stream) {
+    if (!s || !x || !out || n == 0) return ARBO_E_INVALID;
+    uint32_t widest = 0;   // widest layer input: the activations held between layers
+    for (uint32_t l = 0; l < s->L; ++l) widest = std::max(widest, s->w[l]);
+    if ((uint64_t)n * widest > kScratchElems)
+        return fail(s, ARBO_E_CAPACITY, "rows x widest layer input exceed the 2^20 activations of the no-allocation scratch");
+    if (!s->scratch) return fail(s, ARBO_E_STATE, "no scratch address on this device");
+    SUR_TRY(s, hipSetDevice(s->device));
+    const hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+    uint16_t* buf[2] = {s->scratch, s->scratch + kScratchElems};
+    const uint64_t cnt = (uint64_t)n * s->w[0];
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(cnt, 256)), dim3(256), 0, st, x, buf[0], cnt);
+    SUR_TRY(s, hipGetLastError());
+    for (uint32_t l = 0; l < s->L; ++l) {   // the products of forward(), activations ping-ponged in the scratch
+        const uint32_t K = s->w[l], N = s->w[l + 1];
+        const bool last = l + 1 == s->L;
+        GemmArgs g = args(n, N, K, last ? EPI_BIAS : EPI_GELU_ONLY);
+        g.A = buf[l & 1];
+        g.sAi = K;
+        g.sAk = 1;
+        g.B = s->Wh[l].p;
+        g.sBj = K;
+        g.sBk = 1;
+        g.bias = s->P.p + s->boff[l];
+        if (last) {
+            g.outf = out;
+            g.ldf = N;
+        } else {
+            g.outh = buf[(l + 1) & 1];
+            g.ldh = N;
+        }
+        SUR_TRY(s, gemm(st, g));
+    }
     return ARBO_OK;
 }
 

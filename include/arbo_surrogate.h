@@ -44,6 +44,19 @@ ARBO_API int arbo_sur_set_params(ArboSurrogate* s, const float* in);
 /* out = f(x) for n rows. */
 ARBO_API int arbo_sur_forward(ArboSurrogate* s, uint32_t n, const float* x, float* out);
 
+/* out = f(x) for n rows on device buffers, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  x and out are
+ * device pointers (x [n * width_0] float32, out [n * width_last] float32);
+ * the work is queued on `stream` (a hipStream_t; NULL: the surrogate's own
+ * stream) and the call returns without waiting.  The activations between
+ * layers live in a fixed 4 MiB scratch array of the library on each GPU, so
+ * n x (the widest layer input) must not exceed 2^20 (4,096 rows of a
+ * 256-wide network), else ARBO_E_CAPACITY; every call on one GPU shares
+ * that array, so calls that overlap on one GPU race.  The results are those
+ * of arbo_sur_forward, bit for bit. */
+ARBO_API int arbo_sur_forward_noalloc(ArboSurrogate* s, uint32_t n, const float* x, float* out, void* stream);
+
 /* out = f(x) and dx = (d f / d x)^T dout, row by row. */
 ARBO_API int arbo_sur_input_grad(ArboSurrogate* s, uint32_t n, const float* x, const float* dout, float* out,
                                  float* dx);
