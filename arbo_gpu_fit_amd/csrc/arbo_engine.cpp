@@ -528,13 +528,17 @@ Tuning from_abi(const ArboTuning& t) {
 }
 ArboTuning to_abi(const Tuning& t) { return ArboTuning{t.unroll, t.early_m, t.xcd, t.ld_split, t.wps}; }
 
-int apply_tuning(ArboEngine* e, const Tuning& tu) {
+// Sets the launch knobs.  The captured graph holds the old kernels: it is
+// released, and rebuilt at once unless `rebuild` is false (the autotuner
+// launches its trial days directly and rebuilds once, for the tuning it
+// keeps).
+int apply_tuning(ArboEngine* e, const Tuning& tu, bool rebuild = true) {
     if (!tuning_supported(tu)) return fail(e, ARBO_E_INVALID, "unsupported tuning");
     HIP_TRY(e, hipSetDevice(e->device));
     HIP_TRY(e, hipStreamSynchronize(e->stream));
     e->tuning = tu;
-    release_graph(e);   // the graph holds the old kernels
-    if (e->ready && (e->flags & ARBO_F_GRAPH)) return build_graph(e);
+    release_graph(e);
+    if (rebuild && e->ready && (e->flags & ARBO_F_GRAPH)) return build_graph(e);
     return ARBO_OK;
 }
 
@@ -593,7 +597,7 @@ int arbo_autotune(ArboEngine* e, uint32_t days_each, ArboTuning* chosen, double*
         hipError_t st = hipSuccess;
         for (uint32_t j = 0; j < n && rc == ARBO_OK && st == hipSuccess; ++j) {
             const uint32_t k = (r & 1u) ? n - 1 - j : j;
-            if ((rc = apply_tuning(e, from_abi(cand[k])))) break;
+            if ((rc = apply_tuning(e, from_abi(cand[k]), false))) break;
             st = hipEventRecord(ev[3 * k], e->stream);
             if (st == hipSuccess) st = launch_demography(e->view, e->tuning, e->stream);
             if (st == hipSuccess) st = hipEventRecord(ev[3 * k + 1], e->stream);
@@ -620,7 +624,11 @@ int arbo_autotune(ArboEngine* e, uint32_t days_each, ArboTuning* chosen, double*
     }
     for (hipEvent_t x : ev)
         if (x) (void)hipEventDestroy(x);
-    if (rc) return rc;
+    if (rc) {   // leave a usable engine: the tuning before the failure, with its graph
+        const int rc2 = apply_tuning(e, e->tuning);
+        (void)rc2;
+        return rc;
+    }
     Tuning keep = from_abi(cand[ka]);
     keep.xcd = (keep.xcd & 1u) | (cand[kb].xcd_order & 2u);
     if ((rc = apply_tuning(e, keep))) return rc;
