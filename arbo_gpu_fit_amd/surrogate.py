@@ -22,6 +22,7 @@ PROTOTYPES = {
     "arbo_sur_get_params": (C.c_int, [SurPtr, _f32p]),
     "arbo_sur_set_params": (C.c_int, [SurPtr, _f32p]),
     "arbo_sur_forward": (C.c_int, [SurPtr, _u32, _f32p, _f32p]),
+    "arbo_sur_forward_noalloc": (C.c_int, [SurPtr, _u32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "arbo_sur_input_grad": (C.c_int, [SurPtr, _u32, _f32p, _f32p, _f32p, _f32p]),
     "arbo_sur_loss_grad": (C.c_int, [SurPtr, _u32, _f32p, _f32p, _f64p, _f32p]),
     "arbo_sur_train": (C.c_int, [SurPtr, _u32, _f32p, _f32p, _u32, _u32, C.c_float, C.c_uint64, _f64p]),
