@@ -1,16 +1,27 @@
-// arbo_surrogate.hip — the surrogate MLP on gfx950: bfloat16 MFMA matrix
-// products with float32 accumulation, fused bias/GELU epilogues, the Poisson
-// loss, and Adam.
+// arbo_surrogate.hip — the surrogate MLP on gfx950: matrix products on the
+// bfloat16 MFMA units with float32 accumulation, fused bias / GELU / GELU
+// gradient / bias-gradient epilogues, the Poisson loss, and Adam.
 //
 // One kernel does every matrix product of the forward and backward passes:
-// C[i, j] = sum_k A(i, k) B(j, k) with element strides for both operands, so
-// X W^T (forward), dZ W (input gradients, through a transposed copy of W)
-// and dZ^T X (weight gradients, reducing over the batch) are the same code.
-// A wave computes one 32 x 32 tile with v_mfma_f32_32x32x8_bf16 (8-deep
-// k-steps, float32 accumulators in 16 registers per lane).  The networks
-// here are small (<= 4096 wide, batches of a few thousand rows), so the
-// operands are gathered straight from global memory (L2 resident) and the
-// launch count, not the arithmetic, sets the time.
+// C[i, j] = sum_k A(i, k) B(j, k), each operand float32 in memory with its
+// own strides (one of them 1), so X W^T (forward), dZ W (input gradients)
+// and dZ^T X (weight gradients over the batch) are the same code.
+//  * A workgroup of four wave64s computes a 64 x 64 block of C; each wave one
+//    32 x 32 tile with v_mfma_f32_32x32x16_bf16 (16-deep k-steps, float32
+//    accumulators, 16 per lane).
+//  * The block's A and B panels, 64 x 32 each, are staged through LDS: a
+//    thread loads 8 consecutive elements along the operand's contiguous
+//    dimension (two 16-byte loads where aligned), splits each float into two
+//    bfloat16 halves, hi = bf16(x) and lo = bf16(x - hi), and stores both;
+//    a lane then reads its 8 k-values as one 16-byte LDS read per half.
+//  * Each k-step issues three MFMAs, hi*hi + hi*lo + lo*hi: the products
+//    carry about 16 significant bits of every operand (the bf16x3 split),
+//    so gradients hold to float32-like accuracy while every multiply runs
+//    on the bfloat16 matrix cores.
+//  * The weight-gradient product appends a column of ones to its B operand,
+//    so the bias gradient (the batch sum of dZ) comes out of the same launch.
+// Parameters stay float32 and are read by the products directly, so an
+// Adam step needs no repacking.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,12 +39,13 @@
 namespace arbo {
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr uint32_t kTile = 32;
+constexpr uint32_t kBM = 64, kBN = 64, kBK = 32;   // block tile; 4 waves, one 32 x 32 MFMA tile each
+constexpr uint32_t kLdsK = kBK + 8;                 // padded LDS row of a panel (bf16): 80 bytes
 
-// float32 -> bfloat16, round to nearest even (NaN kept quiet)
+// float32 -> bfloat16 bits, round to nearest even (NaN kept quiet)
 __host__ __device__ inline uint16_t f2bf(float f) {
     uint32_t u;
     std::memcpy(&u, &f, 4);
@@ -54,56 +66,133 @@ __device__ inline float gelu_grad(float z) {
 }
 
 enum : uint32_t {
-    EPI_STORE = 0,       // outf = acc
+    EPI_STORE = 0,       // outf = acc (with ones: column N - 1 to outb)
     EPI_BIAS = 1,        // outf = acc + bias
-    EPI_BIAS_GELU = 2,   // outf = z = acc + bias; outh = bf16(gelu(z))
-    EPI_GELU_GRAD = 3,   // outf = acc * gelu'(zprev); outh = bf16(outf)
-    EPI_GELU_ONLY = 4    // outh = bf16(gelu(acc + bias)) (forward passes that keep no z)
+    EPI_BIAS_GELU = 2,   // outf = z = acc + bias; outh = gelu(z)
+    EPI_GELU_GRAD = 3,   // outf = acc * gelu'(zprev)
+    EPI_GELU_ONLY = 4    // outh = gelu(acc + bias) (forward passes that keep no z)
 };
 
 struct GemmArgs {
     uint32_t M, N, K, epi;
-    const uint16_t* A;
-    uint64_t sAi, sAk;
-    const uint16_t* B;
-    uint64_t sBj, sBk;
+    const float* A;
+    uint64_t sAi, sAk;      // A(i, k) = A[i sAi + k sAk]; sAk == 1 (k-contiguous) or sAi == 1
+    const float* B;
+    uint64_t sBj, sBk;      // B(j, k) = B[j sBj + k sBk]; sBk == 1 (k-contiguous) or sBj == 1
+    uint32_t vecA, vecB;    // 1: 16-byte loads along the contiguous dimension are aligned
+    uint32_t ones;          // 1: B(N - 1, k) = 1 for every k (its column of C is a row sum of A)
     const float* bias;
     const float* zprev;
     uint32_t ldz;
     float* outf;
     uint32_t ldf;
-    uint16_t* outh;
+    float* outh;
     uint32_t ldh;
+    float* outb;            // with ones: column N - 1 of C goes to outb[i]
 };
 
-// Operand layout of v_mfma_f32_32x32x8_bf16: lane l holds A(l % 32, 4 (l / 32) + v)
-// and B(4 (l / 32) + v, l % 32) for v = 0..3; accumulator r of lane l is
-// D(8 (r / 4) + 4 (l / 32) + r % 4, l % 32).
-__global__ __launch_bounds__(64) void k_gemm(GemmArgs g) {
-    const uint32_t lane = threadIdx.x;
-    const uint32_t i0 = blockIdx.y * kTile, j0 = blockIdx.x * kTile;
-    const uint32_t ia = i0 + (lane & 31u), jb = j0 + (lane & 31u), kq = 4u * (lane >> 5);
-    f32x16 acc = {};
-    for (uint32_t k0 = 0; k0 < g.K; k0 += 8) {
-        bf16x4 a, b;
+struct Panel {
+    uint16_t hi[kBM][kLdsK];
+    uint16_t lo[kBM][kLdsK];
+};
+
+__device__ inline void split_store(float x, uint16_t& hi, uint16_t& lo) {
+    const uint16_t h = f2bf(x);
+    hi = h;
+    lo = f2bf(x - bf2f(h));
+}
+
+// Stages the 64 x 32 panel of operand rows [r0, r0 + 64) and k in [k0, k0 +
+// 32) into LDS.  rows: the operand's rows that exist in memory; with `ones`
+// row `rows` is all ones.  256 threads, 8 elements each.
+__device__ inline void stage(const float* p, uint64_t s_row, uint64_t s_k, uint32_t rows, uint32_t K, uint32_t r0,
+                             uint32_t k0, bool vec, bool ones, Panel& P, uint32_t tid) {
+    float v[8];
+    if (s_k == 1) {   // 8 consecutive k of one row
+        const uint32_t r = tid >> 2, kc = (tid & 3u) * 8u;
+        const uint32_t gr = r0 + r, gk = k0 + kc;
+        if (gr < rows) {
+            const float* q = p + (uint64_t)gr * s_row + gk;
+            if (vec && gk + 8 <= K) {
+                const float4 a = reinterpret_cast<const float4*>(q)[0], b = reinterpret_cast<const float4*>(q)[1];
+                v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+            } else {
 #pragma unroll
-        for (uint32_t v = 0; v < 4; ++v) {
-            const uint32_t k = k0 + kq + v;
-            a[v] = (ia < g.M && k < g.K) ? (short)g.A[ia * g.sAi + k * g.sAk] : (short)0;
-            b[v] = (jb < g.N && k < g.K) ? (short)g.B[jb * g.sBj + k * g.sBk] : (short)0;
+                for (uint32_t e = 0; e < 8; ++e) v[e] = gk + e < K ? q[e] : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t e = 0; e < 8; ++e) v[e] = (ones && gr == rows && gk + e < K) ? 1.0f : 0.0f;
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        uint16_t h[8], l[8];
+#pragma unroll
+        for (uint32_t e = 0; e < 8; ++e) split_store(v[e], h[e], l[e]);
+        uint4 hv, lv;
+        hv.x = h[0] | ((uint32_t)h[1] << 16); hv.y = h[2] | ((uint32_t)h[3] << 16);
+        hv.z = h[4] | ((uint32_t)h[5] << 16); hv.w = h[6] | ((uint32_t)h[7] << 16);
+        lv.x = l[0] | ((uint32_t)l[1] << 16); lv.y = l[2] | ((uint32_t)l[3] << 16);
+        lv.z = l[4] | ((uint32_t)l[5] << 16); lv.w = l[6] | ((uint32_t)l[7] << 16);
+        *reinterpret_cast<uint4*>(&P.hi[r][kc]) = hv;
+        *reinterpret_cast<uint4*>(&P.lo[r][kc]) = lv;
+    } else {          // 8 consecutive rows of one k (s_row == 1)
+        const uint32_t k = tid >> 3, rc = (tid & 7u) * 8u;
+        const uint32_t gk = k0 + k, gr = r0 + rc;
+        if (gk < K) {
+            const float* q = p + (uint64_t)gk * s_k + gr;
+            if (vec && gr + 8 <= rows) {
+                const float4 a = reinterpret_cast<const float4*>(q)[0], b = reinterpret_cast<const float4*>(q)[1];
+                v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+            } else {
+#pragma unroll
+                for (uint32_t e = 0; e < 8; ++e) v[e] = gr + e < rows ? q[e] : ((ones && gr + e == rows) ? 1.0f : 0.0f);
+            }
+        } else {
+#pragma unroll
+            for (uint32_t e = 0; e < 8; ++e) v[e] = 0.0f;
+        }
+#pragma unroll
+        for (uint32_t e = 0; e < 8; ++e) split_store(v[e], P.hi[rc + e][k], P.lo[rc + e][k]);
     }
-    const uint32_t j = jb;
+}
+
+// MFMA operand layout of v_mfma_f32_32x32x16_bf16: lane l holds A(l % 32,
+// 8 (l / 32) + v) and B(8 (l / 32) + v, l % 32) for v = 0..7; accumulator r
+// of lane l is D(8 (r / 4) + 4 (l / 32) + r % 4, l % 32).
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+    __shared__ Panel sa, sb;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const uint32_t i0 = blockIdx.y * kBM, j0 = blockIdx.x * kBN;
+    const uint32_t wi = (wave >> 1) * 32u, wj = (wave & 1u) * 32u;
+    const uint32_t rowsB = g.ones ? g.N - 1 : g.N;
+    f32x16 acc = {};
+    for (uint32_t k0 = 0; k0 < g.K; k0 += kBK) {
+        stage(g.A, g.sAi, g.sAk, g.M, g.K, i0, k0, g.vecA != 0, false, sa, tid);
+        stage(g.B, g.sBj, g.sBk, rowsB, g.K, j0, k0, g.vecB != 0, g.ones != 0, sb, tid);
+        __syncthreads();
+#pragma unroll
+        for (uint32_t kk = 0; kk < kBK; kk += 16) {
+            const uint32_t kq = kk + 8u * (lane >> 5);
+            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&sa.hi[wi + (lane & 31u)][kq]);
+            const bf16x8 al = *reinterpret_cast<const bf16x8*>(&sa.lo[wi + (lane & 31u)][kq]);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&sb.hi[wj + (lane & 31u)][kq]);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&sb.lo[wj + (lane & 31u)][kq]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const uint32_t j = j0 + wj + (lane & 31u);
     if (j >= g.N) return;
 #pragma unroll
     for (uint32_t r = 0; r < 16; ++r) {
-        const uint32_t i = i0 + 8u * (r >> 2) + 4u * (lane >> 5) + (r & 3u);
+        const uint32_t i = i0 + wi + 8u * (r >> 2) + 4u * (lane >> 5) + (r & 3u);
         if (i >= g.M) continue;
-        float v = acc[r];
+        const float v = acc[r];
         switch (g.epi) {
             case EPI_STORE:
-                g.outf[(size_t)i * g.ldf + j] = v;
+                if (g.ones && j == g.N - 1) g.outb[i] = v;
+                else g.outf[(size_t)i * g.ldf + j] = v;
                 break;
             case EPI_BIAS:
                 g.outf[(size_t)i * g.ldf + j] = v + g.bias[j];
@@ -111,25 +200,17 @@ __global__ __launch_bounds__(64) void k_gemm(GemmArgs g) {
             case EPI_BIAS_GELU: {
                 const float z = v + g.bias[j];
                 g.outf[(size_t)i * g.ldf + j] = z;
-                g.outh[(size_t)i * g.ldh + j] = f2bf(gelu(z));
+                g.outh[(size_t)i * g.ldh + j] = gelu(z);
                 break;
             }
-            case EPI_GELU_ONLY:
-                g.outh[(size_t)i * g.ldh + j] = f2bf(gelu(v + g.bias[j]));
+            case EPI_GELU_GRAD:
+                g.outf[(size_t)i * g.ldf + j] = v * gelu_grad(g.zprev[(size_t)i * g.ldz + j]);
                 break;
-            default: {
-                const float d = v * gelu_grad(g.zprev[(size_t)i * g.ldz + j]);
-                g.outf[(size_t)i * g.ldf + j] = d;
-                g.outh[(size_t)i * g.ldh + j] = f2bf(d);
+            default:
+                g.outh[(size_t)i * g.ldh + j] = gelu(v + g.bias[j]);
                 break;
-            }
         }
     }
-}
-
-__global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, uint64_t n) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = f2bf(x[i]);
 }
 
 // rows idx[r] of x [*, w] to out [r, w]
@@ -141,20 +222,18 @@ __global__ void k_gather(const float* __restrict__ x, const uint32_t* __restrict
     out[t] = x[(uint64_t)idx[r] * w + c];
 }
 
-// Poisson loss terms of the output layer: dF = (exp(f) - y) * scale (float32
-// and bfloat16), loss partial sums per workgroup into loss[blockIdx.x]
+// Poisson loss terms of the output layer: dF = (exp(f) - y) * scale; the
+// loss's partial sums per workgroup into part[blockIdx.x] and, for a running
+// epoch sum, into *acc
 __global__ __launch_bounds__(256) void k_poisson(const float* __restrict__ f, const float* __restrict__ y,
                                                  uint64_t n, float scale, float* __restrict__ df,
-                                                 uint16_t* __restrict__ dfh, double* __restrict__ part,
-                                                 double* __restrict__ acc) {
+                                                 double* __restrict__ part, double* __restrict__ acc) {
     __shared__ double s[256];
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double l = 0.0;
     if (i < n) {
         const float e = expf(f[i]);
-        const float d = (e - y[i]) * scale;
-        df[i] = d;
-        dfh[i] = f2bf(d);
+        df[i] = (e - y[i]) * scale;
         l = (double)e - (double)y[i] * (double)f[i];
     }
     s[threadIdx.x] = l;
@@ -165,18 +244,8 @@ __global__ __launch_bounds__(256) void k_poisson(const float* __restrict__ f, co
     }
     if (threadIdx.x == 0) {
         part[blockIdx.x] = s[0];
-        if (acc) atomicAdd(acc, s[0]);   // running sum over an epoch
+        if (acc) atomicAdd(acc, s[0]);
     }
-}
-
-// db[j] = sum_i dz[i, j]
-__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ dz, uint32_t rows, uint32_t cols,
-                                                float* __restrict__ db) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cols) return;
-    float s = 0.0f;
-    for (uint32_t i = 0; i < rows; ++i) s += dz[(size_t)i * cols + j];
-    db[j] = s;
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -191,15 +260,9 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
     p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
 }
 
-// bfloat16 copies of W [N, K]: w [N, K] and wt [K, N]
-__global__ void k_pack(const float* __restrict__ W, uint32_t N, uint32_t K, uint16_t* __restrict__ w,
-                       uint16_t* __restrict__ wt) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (uint64_t)N * K) return;
-    const uint32_t n = (uint32_t)(t / K), k = (uint32_t)(t % K);
-    const uint16_t h = f2bf(W[t]);
-    w[t] = h;
-    wt[(size_t)k * N + n] = h;
+__global__ void k_zero_f32(float* __restrict__ p, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
 }
 
 template <class T>
@@ -209,27 +272,27 @@ struct Buf {
 };
 
 // Activations of arbo_sur_forward_noalloc: two ping-pong buffers of
-// kScratchElems bfloat16 values (4 MiB in all) that every GPU the library
+// kScratchElems float32 values (8 MiB in all) that every GPU the library
 // runs on holds until the process ends.  Host code never uses the symbol
 // itself (on the host it only names the variable): it takes each device's
 // address once, from hipGetSymbolAddress with that device current, when a
 // surrogate is created there.
 constexpr size_t kScratchElems = size_t(1) << 20;
 constexpr int kMaxDevices = 64;
-static __device__ __align__(256) uint16_t scratch[2 * kScratchElems];
+static __device__ __align__(256) float scratch[2 * kScratchElems];
 
 std::mutex g_scratch_mu;
-uint16_t* g_scratch_ptr[kMaxDevices] = {};
+float* g_scratch_ptr[kMaxDevices] = {};
 
 // The scratch array's address on `device`, which must be current.
-hipError_t scratch_on(int device, uint16_t** out) {
+hipError_t scratch_on(int device, float** out) {
     if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
     std::lock_guard<std::mutex> lock(g_scratch_mu);
     if (!g_scratch_ptr[device]) {
         void* p = nullptr;
         const hipError_t st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
         if (st != hipSuccess) return st;
-        g_scratch_ptr[device] = static_cast<uint16_t*>(p);
+        g_scratch_ptr[device] = static_cast<float*>(p);
     }
     *out = g_scratch_ptr[device];
     return hipSuccess;
@@ -248,17 +311,15 @@ struct ArboSurrogate {
     std::vector<size_t> woff, boff; // offsets of W_l and b_l in the flat parameter array
     size_t n_params = 0;
     Buf<float> P, G, Mo, Vo;        // parameters, gradients, Adam moments
-    std::vector<Buf<uint16_t>> Wh, Wt;
     std::vector<Buf<float>> Z;      // pre-activations [B, w_{l+1}]
-    std::vector<Buf<uint16_t>> Xh;  // layer inputs (bf16) [B, w_l]
-    std::vector<Buf<float>> dZ;     // float32 gradients [B, w_{l+1}]
-    std::vector<Buf<uint16_t>> dZh; // bfloat16 gradients
-    Buf<float> xin, yin, dxin;
-    Buf<double> part;
+    std::vector<Buf<float>> H;      // layer inputs [B, w_l] (H[0]: the rows' inputs)
+    std::vector<Buf<float>> dZ;     // gradients of the pre-activations [B, w_{l+1}]
+    Buf<float> yin, dxin;
+    Buf<double> part;               // the loss's per-workgroup sums (max_batch x outputs / 256)
     Buf<double> epoch_sum;          // the loss summed over one training epoch
     Buf<float> data_x, data_y, by;
     Buf<uint32_t> idx;
-    uint16_t* scratch = nullptr;    // this device's no-allocation activation buffers
+    float* scratch = nullptr;       // this device's no-allocation activation buffers
     uint64_t adam_t = 0;
     std::string err;
 };
@@ -301,13 +362,16 @@ void release(Buf<T>& b) {
 
 uint32_t blocks(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
-hipError_t gemm(hipStream_t st, const GemmArgs& g) {
-    dim3 grid(blocks(g.N, kTile), blocks(g.M, kTile));
-    hipLaunchKernelGGL(k_gemm, grid, dim3(64), 0, st, g);
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// C[M, N] = sum_k A(i, k) B(j, k) with the epilogue of g.epi.
+hipError_t gemm(hipStream_t st, GemmArgs g) {
+    g.vecA = aligned16(g.A) && ((g.sAk == 1 ? g.sAi : g.sAk) % 4 == 0);
+    g.vecB = aligned16(g.B) && ((g.sBk == 1 ? g.sBj : g.sBk) % 4 == 0);
+    dim3 grid(blocks(g.N, kBN), blocks(g.M, kBM));
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, st, g);
     return hipGetLastError();
 }
-
-hipError_t gemm(ArboSurrogate* s, const GemmArgs& g) { return gemm(s->stream, g); }
 
 GemmArgs args(uint32_t M, uint32_t N, uint32_t K, uint32_t epi) {
     GemmArgs g{};
@@ -318,89 +382,75 @@ GemmArgs args(uint32_t M, uint32_t N, uint32_t K, uint32_t epi) {
     return g;
 }
 
-// Forward pass of B rows already in Xh[0]; the output layer's z lands in Z[L-1].
+// Forward pass of B rows in H[0]: z of every layer in Z, the hidden layers'
+// activations in H; the output layer's z in Z[L-1].
 int forward(ArboSurrogate* s, uint32_t B) {
     for (uint32_t l = 0; l < s->L; ++l) {
         const uint32_t K = s->w[l], N = s->w[l + 1];
         const bool last = l + 1 == s->L;
         GemmArgs g = args(B, N, K, last ? EPI_BIAS : EPI_BIAS_GELU);
-        g.A = s->Xh[l].p;
+        g.A = s->H[l].p;
         g.sAi = K;
         g.sAk = 1;
-        g.B = s->Wh[l].p;
+        g.B = s->P.p + s->woff[l];   // W_l [N, K]
         g.sBj = K;
         g.sBk = 1;
         g.bias = s->P.p + s->boff[l];
         g.outf = s->Z[l].p;
         g.ldf = N;
         if (!last) {
-            g.outh = s->Xh[l + 1].p;
+            g.outh = s->H[l + 1].p;
             g.ldh = N;
         }
-        SUR_TRY(s, gemm(s, g));
+        SUR_TRY(s, gemm(s->stream, g));
     }
     return ARBO_OK;
 }
 
-// Backward pass from dZ[L-1] / dZh[L-1]: weight gradients into G when
-// want_w, input gradients into dxin when want_x.
+// Backward pass from dZ[L-1]: weight and bias gradients into G when want_w
+// (one launch per layer: the bias gradient is the ones column), input
+// gradients into dxin when want_x.
 int backward(ArboSurrogate* s, uint32_t B, bool want_w, bool want_x) {
     for (int l = (int)s->L - 1; l >= 0; --l) {
         const uint32_t K = s->w[l], N = s->w[l + 1];
-        if (want_w) {
-            GemmArgs g = args(N, K, B, EPI_STORE);    // dW = dZ^T X
-            g.A = s->dZh[l].p;
+        if (want_w) {   // [dW | db] = dZ^T [X | 1]
+            GemmArgs g = args(N, K + 1, B, EPI_STORE);
+            g.A = s->dZ[l].p;
             g.sAi = 1;
             g.sAk = N;
-            g.B = s->Xh[l].p;
+            g.B = s->H[l].p;
             g.sBj = 1;
             g.sBk = K;
+            g.ones = 1;
             g.outf = s->G.p + s->woff[l];
             g.ldf = K;
-            SUR_TRY(s, gemm(s, g));
-            hipLaunchKernelGGL(k_colsum, dim3(blocks(N, 256)), dim3(256), 0, s->stream, s->dZ[l].p, B, N,
-                               s->G.p + s->boff[l]);
-            SUR_TRY(s, hipGetLastError());
+            g.outb = s->G.p + s->boff[l];
+            SUR_TRY(s, gemm(s->stream, g));
         }
         if (l == 0 && !want_x) break;
         GemmArgs g = args(B, K, N, l > 0 ? EPI_GELU_GRAD : EPI_STORE);   // dX = dZ W
-        g.A = s->dZh[l].p;
+        g.A = s->dZ[l].p;
         g.sAi = N;
         g.sAk = 1;
-        g.B = s->Wt[l].p;
-        g.sBj = N;
-        g.sBk = 1;
+        g.B = s->P.p + s->woff[l];   // B(j, k) = W[k, j]
+        g.sBj = 1;
+        g.sBk = K;
         if (l > 0) {
             g.zprev = s->Z[l - 1].p;
             g.ldz = K;
             g.outf = s->dZ[l - 1].p;
             g.ldf = K;
-            g.outh = s->dZh[l - 1].p;
-            g.ldh = K;
         } else {
             g.outf = s->dxin.p;
             g.ldf = K;
         }
-        SUR_TRY(s, gemm(s, g));
-    }
-    return ARBO_OK;
-}
-
-int pack_weights(ArboSurrogate* s) {
-    for (uint32_t l = 0; l < s->L; ++l) {
-        const uint64_t n = (uint64_t)s->w[l + 1] * s->w[l];
-        hipLaunchKernelGGL(k_pack, dim3(blocks(n, 256)), dim3(256), 0, s->stream, s->P.p + s->woff[l], s->w[l + 1],
-                           s->w[l], s->Wh[l].p, s->Wt[l].p);
-        SUR_TRY(s, hipGetLastError());
+        SUR_TRY(s, gemm(s->stream, g));
     }
     return ARBO_OK;
 }
 
 int upload_x(ArboSurrogate* s, uint32_t n, const float* x) {
-    const uint64_t cnt = (uint64_t)n * s->w[0];
-    SUR_TRY(s, hipMemcpyAsync(s->xin.p, x, cnt * 4, hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(cnt, 256)), dim3(256), 0, s->stream, s->xin.p, s->Xh[0].p, cnt);
-    SUR_TRY(s, hipGetLastError());
+    SUR_TRY(s, hipMemcpyAsync(s->H[0].p, x, (size_t)n * s->w[0] * 4, hipMemcpyHostToDevice, s->stream));
     return ARBO_OK;
 }
 
@@ -410,9 +460,8 @@ int poisson(ArboSurrogate* s, uint32_t B, const float* y, float scale, double* l
     const uint32_t out = s->w[s->L];
     const uint64_t n = (uint64_t)B * out;
     const uint32_t nb = blocks(n, 256);
-    if (int rc = alloc(s, s->part, nb)) return rc;
     hipLaunchKernelGGL(k_poisson, dim3(nb), dim3(256), 0, s->stream, s->Z[s->L - 1].p, y, n, scale,
-                       s->dZ[s->L - 1].p, s->dZh[s->L - 1].p, s->part.p, acc);
+                       s->dZ[s->L - 1].p, s->part.p, acc);
     SUR_TRY(s, hipGetLastError());
     if (loss) {
         std::vector<double> h(nb);
@@ -431,7 +480,7 @@ int adam_step(ArboSurrogate* s, float lr) {
     hipLaunchKernelGGL(k_adam, dim3(blocks(s->n_params, 256)), dim3(256), 0, s->stream, s->P.p, s->G.p, s->Mo.p,
                        s->Vo.p, (uint64_t)s->n_params, lr, c1, c2);
     SUR_TRY(s, hipGetLastError());
-    return pack_weights(s);
+    return ARBO_OK;
 }
 
 }  // namespace
@@ -466,28 +515,22 @@ ARBO_API int arbo_sur_create(int32_t device, uint32_t n_widths, const uint32_t* 
         off += s->w[l + 1];
     }
     s->n_params = off;
-    s->Wh.resize(s->L);
-    s->Wt.resize(s->L);
     s->Z.resize(s->L);
-    s->Xh.resize(s->L);
+    s->H.resize(s->L);
     s->dZ.resize(s->L);
-    s->dZh.resize(s->L);
     int rc = alloc(s, s->P, off);
     if (!rc) rc = alloc(s, s->G, off);
     if (!rc) rc = alloc(s, s->Mo, off);
     if (!rc) rc = alloc(s, s->Vo, off);
     for (uint32_t l = 0; l < s->L && !rc; ++l) {
-        const size_t wn = (size_t)s->w[l + 1] * s->w[l];
-        rc = alloc(s, s->Wh[l], wn);
-        if (!rc) rc = alloc(s, s->Wt[l], wn);
-        if (!rc) rc = alloc(s, s->Z[l], (size_t)max_batch * s->w[l + 1]);
-        if (!rc) rc = alloc(s, s->Xh[l], (size_t)max_batch * s->w[l]);
+        rc = alloc(s, s->Z[l], (size_t)max_batch * s->w[l + 1]);
+        if (!rc) rc = alloc(s, s->H[l], (size_t)max_batch * s->w[l]);
         if (!rc) rc = alloc(s, s->dZ[l], (size_t)max_batch * s->w[l + 1]);
-        if (!rc) rc = alloc(s, s->dZh[l], (size_t)max_batch * s->w[l + 1]);
     }
-    if (!rc) rc = alloc(s, s->xin, (size_t)max_batch * s->w[0]);
     if (!rc) rc = alloc(s, s->dxin, (size_t)max_batch * s->w[0]);
     if (!rc) rc = alloc(s, s->yin, (size_t)max_batch * s->w[s->L]);
+    if (!rc) rc = alloc(s, s->part, blocks((uint64_t)max_batch * s->w[s->L], 256));
+    if (!rc) rc = alloc(s, s->epoch_sum, 1);
     if (rc) {
         arbo_sur_destroy(s);
         return rc;
@@ -540,10 +583,12 @@ ARBO_API int arbo_sur_set_params(ArboSurrogate* s, const float* in) {
     if (!s || !in) return ARBO_E_INVALID;
     SUR_TRY(s, hipSetDevice(s->device));
     SUR_TRY(s, hipMemcpyAsync(s->P.p, in, s->n_params * 4, hipMemcpyHostToDevice, s->stream));
-    SUR_TRY(s, hipMemsetAsync(s->Mo.p, 0, s->n_params * 4, s->stream));
-    SUR_TRY(s, hipMemsetAsync(s->Vo.p, 0, s->n_params * 4, s->stream));
+    hipLaunchKernelGGL(k_zero_f32, dim3(blocks(s->n_params, 256)), dim3(256), 0, s->stream, s->Mo.p,
+                       (uint64_t)s->n_params);
+    hipLaunchKernelGGL(k_zero_f32, dim3(blocks(s->n_params, 256)), dim3(256), 0, s->stream, s->Vo.p,
+                       (uint64_t)s->n_params);
+    SUR_TRY(s, hipGetLastError());
     s->adam_t = 0;
-    if (int rc = pack_weights(s)) return rc;
     SUR_TRY(s, hipStreamSynchronize(s->stream));
     return ARBO_OK;
 }
@@ -561,25 +606,22 @@ ARBO_API int arbo_sur_forward(ArboSurrogate* s, uint32_t n, const float* x, floa
 
 ARBO_API int arbo_sur_forward_noalloc(ArboSurrogate* s, uint32_t n, const float* x, float* out, void* stream) {
     if (!s || !x || !out || n == 0) return ARBO_E_INVALID;
-    uint32_t widest = 0;   // widest layer input: the activations held between layers
-    for (uint32_t l = 0; l < s->L; ++l) widest = std::max(widest, s->w[l]);
+    uint32_t widest = 0;   // widest hidden layer: the activations held between layers
+    for (uint32_t l = 1; l < s->L; ++l) widest = std::max(widest, s->w[l]);
     if ((uint64_t)n * widest > kScratchElems)
-        return fail(s, ARBO_E_CAPACITY, "rows x widest layer input exceed the 2^20 activations of the no-allocation scratch");
+        return fail(s, ARBO_E_CAPACITY, "rows x widest hidden layer exceed the 2^20 activations of the no-allocation scratch");
     if (!s->scratch) return fail(s, ARBO_E_STATE, "no scratch address on this device");
     SUR_TRY(s, hipSetDevice(s->device));
     const hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
-    uint16_t* buf[2] = {s->scratch, s->scratch + kScratchElems};
-    const uint64_t cnt = (uint64_t)n * s->w[0];
-    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(cnt, 256)), dim3(256), 0, st, x, buf[0], cnt);
-    SUR_TRY(s, hipGetLastError());
+    float* buf[2] = {s->scratch, s->scratch + kScratchElems};
     for (uint32_t l = 0; l < s->L; ++l) {   // the products of forward(), activations ping-ponged in the scratch
         const uint32_t K = s->w[l], N = s->w[l + 1];
         const bool last = l + 1 == s->L;
         GemmArgs g = args(n, N, K, last ? EPI_BIAS : EPI_GELU_ONLY);
-        g.A = buf[l & 1];
+        g.A = l == 0 ? x : buf[(l + 1) & 1];
         g.sAi = K;
         g.sAk = 1;
-        g.B = s->Wh[l].p;
+        g.B = s->P.p + s->woff[l];
         g.sBj = K;
         g.sBk = 1;
         g.bias = s->P.p + s->boff[l];
@@ -587,7 +629,7 @@ ARBO_API int arbo_sur_forward_noalloc(ArboSurrogate* s, uint32_t n, const float*
             g.outf = out;
             g.ldf = N;
         } else {
-            g.outh = buf[(l + 1) & 1];
+            g.outh = buf[l & 1];
             g.ldh = N;
         }
         SUR_TRY(s, gemm(st, g));
@@ -604,9 +646,6 @@ ARBO_API int arbo_sur_input_grad(ArboSurrogate* s, uint32_t n, const float* x, c
     if (int rc = forward(s, n)) return rc;
     const uint64_t no = (uint64_t)n * s->w[s->L];
     SUR_TRY(s, hipMemcpyAsync(s->dZ[s->L - 1].p, dout, no * 4, hipMemcpyHostToDevice, s->stream));
-    hipLaunchKernelGGL(k_to_bf16, dim3(blocks(no, 256)), dim3(256), 0, s->stream, s->dZ[s->L - 1].p,
-                       s->dZh[s->L - 1].p, no);
-    SUR_TRY(s, hipGetLastError());
     if (int rc = backward(s, n, false, true)) return rc;
     if (out) SUR_TRY(s, hipMemcpyAsync(out, s->Z[s->L - 1].p, no * 4, hipMemcpyDeviceToHost, s->stream));
     SUR_TRY(s, hipMemcpyAsync(dx, s->dxin.p, (size_t)n * s->w[0] * 4, hipMemcpyDeviceToHost, s->stream));
@@ -642,7 +681,6 @@ ARBO_API int arbo_sur_train(ArboSurrogate* s, uint32_t n, const float* x, const 
     if (!rc) rc = alloc(s, s->data_y, (size_t)n * out);
     if (!rc) rc = alloc(s, s->by, (size_t)batch * out);
     if (!rc) rc = alloc(s, s->idx, n);
-    if (!rc) rc = alloc(s, s->epoch_sum, 1);
     if (rc) return rc;
     SUR_TRY(s, hipMemcpyAsync(s->data_x.p, x, (size_t)n * in * 4, hipMemcpyHostToDevice, s->stream));
     SUR_TRY(s, hipMemcpyAsync(s->data_y.p, y, (size_t)n * out * 4, hipMemcpyHostToDevice, s->stream));
@@ -659,15 +697,17 @@ ARBO_API int arbo_sur_train(ArboSurrogate* s, uint32_t n, const float* x, const 
         }
         SUR_TRY(s, hipStreamSynchronize(s->stream));   // perm is reused below
         SUR_TRY(s, hipMemcpyAsync(s->idx.p, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s->stream));
-        if (epoch_loss) SUR_TRY(s, hipMemsetAsync(s->epoch_sum.p, 0, sizeof(double), s->stream));
+        if (epoch_loss) {
+            hipLaunchKernelGGL(k_zero_f32, dim3(1), dim3(64), 0, s->stream, reinterpret_cast<float*>(s->epoch_sum.p),
+                               (uint64_t)2);
+            SUR_TRY(s, hipGetLastError());
+        }
         for (uint32_t b0 = 0; b0 < n; b0 += batch) {
             const uint32_t B = std::min(batch, n - b0);
             hipLaunchKernelGGL(k_gather, dim3(blocks((uint64_t)B * in, 256)), dim3(256), 0, s->stream, s->data_x.p,
-                               s->idx.p + b0, s->xin.p, B, in);
+                               s->idx.p + b0, s->H[0].p, B, in);
             hipLaunchKernelGGL(k_gather, dim3(blocks((uint64_t)B * out, 256)), dim3(256), 0, s->stream, s->data_y.p,
                                s->idx.p + b0, s->by.p, B, out);
-            hipLaunchKernelGGL(k_to_bf16, dim3(blocks((uint64_t)B * in, 256)), dim3(256), 0, s->stream, s->xin.p,
-                               s->Xh[0].p, (uint64_t)B * in);
             SUR_TRY(s, hipGetLastError());
             if ((rc = forward(s, B))) return rc;
             // each batch's loss before its update, summed on the device
@@ -695,13 +735,9 @@ ARBO_API void arbo_sur_destroy(ArboSurrogate* s) {
     release(s->G);
     release(s->Mo);
     release(s->Vo);
-    for (auto& b : s->Wh) release(b);
-    for (auto& b : s->Wt) release(b);
     for (auto& b : s->Z) release(b);
-    for (auto& b : s->Xh) release(b);
+    for (auto& b : s->H) release(b);
     for (auto& b : s->dZ) release(b);
-    for (auto& b : s->dZh) release(b);
-    release(s->xin);
     release(s->yin);
     release(s->dxin);
     release(s->part);

@@ -376,6 +376,54 @@ def mcmc_bench(args) -> dict:
     return out
 
 
+def surrogate_bench(args) -> dict:
+    """Training throughput of the surrogate MLP 6 -> 256^4 -> 260 (GELU) on
+    the MFMA kernels: `train_rows` rows of synthetic data (inputs uniform on
+    [-1, 1]^6, Poisson counts of a fixed random teacher network), minibatches
+    of 512, Adam, `sur_epochs` epochs after one untimed warm-up epoch.  The
+    matrix-product FLOPs per training row are those of the float32 network
+    (forward, weight and input gradients); the MFMA units execute three
+    bfloat16 products for each (the operands' hi/lo split)."""
+    import numpy as np
+
+    from arbo_gpu_fit_amd import abi
+    from arbo_gpu_fit_amd.surrogate import Surrogate
+    from tests.mlp_ref import forward64, random_layers
+
+    widths = [6, 256, 256, 256, 256, 260]
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, size=(args.train_rows, widths[0])).astype(np.float32)
+    teacher = random_layers(widths, 7)
+    mu = np.exp(np.clip(0.3 * forward64(teacher, x)[0] + 2.0, -3, 6))
+    y = rng.poisson(mu).astype(np.float32)
+    sur = Surrogate(widths, max_batch=4096, seed=1, lib=abi.load_hip_library())
+    sur.train(x, y, epochs=1, batch=512, lr=1e-3, seed=2)   # warm-up
+    t0 = time.perf_counter()
+    hist = sur.train(x, y, epochs=args.sur_epochs, batch=512, lr=1e-3, seed=3)
+    dt = time.perf_counter() - t0
+    sur.close()
+    rows = args.train_rows * args.sur_epochs
+    steps = -(-args.train_rows // 512) * args.sur_epochs
+    mm = sum(a * b for a, b in zip(widths[:-1], widths[1:]))
+    flop_row = 2 * mm * 3 - 2 * widths[0] * widths[1]   # forward + weight gradients + input gradients (not of layer 0)
+    tf = flop_row * rows / dt / 1e12
+    return {
+        "metric": "surrogate training throughput (6-256^4-260 MLP, bfloat16 MFMA)",
+        "value": rows / dt,
+        "unit": "training rows/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "ms_per_step": dt * 1e3 / steps,
+        "higher_is_better": True,
+        "dtype": "bf16x3 (float32 operands split into two bfloat16 halves), float32 accumulation",
+        "data": "synthetic (a random teacher network's Poisson counts)",
+        "config": {"workload": f"{args.train_rows} rows x {args.sur_epochs} epochs, minibatch 512, Adam"},
+        "roofline": {"bound": "mfma", "achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": tf / 2500.0,
+                     "mfma_issued": 3 * tf, "mfma_issued_frac": 3 * tf / 2500.0, "flop_per_row": flop_row},
+        "loss_first_last": [float(hist[0]), float(hist[-1])],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,6 +453,9 @@ def main():
     ap.add_argument("--sub-steps", type=int, default=10)
     ap.add_argument("--save", default="", help="save the surrogate's training set (npz)")
     ap.add_argument("--load", default="", help="load the surrogate's training set (npz)")
+    ap.add_argument("--surrogate", action="store_true", help="surrogate training throughput instead")
+    ap.add_argument("--train-rows", type=int, default=50000)
+    ap.add_argument("--sur-epochs", type=int, default=20)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -428,6 +479,9 @@ def main():
 
     if args.mcmc:
         print(json.dumps(mcmc_bench(args)), flush=True)
+        return
+    if args.surrogate:
+        print(json.dumps(surrogate_bench(args)), flush=True)
         return
     if args.cpu_sims == 0:
         args.cpu_sims = 64 if args.steps <= 60 else 8

@@ -7,8 +7,10 @@
  * from the engine and screens proposals as the first stage of
  * delayed-acceptance MCMC, the simulator staying the second stage.
  *
- * Every matrix product runs on the MFMA units in bfloat16 with float32
- * accumulation; parameters, gradients and the optimiser state are float32.
+ * Every matrix product runs on the bfloat16 MFMA units with float32
+ * accumulation, each float32 operand split into two bfloat16 halves (three
+ * products per step: about 16 significant bits per operand); parameters,
+ * activations, gradients and the optimiser state are float32.
  * Host buffers are row-major float32: inputs [n * width_0], outputs
  * [n * width_last].  Parameters flatten layer by layer as W [width_{l+1} x
  * width_l] (row-major) then b [width_{l+1}].
@@ -50,11 +52,11 @@ ARBO_API int arbo_sur_forward(ArboSurrogate* s, uint32_t n, const float* x, floa
  * device pointers (x [n * width_0] float32, out [n * width_last] float32);
  * the work is queued on `stream` (a hipStream_t; NULL: the surrogate's own
  * stream) and the call returns without waiting.  The activations between
- * layers live in a fixed 4 MiB scratch array of the library on each GPU, so
- * n x (the widest layer input) must not exceed 2^20 (4,096 rows of a
- * 256-wide network), else ARBO_E_CAPACITY; every call on one GPU shares
- * that array, so calls that overlap on one GPU race.  The results are those
- * of arbo_sur_forward, bit for bit. */
+ * layers live in a fixed 8 MiB scratch array of the library on each GPU
+ * (two buffers of 2^20 float32), so n x (the widest hidden layer) must not
+ * exceed 2^20 (4,096 rows of a 256-wide network), else ARBO_E_CAPACITY;
+ * every call on one GPU shares that array, so calls that overlap on one GPU
+ * race.  The results are those of arbo_sur_forward, bit for bit. */
 ARBO_API int arbo_sur_forward_noalloc(ArboSurrogate* s, uint32_t n, const float* x, float* out, void* stream);
 
 /* out = f(x) and dx = (d f / d x)^T dout, row by row. */

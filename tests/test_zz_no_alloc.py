@@ -51,9 +51,7 @@ class Hip:
 
 
 def test_forward_noalloc_matches_forward(hip_lib):
-    lib = hip_lib
-    lib.arbo_sur_forward_noalloc.restype = C.c_int
-    lib.arbo_sur_forward_noalloc.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib = hip_lib   # arbo_sur_forward_noalloc bound by surrogate.bind (device pointers as void*)
     widths = [6, 256, 256, 256, 256, 260]
     sur = Surrogate(widths, max_batch=4096, seed=3, lib=lib)
     hip = Hip()
@@ -74,7 +72,7 @@ def test_forward_noalloc_matches_forward(hip_lib):
         assert np.array_equal(got, ref), f"{n} rows: max difference {np.nanmax(np.abs(got - ref))}"
         hip.lib.hipFree(xd)
         hip.lib.hipFree(od)
-    # beyond the scratch: 4,097 rows of a 256-wide layer input
+    # beyond the scratch: 4,097 rows of 256-wide hidden layers
     big = hip.alloc(4097 * widths[0] * 4)
     out = hip.alloc(4097 * widths[-1] * 4)
     assert lib.arbo_sur_forward_noalloc(sur._h, 4097, big, out, None) == abi.E_CAPACITY
