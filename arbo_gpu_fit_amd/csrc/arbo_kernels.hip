@@ -278,10 +278,18 @@ __device__ __forceinline__ uint32_t day_of(uint32_t t, uint32_t nx) {   // absol
 
 // Four humans of ndemographic.cu:57-103 (one 16-byte vector of packed words,
 // index 4q..4q+3).  The next-event days of the vector are read with one
-// 8-byte load, only when one of its members is infected and survives.
-template <bool HIST>
-__device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& cd, const SimDesc& sd, Key key,
-                                          uint32_t t, uint32_t q, uint4 w4, HumanCount& hc) {
+// 8-byte load, only when one of its members is infected and survives.  The
+// changes that need a random draw (a death with its rebirth's lifetime, an
+// E -> I onset with its infectious period) are not made here: they come back
+// as `defer` (bit e: member e) with `onset` (bit e: an onset, else a death)
+// and the onset's E -> I day in nxd[e], for the whole wave to process
+// together (human_deferred).  Everything else (counts, recoveries) is done
+// here; an onset is counted here too.
+struct HumanDefer { uint32_t defer, onset, nxd[4]; };
+
+__device__ __forceinline__ HumanDefer human_vec(const EngineView& v, const CommDesc& cd, uint32_t t, uint32_t q,
+                                                uint4 w4, HumanCount& hc) {
+    HumanDefer d{0u, 0u, {0u, 0u, 0u, 0u}};
     const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
     // death day <= t  <=>  word < (t + 1) 2^8 (t < 2^23, so no overflow)
     const uint32_t tt = (t + 1u) << 8;
@@ -303,50 +311,91 @@ __device__ __forceinline__ void human_vec(const EngineView& v, const CommDesc& c
         const uint32_t md = mw >> 8;
         hc.minev = md < hc.minev ? md : hc.minev;
     }
-    if (!(dying | need)) return;
-    uint32_t nxs[4] = {0, 0, 0, 0};
-    if (need) {
+    d.defer = dying;   // death and instant rebirth
+    if (!need) return d;
+    uint32_t nxs[4];
+    {
         const uint2 nx2 = *reinterpret_cast<const uint2*>(v.h_next + cd.h_off + 4 * q);
         nxs[0] = nx2.x & 0xFFFFu; nxs[1] = nx2.x >> 16; nxs[2] = nx2.y & 0xFFFFu; nxs[3] = nx2.y >> 16;
     }
 #pragma unroll
     for (uint32_t e = 0; e < 4; ++e) {
-        const uint32_t i = 4 * q + e;
-        const uint64_t gi = cd.h_off + i;
-        const uint32_t w = ws[e];
-        if ((dying >> e) & 1u) {  // death and instant rebirth
-            const U4 r = draw(key, cd.h_base + i, t, P_H_BIRTH, 0);
-            const uint32_t da = death_age(v.tables + sd.death_h_off, kHumanTable - 1, (r.x >> 8) + 1);
-            v.h_word[gi] = (t + da) << 8;
-            v.h_birth[gi] = (int32_t)t;
-            if (HIST && v.hist) reinterpret_cast<int4*>(v.hist)[gi] = make_int4(-1, -1, -1, -1);   // naive newborn
-            hc.minev = t + da < hc.minev ? t + da : hc.minev;
-            continue;
-        }
         if (!((need >> e) & 1u)) continue;
+        const uint32_t w = ws[e];
         const uint32_t st = w & 3u, strain = (w >> 2) & 3u;
         const uint32_t nx = nxs[e];
         const bool due = (int16_t)(uint16_t)(t - nx) > 0;
         uint32_t ev = 0xFFFFFFFFu;                // next state change: the day after `next`
         if (st == 2) {
             if (due) {
-                v.h_word[gi] = w & ~0xFu;        // recovered
+                v.h_word[cd.h_off + 4 * q + e] = w & ~0xFu;   // recovered
             } else {
                 add_pair(hc.ih01, hc.ih23, strain);
                 ev = day_of(t, nx) + 1;
             }
-        } else if (due) {                        // E -> I
-            const U4 r = draw(key, cd.h_base + i, t, P_H_REC, 0);
-            const uint32_t nn = nx + stoch_round(sd.recovery, r.x);
-            v.h_next[gi] = (uint16_t)nn;
-            v.h_word[gi] = (w & ~3u) | 2u;
+        } else if (due) {                        // E -> I: counted now, its infectious period drawn later
             add_pair(hc.inc01, hc.inc23, strain);
             add_pair(hc.ih01, hc.ih23, strain);
-            ev = day_of(t, nn) + 1;
+            d.defer |= 1u << e;
+            d.onset |= 1u << e;
+            d.nxd[e] = nx;
         } else {
             ev = day_of(t, nx) + 1;
         }
         hc.minev = ev < hc.minev ? ev : hc.minev;
+    }
+    return d;
+}
+
+// Queue of the wave's deferred human changes (in LDS): entry k holds the
+// member index (bits 0-13), 1 << 14 for an onset, and the onset's E -> I day
+// (bits 16-31) in qi[k], the member's word in qw[k].
+constexpr uint32_t kHumanQueue = 256;
+
+// Appends every lane's deferred changes of one vector at qi/qw[base...]:
+// member e of each lane goes to base + (changes of members < e in the wave)
+// + (lanes below with a change of member e).  All lanes call it; `bal` are
+// the wave's ballots of defer bits 0..3 and `before` their running sums.
+__device__ __forceinline__ void human_enqueue(const HumanDefer& d, uint32_t q, uint4 w4, const uint64_t (&bal)[4],
+                                              const uint32_t (&before)[4], uint32_t* qi, uint32_t* qw, uint32_t base) {
+    const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+        if (!((d.defer >> e) & 1u)) continue;
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[e] >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bal[e], 0u));
+        const uint32_t pos = base + before[e] + below;
+        qi[pos] = (4u * q + e) | (((d.onset >> e) & 1u) << 14) | (d.nxd[e] << 16);
+        qw[pos] = ws[e];
+    }
+}
+
+// The deferred changes, one per lane: a death draws the newborn's lifetime
+// (P_H_BIRTH) and rebirths the member; an onset draws the infectious period
+// (P_H_REC) and makes the member infectious.  The same draws and writes as
+// made in place, so the census is unchanged by the deferral.
+template <bool HIST>
+__device__ __forceinline__ void human_deferred(const EngineView& v, const CommDesc& cd, const SimDesc& sd, Key key,
+                                               uint32_t t, const uint32_t* qi, const uint32_t* qw, uint32_t n,
+                                               HumanCount& hc) {
+    for (uint32_t k = threadIdx.x; k < n; k += kWave) {
+        const uint32_t e = qi[k], i = e & kKeyMask;
+        const uint64_t gi = cd.h_off + i;
+        if (!(e & (1u << 14))) {   // death and instant rebirth
+            const U4 r = draw(key, cd.h_base + i, t, P_H_BIRTH, 0);
+            const uint32_t da = death_age(v.tables + sd.death_h_off, kHumanTable - 1, (r.x >> 8) + 1);
+            v.h_word[gi] = (t + da) << 8;
+            v.h_birth[gi] = (int32_t)t;
+            if (HIST && v.hist) reinterpret_cast<int4*>(v.hist)[gi] = make_int4(-1, -1, -1, -1);   // naive newborn
+            hc.minev = t + da < hc.minev ? t + da : hc.minev;
+        } else {                   // E -> I
+            const U4 r = draw(key, cd.h_base + i, t, P_H_REC, 0);
+            const uint32_t nn = (e >> 16) + stoch_round(sd.recovery, r.x);
+            v.h_next[gi] = (uint16_t)nn;
+            v.h_word[gi] = (qw[k] & ~3u) | 2u;
+            const uint32_t ev = day_of(t, nn) + 1;
+            hc.minev = ev < hc.minev ? ev : hc.minev;
+        }
     }
 }
 
@@ -448,22 +497,51 @@ void k_day_demography(EngineView v) {
         hc.ih01 = hs.y;
         hc.ih23 = hs.z;
     }
+    // Changes that need a random draw queue up in LDS (the dead-slot bitmap
+    // arrays, not used before the mosquito scan) and are processed a lane
+    // each when the queue could overflow and after the scan.
+    uint32_t* hq_i = s_dead;
+    uint32_t* hq_w = s_died;
+    uint32_t hq_n = 0;   // wave-uniform
+    static_assert(kMaxWords >= kHumanQueue, "the human queue fits the bitmap arrays");
+    auto scan_vector = [&](uint32_t q, uint4 w4) {   // every lane calls it; lanes past nvec add nothing
+        HumanDefer d{0u, 0u, {0u, 0u, 0u, 0u}};
+        if (q < nvec) d = human_vec(v, cd, t, q, w4, hc);
+        uint64_t bal[4];
+        uint32_t before[4], many = 0;
 #pragma unroll
-    for (uint32_t u = 0; u < UNROLL; ++u) {
-        const uint32_t q = u * kWave + lane;
-        if (scan_h && q < nvec) human_vec<HIST>(v, cd, sd, key, t, q, hb[u], hc);
-    }
-    for (uint32_t q0 = UNROLL * kWave; scan_h && q0 < nvec; q0 += UNROLL * kWave) {
-        uint4 w4[UNROLL];
-#pragma unroll
-        for (uint32_t u = 0; u < UNROLL; ++u) {
-            const uint32_t q = q0 + u * kWave + lane;
-            if (q < nvec) w4[u] = w4p[q];
+        for (uint32_t e = 0; e < 4; ++e) {
+            bal[e] = __ballot((d.defer >> e) & 1u);
+            before[e] = many;
+            many += (uint32_t)__popcll(bal[e]);
         }
+        if (many == 0) return;
+        if (hq_n + many > kHumanQueue) {
+            __syncthreads();
+            human_deferred<HIST>(v, cd, sd, key, t, hq_i, hq_w, hq_n, hc);
+            __syncthreads();
+            hq_n = 0;
+        }
+        human_enqueue(d, q, w4, bal, before, hq_i, hq_w, hq_n);
+        hq_n += many;
+    };
+    if (scan_h) {
 #pragma unroll
-        for (uint32_t u = 0; u < UNROLL; ++u) {
-            const uint32_t q = q0 + u * kWave + lane;
-            if (q < nvec) human_vec<HIST>(v, cd, sd, key, t, q, w4[u], hc);
+        for (uint32_t u = 0; u < UNROLL; ++u) scan_vector(u * kWave + lane, hb[u]);
+        for (uint32_t q0 = UNROLL * kWave; q0 < nvec; q0 += UNROLL * kWave) {
+            uint4 w4[UNROLL];
+#pragma unroll
+            for (uint32_t u = 0; u < UNROLL; ++u) {
+                const uint32_t q = q0 + u * kWave + lane;
+                if (q < nvec) w4[u] = w4p[q];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < UNROLL; ++u) scan_vector(q0 + u * kWave + lane, w4[u]);
+        }
+        if (hq_n) {
+            __syncthreads();
+            human_deferred<HIST>(v, cd, sd, key, t, hq_i, hq_w, hq_n, hc);
+            __syncthreads();
         }
     }
 

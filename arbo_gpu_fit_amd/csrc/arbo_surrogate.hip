@@ -9,11 +9,13 @@
 //  * A workgroup of four wave64s computes a 64 x 64 block of C; each wave one
 //    32 x 32 tile with v_mfma_f32_32x32x16_bf16 (16-deep k-steps, float32
 //    accumulators, 16 per lane).
-//  * The block's A and B panels, 64 x 32 each, are staged through LDS: a
-//    thread loads 8 consecutive elements along the operand's contiguous
-//    dimension (two 16-byte loads where aligned), splits each float into two
-//    bfloat16 halves, hi = bf16(x) and lo = bf16(x - hi), and stores both;
-//    a lane then reads its 8 k-values as one 16-byte LDS read per half.
+//  * The block's A and B panels, 64 rows x 64 k each, are staged through
+//    LDS: a thread loads 16 consecutive elements along the operand's
+//    contiguous dimension (16-byte loads where aligned), splits each float
+//    into two bfloat16 halves, hi = bf16(x) and lo = bf16(x - hi), and
+//    stores both; a lane then reads its 8 k-values as one 16-byte LDS read
+//    per half.  The next panels' loads are issued before the current
+//    panels' MFMAs, so their latency overlaps the arithmetic.
 //  * Each k-step issues three MFMAs, hi*hi + hi*lo + lo*hi: the products
 //    carry about 16 significant bits of every operand (the bf16x3 split),
 //    so gradients hold to float32-like accuracy while every multiply runs
@@ -42,8 +44,9 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr uint32_t kBM = 64, kBN = 64, kBK = 32;   // block tile; 4 waves, one 32 x 32 MFMA tile each
-constexpr uint32_t kLdsK = kBK + 8;                 // padded LDS row of a panel (bf16): 80 bytes
+constexpr uint32_t kBM = 64, kBN = 64, kBK = 64;   // block tile; 4 waves, one 32 x 32 MFMA tile each
+constexpr uint32_t kLdsK = kBK + 8;                 // padded LDS row of a panel (bf16): 144 bytes
+constexpr uint32_t kPer = kBM * kBK / 256;          // panel elements per thread: 16
 
 // float32 -> bfloat16 bits, round to nearest even (NaN kept quiet)
 __host__ __device__ inline uint16_t f2bf(float f) {
@@ -96,62 +99,81 @@ struct Panel {
     uint16_t lo[kBM][kLdsK];
 };
 
-__device__ inline void split_store(float x, uint16_t& hi, uint16_t& lo) {
-    const uint16_t h = f2bf(x);
-    hi = h;
-    lo = f2bf(x - bf2f(h));
-}
-
-// Stages the 64 x 32 panel of operand rows [r0, r0 + 64) and k in [k0, k0 +
-// 32) into LDS.  rows: the operand's rows that exist in memory; with `ones`
-// row `rows` is all ones.  256 threads, 8 elements each.
-__device__ inline void stage(const float* p, uint64_t s_row, uint64_t s_k, uint32_t rows, uint32_t K, uint32_t r0,
-                             uint32_t k0, bool vec, bool ones, Panel& P, uint32_t tid) {
-    float v[8];
-    if (s_k == 1) {   // 8 consecutive k of one row
-        const uint32_t r = tid >> 2, kc = (tid & 3u) * 8u;
-        const uint32_t gr = r0 + r, gk = k0 + kc;
+// One thread's share of a 64 x 64 panel (operand rows [r0, r0 + 64), k in
+// [k0, k0 + 64)): 16 consecutive elements along the operand's contiguous
+// dimension.  rows: the operand's rows that exist in memory; with `ones`
+// row `rows` is all ones.
+__device__ inline void load_panel(const float* p, uint64_t s_row, uint64_t s_k, uint32_t rows, uint32_t K,
+                                  uint32_t r0, uint32_t k0, bool vec, bool ones, uint32_t tid, float* v) {
+    const uint32_t a = tid >> 2, c = (tid & 3u) * kPer;   // k-contiguous: row a, k c..; else k a, rows c..
+    if (s_k == 1) {
+        const uint32_t gr = r0 + a, gk = k0 + c;
         if (gr < rows) {
             const float* q = p + (uint64_t)gr * s_row + gk;
-            if (vec && gk + 8 <= K) {
-                const float4 a = reinterpret_cast<const float4*>(q)[0], b = reinterpret_cast<const float4*>(q)[1];
-                v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+            if (vec && gk + kPer <= K) {
+#pragma unroll
+                for (uint32_t e = 0; e < kPer; e += 4) {
+                    const float4 f = reinterpret_cast<const float4*>(q + e)[0];
+                    v[e] = f.x; v[e + 1] = f.y; v[e + 2] = f.z; v[e + 3] = f.w;
+                }
             } else {
 #pragma unroll
-                for (uint32_t e = 0; e < 8; ++e) v[e] = gk + e < K ? q[e] : 0.0f;
+                for (uint32_t e = 0; e < kPer; ++e) v[e] = gk + e < K ? q[e] : 0.0f;
             }
         } else {
 #pragma unroll
-            for (uint32_t e = 0; e < 8; ++e) v[e] = (ones && gr == rows && gk + e < K) ? 1.0f : 0.0f;
+            for (uint32_t e = 0; e < kPer; ++e) v[e] = (ones && gr == rows && gk + e < K) ? 1.0f : 0.0f;
         }
-        uint16_t h[8], l[8];
-#pragma unroll
-        for (uint32_t e = 0; e < 8; ++e) split_store(v[e], h[e], l[e]);
-        uint4 hv, lv;
-        hv.x = h[0] | ((uint32_t)h[1] << 16); hv.y = h[2] | ((uint32_t)h[3] << 16);
-        hv.z = h[4] | ((uint32_t)h[5] << 16); hv.w = h[6] | ((uint32_t)h[7] << 16);
-        lv.x = l[0] | ((uint32_t)l[1] << 16); lv.y = l[2] | ((uint32_t)l[3] << 16);
-        lv.z = l[4] | ((uint32_t)l[5] << 16); lv.w = l[6] | ((uint32_t)l[7] << 16);
-        *reinterpret_cast<uint4*>(&P.hi[r][kc]) = hv;
-        *reinterpret_cast<uint4*>(&P.lo[r][kc]) = lv;
-    } else {          // 8 consecutive rows of one k (s_row == 1)
-        const uint32_t k = tid >> 3, rc = (tid & 7u) * 8u;
-        const uint32_t gk = k0 + k, gr = r0 + rc;
+    } else {   // s_row == 1
+        const uint32_t gk = k0 + a, gr = r0 + c;
         if (gk < K) {
             const float* q = p + (uint64_t)gk * s_k + gr;
-            if (vec && gr + 8 <= rows) {
-                const float4 a = reinterpret_cast<const float4*>(q)[0], b = reinterpret_cast<const float4*>(q)[1];
-                v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+            if (vec && gr + kPer <= rows) {
+#pragma unroll
+                for (uint32_t e = 0; e < kPer; e += 4) {
+                    const float4 f = reinterpret_cast<const float4*>(q + e)[0];
+                    v[e] = f.x; v[e + 1] = f.y; v[e + 2] = f.z; v[e + 3] = f.w;
+                }
             } else {
 #pragma unroll
-                for (uint32_t e = 0; e < 8; ++e) v[e] = gr + e < rows ? q[e] : ((ones && gr + e == rows) ? 1.0f : 0.0f);
+                for (uint32_t e = 0; e < kPer; ++e)
+                    v[e] = gr + e < rows ? q[e] : ((ones && gr + e == rows) ? 1.0f : 0.0f);
             }
         } else {
 #pragma unroll
-            for (uint32_t e = 0; e < 8; ++e) v[e] = 0.0f;
+            for (uint32_t e = 0; e < kPer; ++e) v[e] = 0.0f;
         }
+    }
+}
+
+__device__ inline uint32_t pack2(uint16_t a, uint16_t b) { return (uint32_t)a | ((uint32_t)b << 16); }
+
+// Splits a thread's share into bfloat16 halves, hi = bf16(x) and lo =
+// bf16(x - hi), and stores both into the panel in [row][k] order.
+__device__ inline void store_panel(const float* v, bool kcontig, uint32_t tid, Panel& P) {
+    const uint32_t a = tid >> 2, c = (tid & 3u) * kPer;
+    uint16_t h[kPer], l[kPer];
 #pragma unroll
-        for (uint32_t e = 0; e < 8; ++e) split_store(v[e], P.hi[rc + e][k], P.lo[rc + e][k]);
+    for (uint32_t e = 0; e < kPer; ++e) {
+        h[e] = f2bf(v[e]);
+        l[e] = f2bf(v[e] - bf2f(h[e]));
+    }
+    if (kcontig) {
+#pragma unroll
+        for (uint32_t e = 0; e < kPer; e += 8) {
+            *reinterpret_cast<uint4*>(&P.hi[a][c + e]) =
+                make_uint4(pack2(h[e], h[e + 1]), pack2(h[e + 2], h[e + 3]), pack2(h[e + 4], h[e + 5]),
+                           pack2(h[e + 6], h[e + 7]));
+            *reinterpret_cast<uint4*>(&P.lo[a][c + e]) =
+                make_uint4(pack2(l[e], l[e + 1]), pack2(l[e + 2], l[e + 3]), pack2(l[e + 4], l[e + 5]),
+                           pack2(l[e + 6], l[e + 7]));
+        }
+    } else {
+#pragma unroll
+        for (uint32_t e = 0; e < kPer; ++e) {
+            P.hi[c + e][a] = h[e];
+            P.lo[c + e][a] = l[e];
+        }
     }
 }
 
@@ -164,11 +186,20 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     const uint32_t i0 = blockIdx.y * kBM, j0 = blockIdx.x * kBN;
     const uint32_t wi = (wave >> 1) * 32u, wj = (wave & 1u) * 32u;
     const uint32_t rowsB = g.ones ? g.N - 1 : g.N;
+    const bool ka = g.sAk == 1, kb = g.sBk == 1;
     f32x16 acc = {};
+    // the next panels' elements are loaded while the current ones multiply
+    float ra[kPer], rb[kPer];
+    load_panel(g.A, g.sAi, g.sAk, g.M, g.K, i0, 0, g.vecA != 0, false, tid, ra);
+    load_panel(g.B, g.sBj, g.sBk, rowsB, g.K, j0, 0, g.vecB != 0, g.ones != 0, tid, rb);
     for (uint32_t k0 = 0; k0 < g.K; k0 += kBK) {
-        stage(g.A, g.sAi, g.sAk, g.M, g.K, i0, k0, g.vecA != 0, false, sa, tid);
-        stage(g.B, g.sBj, g.sBk, rowsB, g.K, j0, k0, g.vecB != 0, g.ones != 0, sb, tid);
+        store_panel(ra, ka, tid, sa);
+        store_panel(rb, kb, tid, sb);
         __syncthreads();
+        if (k0 + kBK < g.K) {
+            load_panel(g.A, g.sAi, g.sAk, g.M, g.K, i0, k0 + kBK, g.vecA != 0, false, tid, ra);
+            load_panel(g.B, g.sBj, g.sBk, rowsB, g.K, j0, k0 + kBK, g.vecB != 0, g.ones != 0, tid, rb);
+        }
 #pragma unroll
         for (uint32_t kk = 0; kk < kBK; kk += 16) {
             const uint32_t kq = kk + 8u * (lane >> 5);
