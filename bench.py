@@ -175,7 +175,7 @@ def sim_bench(workload: str, args, lib) -> dict:
     days = warmup + steps
     if days > days_cfg:
         raise SystemExit(f"{workload}: warm-up + steps = {days} exceeds the configured {days_cfg} days")
-    graph = plain or args.graph
+    graph = (plain and not args.no_graph) or args.graph
     eng = Engine(lib, device=0, flags=abi.F_GRAPH if graph else abi.F_TIMING)
     t0 = time.perf_counter()
     eng.init(specs, days)
@@ -439,6 +439,8 @@ def main():
     ap.add_argument("--cpu-sims", type=int, default=0,
                     help="simulations of the batch in the CPU sample (0: 64 for windows up to 60 days, else 8)")
     ap.add_argument("--graph", action="store_true", help="batch: replay the day loop as hipGraphs")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="plain: launch every kernel on its own (for profilers; the same kernels and days)")
     ap.add_argument("--autotune-days", type=int, default=2,
                     help="days per candidate launch tuning, timed within the warm-up (0: default tuning)")
     ap.add_argument("--mcmc", action="store_true", help="MCMC ESS/hour, plain vs surrogate, instead")
