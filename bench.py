@@ -290,7 +290,8 @@ def sim_bench(workload: str, args, lib) -> dict:
 
 def mcmc_bench(args) -> dict:
     """MCMC ESS/hour, plain batched adaptive Metropolis vs surrogate delayed
-    acceptance, on the 6-parameter model (1e6 humans, 1,024 communities, 260
+    acceptance, on the 7-parameter model (five epidemic parameters, the
+    reporting rate and the NB dispersion; 1e6 humans, 1,024 communities, 260
     weekly counts, synthetic data at the true parameters):
       1. `train_runs` simulations at prior draws -> weekly incidence;
       2. the surrogate MLP trained on them (Poisson loss, Adam, `epochs`);
@@ -299,9 +300,11 @@ def mcmc_bench(args) -> dict:
       4. `iters` iterations of plain adaptive Metropolis and of Langevin
          delayed acceptance, `screen_iters` of screened delayed acceptance
          (cheaper iterations: only proposals the surrogate passes are
-         simulated); 64 chains, one simulation per proposal; ESS over the
-         iterations after the first quarter, per hour of that sampler's own
-         wall clock."""
+         simulated); 64 chains, `particles` simulations per proposal
+         (their likelihoods averaged; with `blocks` > 0 the particle seeds
+         are kept in the chain state and one block refreshed per iteration);
+         ESS over the iterations after the first quarter, per hour of that
+         sampler's own wall clock."""
     import numpy as np
 
     from arbo_gpu_fit_amd import abi, configs, fit, mcmc
@@ -366,7 +369,8 @@ def mcmc_bench(args) -> dict:
                      "loss_first": float(hist[0]), "loss_last": float(hist[-1])},
         "truth": dict(zip(mcmc.PARAM_NAMES, configs.MCMC_TRUTH.tolist())),
         "data": "synthetic (the model at the true parameters, NB(10) observations)",
-        "config": {"workload": f"MCMC fit: {args.chains} chains x 1 simulation per proposal, 1e6 humans in "
+        "config": {"workload": f"MCMC fit: {args.chains} chains x {args.particles} simulation(s) per proposal"
+                               f"{f' in {args.blocks} seed blocks' if args.blocks else ''}, 1e6 humans in "
                                f"32x32 communities, 1,819 days, 260 weekly counts; {args.iters} iterations plain "
                                f"and Langevin delayed acceptance, {args.screen_iters} screened",
                    "sub_steps": args.sub_steps},
