@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 MAX_SEROTYPES = 4
 MAX_COMMUNITY_HUMANS = 8192
 MAX_COMMUNITY_SLOTS = 8192
@@ -83,7 +83,7 @@ class State(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("census_unroll", _u32), ("early_mosq", _u32), ("xcd_order", _u32), ("ld_split", _u32),
-                ("waves_per_simd", _u32)]
+                ("waves_per_simd", _u32), ("waves_per_comm", _u32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -163,9 +163,11 @@ struct Tuning {
                          // contiguous range of communities
     uint32_t ld_split;   // long-distance kernel: workgroups per simulation
     uint32_t wps;        // demography: 0 the compiler's register budget, 8 eight waves per SIMD
+    uint32_t nw;         // demography: waves per community (1, 2, 4)
 };
 constexpr uint32_t kDemographyShapes[][2] = {{4, 4}, {4, 2}, {8, 4}, {2, 2}};   // (unroll, early_m)
 constexpr uint32_t kOccShapes[][2] = {{4, 2}, {2, 2}};   // shapes also built for 8 waves per SIMD
+constexpr uint32_t kWideShapes[][3] = {{4, 2, 2}, {2, 2, 4}};   // (unroll, early_m, waves per community)
 constexpr uint32_t kMaxLdSplit = 64;
 constexpr uint32_t kXcds = 8;   // gfx950: workgroups go to the 8 XCDs in turn
 

@@ -74,7 +74,7 @@ struct ArboEngine {
     // timing
     std::vector<hipEvent_t> ev_pool, ev_used;
     double kernel_ms[2] = {0.0, 0.0};
-    Tuning tuning{4, 4, 0, 8, 0};
+    Tuning tuning{4, 4, 0, 8, 0, 1};
     uint64_t kernel_launches[2] = {0, 0};
 };
 
@@ -524,9 +524,9 @@ int arbo_batch_step(ArboEngine* e, uint32_t n_days) {
 namespace {
 
 Tuning from_abi(const ArboTuning& t) {
-    return Tuning{t.census_unroll, t.early_mosq, t.xcd_order, t.ld_split, t.waves_per_simd};
+    return Tuning{t.census_unroll, t.early_mosq, t.xcd_order, t.ld_split, t.waves_per_simd, t.waves_per_comm};
 }
-ArboTuning to_abi(const Tuning& t) { return ArboTuning{t.unroll, t.early_m, t.xcd, t.ld_split, t.wps}; }
+ArboTuning to_abi(const Tuning& t) { return ArboTuning{t.unroll, t.early_m, t.xcd, t.ld_split, t.wps, t.nw}; }
 
 // Sets the launch knobs.  The captured graph holds the old kernels: it is
 // released, and rebuilt at once unless `rebuild` is false (the autotuner
@@ -549,11 +549,15 @@ int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n) {
     uint32_t k = 0;
     for (const auto& d : kDemographyShapes)
         for (uint32_t x = 0; x <= 3; x += 3) {
-            if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8, 0};
+            if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8, 0, 1};
             ++k;
         }
     for (const auto& d : kOccShapes) {   // eight waves per SIMD: for batches of few communities
-        if (out && k < cap) out[k] = ArboTuning{d[0], d[1], 0, 8, 8};
+        if (out && k < cap) out[k] = ArboTuning{d[0], d[1], 0, 8, 8, 1};
+        ++k;
+    }
+    for (const auto& d : kWideShapes) {   // several waves per community: for batches of few communities
+        if (out && k < cap) out[k] = ArboTuning{d[0], d[1], 0, 8, 0, d[2]};
         ++k;
     }
     *n = k;

@@ -90,7 +90,7 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
 
 // Exclusive prefix sum over the wavefront; `total` gets the wave sum.
 __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, uint32_t& total) {
-    const uint32_t lane = threadIdx.x;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     uint32_t incl = x;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -377,8 +377,8 @@ __device__ __forceinline__ void human_enqueue(const HumanDefer& d, uint32_t q, u
 template <bool HIST>
 __device__ __forceinline__ void human_deferred(const EngineView& v, const CommDesc& cd, const SimDesc& sd, Key key,
                                                uint32_t t, const uint32_t* qi, const uint32_t* qw, uint32_t n,
-                                               HumanCount& hc) {
-    for (uint32_t k = threadIdx.x; k < n; k += kWave) {
+                                               uint32_t lane, HumanCount& hc) {
+    for (uint32_t k = lane; k < n; k += kWave) {
         const uint32_t e = qi[k], i = e & kKeyMask;
         const uint64_t gi = cd.h_off + i;
         if (!(e & (1u << 14))) {   // death and instant rebirth
@@ -403,11 +403,19 @@ __device__ __forceinline__ void human_deferred(const EngineView& v, const CommDe
 // (v.hist or v.spatial not null); a separate instantiation keeps their
 // stores out of the plain day loop.
 // WPS: 0, or the resident waves per SIMD the register budget must allow.
-template <uint32_t UNROLL, uint32_t EARLY_M, bool XCD, bool HIST, uint32_t WPS>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPS ? WPS : 1)))
+// NW: waves per community (workgroup of 64 NW threads).  The census loops
+// stride over the workgroup, each wave queues its own deferred human changes,
+// and the counts, the dead-slot prefix and the high-water mark are combined
+// across the waves through LDS; a community may hold up to NW times
+// kMaxWords bitmap words of slots (four per thread).
+template <uint32_t UNROLL, uint32_t EARLY_M, bool XCD, bool HIST, uint32_t WPS, uint32_t NW>
+__global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(WPS ? WPS : 1)))
 void k_day_demography(EngineView v) {
     static_assert(EARLY_M >= 1 && EARLY_M <= UNROLL, "early mosquito batch within the first unrolled batch");
-    const uint32_t g = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, lane = threadIdx.x;
+    static_assert(NW == 1 || NW == 2 || NW == 4 || NW == 8, "one to eight waves per community");
+    constexpr uint32_t NT = kWave * NW;   // threads per community
+    const uint32_t g = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, tid = threadIdx.x;
+    const uint32_t lane = NW == 1 ? tid : tid & (kWave - 1), wv = NW == 1 ? 0u : tid / kWave;
     const uint32_t t = *v.day;
     const CommDesc cd = v.comm[g];
     // Humans are scanned only on days when one of them may die or change
@@ -427,12 +435,12 @@ void k_day_demography(EngineView v) {
     uint4 hb[UNROLL], mb[EARLY_M];
 #pragma unroll
     for (uint32_t u = 0; u < UNROLL; ++u) {
-        const uint32_t q = u * kWave + lane;
+        const uint32_t q = u * NT + tid;
         if (scan_h && q < nvec) hb[u] = w4p[q];
     }
 #pragma unroll
     for (uint32_t u = 0; u < EARLY_M; ++u) {
-        const uint32_t q = u * kWave + lane;
+        const uint32_t q = u * NT + tid;
         if (q < mlim) mb[u] = m4p[q];
     }
     const SimDesc sd = v.sims[cd.sim];
@@ -440,9 +448,12 @@ void k_day_demography(EngineView v) {
     const uint32_t S = sd.S;
 
     extern __shared__ uint32_t s_dtab[];                            // mosquito lifetime thresholds
-    __shared__ uint32_t s_dead[kMaxWords];                          // dead-slot bitmap
-    __shared__ uint32_t s_died[kMaxWords];                          // died today
-    __shared__ uint32_t s_dpre[kMaxWords];                          // dead slots before each word
+    __shared__ uint32_t s_dead[kMaxWords * NW];                     // dead-slot bitmap
+    __shared__ uint32_t s_died[kMaxWords * NW];                     // died today
+    __shared__ uint32_t s_dpre[kMaxWords * NW];                     // dead slots before each word
+    __shared__ uint32_t s_red[NW][8];                               // per-wave counts (NW > 1)
+    __shared__ uint32_t s_wtot[NW];                                 // per-wave dead slots (NW > 1)
+    __shared__ uint32_t s_wtop[NW];                                 // per-wave high-water marks (NW > 1)
     __shared__ uint32_t s_mguide[kLifeBuckets];                     // death-age guide of the lifetime table
     __shared__ uint32_t s_row[kMaxRowLen];                          // alias row of the community
     __shared__ uint32_t s_n[8];
@@ -460,49 +471,62 @@ void k_day_demography(EngineView v) {
         const uint32_t row = birth_row(sd, t);   // today's births
         const uint32_t* dt = v.tables + sd.death_m_off + (size_t)row * nd;
         const uint32_t* gt = v.tables + sd.death_m_guide_off + (size_t)row * kGuide;
-        constexpr uint32_t kT = 4, kG = (kLifeBuckets + kWave - 1) / kWave;
+        constexpr uint32_t kT = (4 + NW - 1) / NW, kG = (kLifeBuckets + NT - 1) / NT;
         uint32_t tv[kT], gv[kG];
 #pragma unroll
         for (uint32_t k = 0; k < kT; ++k) {
-            const uint32_t a = k * kWave + lane;
+            const uint32_t a = k * NT + tid;
             tv[k] = a < nd ? dt[a] : 0u;
         }
 #pragma unroll
         for (uint32_t k = 0; k < kG; ++k) {
-            const uint32_t b = k * kWave + lane;
+            const uint32_t b = k * NT + tid;
             gv[k] = b < kLifeBuckets ? gt[b] : 0u;
         }
 #pragma unroll
         for (uint32_t k = 0; k < kT; ++k) {
-            const uint32_t a = k * kWave + lane;
+            const uint32_t a = k * NT + tid;
             if (a < nd) s_dtab[a] = tv[k];
         }
-        for (uint32_t a = kT * kWave + lane; a < nd; a += kWave) s_dtab[a] = dt[a];   // long lifetimes
+        for (uint32_t a = kT * NT + tid; a < nd; a += NT) s_dtab[a] = dt[a];   // long lifetimes
 #pragma unroll
         for (uint32_t k = 0; k < kG; ++k) {
-            const uint32_t b = k * kWave + lane;
+            const uint32_t b = k * NT + tid;
             if (b < kLifeBuckets) s_mguide[b] = gv[k];
         }
     }
-    if (lane < cd.row_len) s_row[lane] = v.csr_thr[cd.row_off + lane];
-    const uint32_t inboxent = lane < cd.row_len ? v.csr_inbox[cd.row_off + lane] : 0u;
-    for (uint32_t i = lane; i < kCntPerComm; i += kWave) s_hist[i] = 0;
+    if (tid < cd.row_len) s_row[tid] = v.csr_thr[cd.row_off + tid];
+    const uint32_t inboxent = tid < cd.row_len ? v.csr_inbox[cd.row_off + tid] : 0u;
+    for (uint32_t i = tid; i < kCntPerComm; i += NT) s_hist[i] = 0;
     // emergence target of the day: the next day's expected population E[t+1]
     // (simulation.cpp:129-132), drawn once the census is counted
     const float dens_next = v.dens[sd.dens_off + t + 1];
 
     // ---- humans ----
     HumanCount hc{0, 0, 0, 0, 0xFFFFFFFFu};
-    if (!scan_h && lane == 0) {
+    if (!scan_h && tid == 0) {
         hc.ih01 = hs.y;
         hc.ih23 = hs.z;
     }
     // Changes that need a random draw queue up in LDS (the dead-slot bitmap
-    // arrays, not used before the mosquito scan) and are processed a lane
-    // each when the queue could overflow and after the scan.
-    uint32_t* hq_i = s_dead;
-    uint32_t* hq_w = s_died;
+    // arrays, not used before the mosquito scan; one queue per wave) and are
+    // processed a lane each when the queue could overflow and after the scan.
+    // With one wave the queue is guarded by workgroup barriers; with several
+    // the waves flush at different times, so each wave waits for its own LDS
+    // operations instead (s_waitcnt lgkmcnt(0)), behind a fence that keeps
+    // the compiler from moving memory accesses across.
+    uint32_t* hq_i = s_dead + wv * kHumanQueue;
+    uint32_t* hq_w = s_died + wv * kHumanQueue;
     uint32_t hq_n = 0;   // wave-uniform
+    auto queue_sync = []() {
+        if (NW == 1) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_s_waitcnt(0xC07F);   // gfx9 encoding: lgkmcnt(0), vmcnt and expcnt unconstrained
+            __builtin_amdgcn_wave_barrier();
+        }
+    };
     static_assert(kMaxWords >= kHumanQueue, "the human queue fits the bitmap arrays");
     auto scan_vector = [&](uint32_t q, uint4 w4) {   // every lane calls it; lanes past nvec add nothing
         HumanDefer d{0u, 0u, {0u, 0u, 0u, 0u}};
@@ -517,9 +541,9 @@ void k_day_demography(EngineView v) {
         }
         if (many == 0) return;
         if (hq_n + many > kHumanQueue) {
-            __syncthreads();
-            human_deferred<HIST>(v, cd, sd, key, t, hq_i, hq_w, hq_n, hc);
-            __syncthreads();
+            queue_sync();
+            human_deferred<HIST>(v, cd, sd, key, t, hq_i, hq_w, hq_n, lane, hc);
+            queue_sync();
             hq_n = 0;
         }
         human_enqueue(d, q, w4, bal, before, hq_i, hq_w, hq_n);
@@ -527,23 +551,24 @@ void k_day_demography(EngineView v) {
     };
     if (scan_h) {
 #pragma unroll
-        for (uint32_t u = 0; u < UNROLL; ++u) scan_vector(u * kWave + lane, hb[u]);
-        for (uint32_t q0 = UNROLL * kWave; q0 < nvec; q0 += UNROLL * kWave) {
+        for (uint32_t u = 0; u < UNROLL; ++u) scan_vector(u * NT + tid, hb[u]);
+        for (uint32_t q0 = UNROLL * NT; q0 < nvec; q0 += UNROLL * NT) {
             uint4 w4[UNROLL];
 #pragma unroll
             for (uint32_t u = 0; u < UNROLL; ++u) {
-                const uint32_t q = q0 + u * kWave + lane;
+                const uint32_t q = q0 + u * NT + tid;
                 if (q < nvec) w4[u] = w4p[q];
             }
 #pragma unroll
-            for (uint32_t u = 0; u < UNROLL; ++u) scan_vector(q0 + u * kWave + lane, w4[u]);
+            for (uint32_t u = 0; u < UNROLL; ++u) scan_vector(q0 + u * NT + tid, w4[u]);
         }
         if (hq_n) {
-            __syncthreads();
-            human_deferred<HIST>(v, cd, sd, key, t, hq_i, hq_w, hq_n, hc);
-            __syncthreads();
+            queue_sync();
+            human_deferred<HIST>(v, cd, sd, key, t, hq_i, hq_w, hq_n, lane, hc);
+            queue_sync();
         }
     }
+    if (NW > 1) __syncthreads();   // every wave's queue is done before the bitmap bytes are written
 
     // ---- mosquitoes: death and E->I (mdemographic.cu:68-98), eight slots per u32 op ----
     // A 16-byte vector holds slots 0..7 as four u32 pairs.  The status and
@@ -555,25 +580,25 @@ void k_day_demography(EngineView v) {
         // per 16-bit half: t mod 4096 plus a 4096 guard, so (t - death day) mod 4096
         // is the half's low 12 bits with no borrow between halves
         const uint32_t T2 = ((t & 0xFFFu) * 0x00010001u) | 0x10001000u;
-        for (uint32_t q0 = 0; q0 < mlim; q0 += UNROLL * kWave) {
+        for (uint32_t q0 = 0; q0 < mlim; q0 += UNROLL * NT) {
             uint4 raw[UNROLL];
             if (q0 == 0) {
 #pragma unroll
                 for (uint32_t u = 0; u < UNROLL; ++u) {
-                    const uint32_t q = u * kWave + lane;
+                    const uint32_t q = u * NT + tid;
                     if (u < EARLY_M) raw[u] = mb[u < EARLY_M ? u : 0];
                     else if (q < mlim) raw[u] = m4p[q];
                 }
             } else {
 #pragma unroll
                 for (uint32_t u = 0; u < UNROLL; ++u) {
-                    const uint32_t q = q0 + u * kWave + lane;
+                    const uint32_t q = q0 + u * NT + tid;
                     if (q < mlim) raw[u] = m4p[q];
                 }
             }
 #pragma unroll
             for (uint32_t u = 0; u < UNROLL; ++u) {
-                const uint32_t q = q0 + u * kWave + lane;
+                const uint32_t q = q0 + u * NT + tid;
                 if (q >= mlim) continue;
                 const uint32_t rw[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
                 uint32_t st = 0, sn = 0, dy = 0;
@@ -619,18 +644,33 @@ void k_day_demography(EngineView v) {
         }
         // the empty slots not read (all dead, none died today) and the bytes
         // past the capacity in the last bitmap word
-        for (uint32_t q = mlim + lane; q < 4 * nwords; q += kWave) {
+        for (uint32_t q = mlim + tid; q < 4 * nwords; q += NT) {
             const uint32_t left = q < mvec ? cd.cap - 8 * q : 0u;
             s_dead8[q] = (uint8_t)(left >= 8 ? 0xFFu : ((1u << left) - 1u));
             s_died8[q] = 0;
         }
     }
-    const uint32_t ih01 = wave_sum(hc.ih01), ih23 = wave_sum(hc.ih23);
-    const uint32_t inc01 = wave_sum(hc.inc01), inc23 = wave_sum(hc.inc23);
+    uint32_t ih01 = wave_sum(hc.ih01), ih23 = wave_sum(hc.ih23);
+    uint32_t inc01 = wave_sum(hc.inc01), inc23 = wave_sum(hc.inc23);
     im01 = wave_sum(im01);
     im23 = wave_sum(im23);
-    const uint32_t minev = scan_h ? wave_min(hc.minev) : hs.x;
-    const uint32_t alive_c = wave_sum(alive);
+    uint32_t minev = scan_h ? wave_min(hc.minev) : hs.x;
+    uint32_t alive_c = wave_sum(alive);
+    if (NW > 1) {   // the waves' counts through LDS (16-bit pair halves cannot carry: each is at most 8,192 NW)
+        if (lane == 0) {
+            s_red[wv][0] = ih01; s_red[wv][1] = ih23; s_red[wv][2] = inc01; s_red[wv][3] = inc23;
+            s_red[wv][4] = im01; s_red[wv][5] = im23; s_red[wv][6] = minev; s_red[wv][7] = alive_c;
+        }
+        __syncthreads();
+        ih01 = ih23 = inc01 = inc23 = im01 = im23 = alive_c = 0;
+        minev = 0xFFFFFFFFu;
+#pragma unroll
+        for (uint32_t w = 0; w < NW; ++w) {
+            ih01 += s_red[w][0]; ih23 += s_red[w][1]; inc01 += s_red[w][2]; inc23 += s_red[w][3];
+            im01 += s_red[w][4]; im23 += s_red[w][5]; alive_c += s_red[w][7];
+            minev = s_red[w][6] < minev ? s_red[w][6] : minev;
+        }
+    }
     const uint32_t dead_c = cd.cap - alive_c;
 
     // ---- emergence into the lowest-index dead slots (mdemographic.cu:104-142) ----
@@ -647,16 +687,22 @@ void k_day_demography(EngineView v) {
         uint32_t cnt = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kWordsPerLane; ++k) {
-            const uint32_t w = kWordsPerLane * lane + k;
+            const uint32_t w = kWordsPerLane * tid + k;
             dw[k] = w < nwords ? s_dead[w] : 0u;
             kw[k] = w < nwords ? s_died[w] : 0u;
             cnt += __popc(dw[k]);
         }
         uint32_t total;
         uint32_t pre = wave_exclusive_scan(cnt, total);
+        if (NW > 1) {   // plus the dead slots of the waves before
+            if (lane == 0) s_wtot[wv] = total;
+            __syncthreads();
+#pragma unroll
+            for (uint32_t w = 0; w < NW; ++w) pre += w < wv ? s_wtot[w] : 0u;
+        }
 #pragma unroll
         for (uint32_t k = 0; k < kWordsPerLane; ++k) {
-            const uint32_t w = kWordsPerLane * lane + k;
+            const uint32_t w = kWordsPerLane * tid + k;
             wpre[k] = pre;
             if (w < nwords) s_dpre[w] = pre;
             pre += __popc(dw[k]);
@@ -664,7 +710,7 @@ void k_day_demography(EngineView v) {
     }
     __syncthreads();
     // births i = 0..births-1 fill the i-th dead slot; births 4g..4g+3 share one Philox block
-    for (uint32_t g4 = lane; 4 * g4 < births; g4 += kWave) {
+    for (uint32_t g4 = tid; 4 * g4 < births; g4 += NT) {
         const U4 r = draw(key, cd.c, t, P_M_BIRTH, g4);
         const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
         uint32_t w, pos;
@@ -686,7 +732,7 @@ void k_day_demography(EngineView v) {
     uint32_t awv[kWordsPerLane] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (uint32_t k = 0; k < kWordsPerLane; ++k) {
-        const uint32_t w = kWordsPerLane * lane + k;
+        const uint32_t w = kWordsPerLane * tid + k;
         if (w >= nwords) continue;
         const uint32_t d = dw[k], c = __popc(d);
         uint32_t reborn;
@@ -705,16 +751,22 @@ void k_day_demography(EngineView v) {
     }
     // the lane's four bitmap words in one store (communities start on 128-slot
     // boundaries and own whole multiples of 128 slots)
-    if (kWordsPerLane * lane < nwords)
-        reinterpret_cast<uint4*>(v.alive + (cd.m_off >> 5))[lane] = make_uint4(awv[0], awv[1], awv[2], awv[3]);
+    if (kWordsPerLane * tid < nwords)
+        reinterpret_cast<uint4*>(v.alive + (cd.m_off >> 5))[tid] = make_uint4(awv[0], awv[1], awv[2], awv[3]);
     top = wave_max(top);
-    if (lane == 0)   // the day's summary: human part (unchanged when not scanned) and mosquito limit
+    if (NW > 1) {
+        if (lane == 0) s_wtop[wv] = top;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t w = 0; w < NW; ++w) top = s_wtop[w] > top ? s_wtop[w] : top;
+    }
+    if (tid == 0)   // the day's summary: human part (unchanged when not scanned) and mosquito limit
         reinterpret_cast<uint4*>(v.hsum)[g] = make_uint4(minev, ih01, ih23, ((top + 7) >> 3) + 1u);
     const uint32_t Mc = alive_c + births;
 
     // ---- event counts of this community as a source (nepidemic.cu:49-136) ----
-    if (lane < 8) {
-        const uint32_t dir = lane >> 2, s = lane & 3u;
+    if (tid < 8) {
+        const uint32_t dir = tid >> 2, s = tid & 3u;
         uint32_t n = 0;
         unsigned long long fx = 0;
         if (s < S) {
@@ -746,13 +798,13 @@ void k_day_demography(EngineView v) {
                     }
                 }
                 // staying events join the count of the source's own column
-                if (stay) atomicAdd(&s_hist[lane * 9 + ((cd.flags >> kCommSelfShift) & 15u)], stay);
+                if (stay) atomicAdd(&s_hist[tid * 9 + ((cd.flags >> kCommSelfShift) & 15u)], stay);
             }
         }
-        s_n[lane] = n;
+        s_n[tid] = n;
         // summed over the simulation by k_day_reduce (no same-address atomics
         // across the simulation's communities)
-        reinterpret_cast<unsigned long long*>(v.part + (size_t)g * kPartWords + 8)[lane] = fx;
+        reinterpret_cast<unsigned long long*>(v.part + (size_t)g * kPartWords + 8)[tid] = fx;
     }
     __syncthreads();
     // Destinations: event i of source segment k takes word i%4 of Philox
@@ -764,7 +816,7 @@ void k_day_demography(EngineView v) {
     qpre[0] = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) qpre[k + 1] = qpre[k] + ((s_n[k] + 3) >> 2);
-    for (uint32_t gq = lane; gq < qpre[8]; gq += kWave) {
+    for (uint32_t gq = tid; gq < qpre[8]; gq += NT) {
         uint32_t k = 0;
 #pragma unroll
         for (uint32_t q = 1; q < 8; ++q) k += (gq >= qpre[q]) ? 1u : 0u;
@@ -784,7 +836,7 @@ void k_day_demography(EngineView v) {
     }
     __syncthreads();
     // push the counts into each destination's inbox: 8 contiguous words per (destination, in-edge)
-    for (uint32_t i0 = 0; i0 < 8 * rl; i0 += kWave) {   // wave-uniform trip count: every lane shuffles
+    for (uint32_t i0 = 0; wv == 0 && i0 < 8 * rl; i0 += kWave) {   // wave 0, every lane shuffles
         const uint32_t i = i0 + lane, col = i >> 3, k = i & 7u;
         const uint32_t dst = __shfl(inboxent, col < rl ? col : 0u, kWave);
         if (i < 8 * rl) {
@@ -797,15 +849,15 @@ void k_day_demography(EngineView v) {
     // ---- series row t+1 (datacollect.cu:79-189): this community's part ----
     const SeriesIdx si = series_index(sd.series_off, v.max_days, S);
     const uint32_t row = t + 1;
-    if (lane == 0)
+    if (tid == 0)
         reinterpret_cast<uint4*>(v.part + (size_t)g * kPartWords)[0] = make_uint4(Mc, cd.n, ih01, ih23);
-    else if (lane == 1)   // with the census vectors this scan read (the exact skips leave the rest)
+    else if (tid == 1)   // with the census vectors this scan read (the exact skips leave the rest)
         reinterpret_cast<uint4*>(v.part + (size_t)g * kPartWords + 4)[0] =
             make_uint4(inc01, inc23, scan_h ? nvec : 0u, mlim);
-    if (lane < S) {
-        const uint32_t ih = unpack_pair(ih01, ih23, lane);
-        if (cd.c == 0) v.series[si.n_one + (size_t)row * S + lane] = ih;
-        if (HIST && v.spatial) v.spatial[sd.spatial_off + ((uint64_t)row * S + lane) * sd.C + cd.c] = ih;
+    if (tid < S) {
+        const uint32_t ih = unpack_pair(ih01, ih23, tid);
+        if (cd.c == 0) v.series[si.n_one + (size_t)row * S + tid] = ih;
+        if (HIST && v.spatial) v.spatial[sd.spatial_off + ((uint64_t)row * S + tid) * sd.C + cd.c] = ih;
     }
 }
 
@@ -1346,26 +1398,40 @@ hipError_t launch_init(const EngineView& v, hipStream_t st) {
     return hipGetLastError();
 }
 
-// Demography variants: (UNROLL, EARLY_M) pairs of kDemographyShapes.
+// Demography variants: (UNROLL, EARLY_M) pairs of kDemographyShapes with one
+// wave per community, the kOccShapes pairs under an 8-waves-per-SIMD register
+// budget, and the kWideShapes with several waves per community.
 template <bool XCD, bool HIST>
 static hipError_t launch_demography_xh(const EngineView& v, const Tuning& tu, hipStream_t st) {
-    const dim3 grid(v.n_comm), block(kWave);
+    const dim3 grid(v.n_comm), block(kWave * tu.nw);
     const size_t lds = (size_t)v.death_m_lds * 4;
-    if (tu.wps == 8) {
+    if (tu.nw == 2) {
+        if (tu.unroll == 4 && tu.early_m == 2 && tu.wps == 0)
+            hipLaunchKernelGGL((k_day_demography<4, 2, XCD, HIST, 0, 2>), grid, block, lds, st, v);
+        else
+            return hipErrorInvalidValue;
+    } else if (tu.nw == 4) {
+        if (tu.unroll == 2 && tu.early_m == 2 && tu.wps == 0)
+            hipLaunchKernelGGL((k_day_demography<2, 2, XCD, HIST, 0, 4>), grid, block, lds, st, v);
+        else
+            return hipErrorInvalidValue;
+    } else if (tu.nw != 1) {
+        return hipErrorInvalidValue;
+    } else if (tu.wps == 8) {
         if (tu.unroll == 4 && tu.early_m == 2)
-            hipLaunchKernelGGL((k_day_demography<4, 2, XCD, HIST, 8>), grid, block, lds, st, v);
+            hipLaunchKernelGGL((k_day_demography<4, 2, XCD, HIST, 8, 1>), grid, block, lds, st, v);
         else if (tu.unroll == 2 && tu.early_m == 2)
-            hipLaunchKernelGGL((k_day_demography<2, 2, XCD, HIST, 8>), grid, block, lds, st, v);
+            hipLaunchKernelGGL((k_day_demography<2, 2, XCD, HIST, 8, 1>), grid, block, lds, st, v);
         else
             return hipErrorInvalidValue;
     } else if (tu.unroll == 4 && tu.early_m == 4)
-        hipLaunchKernelGGL((k_day_demography<4, 4, XCD, HIST, 0>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<4, 4, XCD, HIST, 0, 1>), grid, block, lds, st, v);
     else if (tu.unroll == 4 && tu.early_m == 2)
-        hipLaunchKernelGGL((k_day_demography<4, 2, XCD, HIST, 0>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<4, 2, XCD, HIST, 0, 1>), grid, block, lds, st, v);
     else if (tu.unroll == 8 && tu.early_m == 4)
-        hipLaunchKernelGGL((k_day_demography<8, 4, XCD, HIST, 0>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<8, 4, XCD, HIST, 0, 1>), grid, block, lds, st, v);
     else if (tu.unroll == 2 && tu.early_m == 2)
-        hipLaunchKernelGGL((k_day_demography<2, 2, XCD, HIST, 0>), grid, block, lds, st, v);
+        hipLaunchKernelGGL((k_day_demography<2, 2, XCD, HIST, 0, 1>), grid, block, lds, st, v);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1379,7 +1445,12 @@ static hipError_t launch_demography_x(const EngineView& v, const Tuning& tu, hip
 
 bool tuning_supported(const Tuning& tu) {
     bool shape = false;
-    if (tu.wps == 8) {
+    if (tu.nw > 1) {
+        for (const auto& d : kWideShapes)
+            shape = shape || (d[0] == tu.unroll && d[1] == tu.early_m && d[2] == tu.nw && tu.wps == 0);
+    } else if (tu.nw != 1) {
+        shape = false;
+    } else if (tu.wps == 8) {
         for (const auto& d : kOccShapes) shape = shape || (d[0] == tu.unroll && d[1] == tu.early_m);
     } else if (tu.wps == 0) {
         for (const auto& d : kDemographyShapes) shape = shape || (d[0] == tu.unroll && d[1] == tu.early_m);

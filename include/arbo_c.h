@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define ARBO_ABI_VERSION 6u
+#define ARBO_ABI_VERSION 7u
 #define ARBO_MAX_SEROTYPES 4u
 #define ARBO_MAX_COMMUNITY_HUMANS 8192u
 #define ARBO_MAX_COMMUNITY_SLOTS 8192u
@@ -290,11 +290,13 @@ typedef struct ArboTuning {
     uint32_t waves_per_simd;  /* demography kernel: 0 the compiler's register budget, 8 one that
                                  keeps 8 waves per SIMD (with census_unroll/early_mosq 4/2 or
                                  2/2): more resident waves for batches of few communities */
+    uint32_t waves_per_comm;  /* demography kernel: waves per community, 1, or 2 (with 4/2) or 4
+                                 (with 2/2): shorter waves for batches of few communities */
 } ArboTuning;
 
 /* The tunings arbo_autotune tries (out may be NULL to count them). */
 ARBO_API int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n);
-/* Set or read the engine's tuning (default {4, 4, 0, 8, 0}); ARBO_E_INVALID for a
+/* Set or read the engine's tuning (default {4, 4, 0, 8, 0, 1}); ARBO_E_INVALID for a
  * combination the library was not built with. */
 ARBO_API int arbo_set_tuning(ArboEngine* e, const ArboTuning* t);
 ARBO_API int arbo_get_tuning(const ArboEngine* e, ArboTuning* out);

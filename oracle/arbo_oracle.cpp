@@ -110,7 +110,7 @@ struct ArboEngine {
     int threads = 0;
     bool ready = false;
     std::string err;
-    ArboTuning tuning{4, 4, 0, 8, 0};   // launch knobs: accepted and reported, no effect on the CPU
+    ArboTuning tuning{4, 4, 0, 8, 0, 1};   // launch knobs: accepted and reported, no effect on the CPU
     uint32_t flags = 0;
     uint64_t scan_h = 0, scan_m = 0;    // census vectors of the days stepped (every member, every day)
     uint64_t full_h = 0, full_m = 0;    // one full scan: 4-human and 8-slot vectors of every community
@@ -1094,17 +1094,22 @@ int arbo_scan_counts(ArboEngine* e, uint64_t* human_vectors, uint64_t* mosquito_
 // tuning calls run unchanged; the CPU engine's results never depend on them.
 static const uint32_t kShapes[4][2] = {{4, 4}, {4, 2}, {8, 4}, {2, 2}};
 static const uint32_t kOcc8Shapes[2][2] = {{4, 2}, {2, 2}};   // also offered with 8 waves per SIMD
+static const uint32_t kWideShapes[2][3] = {{4, 2, 2}, {2, 2, 4}};   // several waves per community
 
 int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n) {
     if (!n) return ARBO_E_INVALID;
     uint32_t k = 0;
     for (const auto& d : kShapes)
         for (uint32_t x = 0; x <= 3; x += 3) {
-            if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8, 0};
+            if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8, 0, 1};
             ++k;
         }
     for (const auto& d : kOcc8Shapes) {
-        if (out && k < cap) out[k] = ArboTuning{d[0], d[1], 0, 8, 8};
+        if (out && k < cap) out[k] = ArboTuning{d[0], d[1], 0, 8, 8, 1};
+        ++k;
+    }
+    for (const auto& d : kWideShapes) {
+        if (out && k < cap) out[k] = ArboTuning{d[0], d[1], 0, 8, 0, d[2]};
         ++k;
     }
     *n = k;
@@ -1114,7 +1119,13 @@ int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n) {
 int arbo_set_tuning(ArboEngine* e, const ArboTuning* t) {
     if (!e || !t) return ARBO_E_INVALID;
     bool shape = false;
-    if (t->waves_per_simd == 8) {
+    if (t->waves_per_comm > 1) {
+        for (const auto& d : kWideShapes)
+            shape = shape || (d[0] == t->census_unroll && d[1] == t->early_mosq && d[2] == t->waves_per_comm &&
+                              t->waves_per_simd == 0);
+    } else if (t->waves_per_comm != 1) {
+        shape = false;
+    } else if (t->waves_per_simd == 8) {
         for (const auto& d : kOcc8Shapes) shape = shape || (d[0] == t->census_unroll && d[1] == t->early_mosq);
     } else if (t->waves_per_simd == 0) {
         for (const auto& d : kShapes) shape = shape || (d[0] == t->census_unroll && d[1] == t->early_mosq);

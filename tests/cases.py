@@ -8,8 +8,9 @@ communities (1 human; the maximum of 8,192 humans and slots; a community with
 no mosquito slots), large Poisson means (the PTRS branch), Beta(2,5) immunity,
 both event-count laws with and without a staying fraction, two
 simulations of the 256-simulation batch, climate-driven days (per-day
-transmission, EIP and mosquito lifetimes, the logistic emergence target), and
-degree-heterogeneous human and mosquito community sizes.
+transmission, EIP and mosquito lifetimes, the logistic emergence target),
+degree-heterogeneous human and mosquito community sizes, and bursts of
+hundreds of human deaths and onsets per community and day.
 """
 from __future__ import annotations
 
@@ -94,6 +95,20 @@ def case_hetero():
     return [SimSpec(6, 5, humans, p, 19, mosq_share=mosquito_shares(6, 5, 2.5, neighbourhood=4))], 90
 
 
+def case_burst():
+    """Days with hundreds of human deaths and E->I onsets in one community:
+    8,192 humans with exponential lifetimes of mean 20 days (about 400 deaths
+    a day) and a synchronised epidemic (half the mosquitoes infectious at
+    t = 0, a fixed 5-day incubation, so thousands of onsets on day 5), which
+    fills the demography kernel's queue of deferred human changes several
+    times in a day, with one wave per community and with several."""
+    humans = np.array([8192, 8192, 5000, 3001], np.uint32)
+    slots = np.array([8192, 4096, 8192, 2000], np.uint32)
+    p = configs.base_params(n_serotypes=2, h_scale=20.0, h_shape=1.0, seed_frac_h=0.0, seed_frac_m=0.5,
+                            beta_hm=0.9, beta_mh=0.9, iip_min=5, iip_max=5, m_ratio=0.5, season_amp=0.0)
+    return [SimSpec(2, 2, humans, p, 31, mosq_slots=slots)], 40
+
+
 def case_batch_pair():
     sims = [configs.batch_sim(i) for i in range(24)]
     sims.sort(key=lambda s: int(s.humans.sum()))
@@ -112,6 +127,7 @@ CASES = {
     "batch_pair": case_batch_pair,
     "climate": case_climate,
     "hetero": case_hetero,
+    "burst": case_burst,
 }
 
 # days at which the full census is compared (plus the last day)

@@ -1,7 +1,8 @@
 """Launch tunings of the day kernels change no result (GPU).
 
 Every candidate of arbo_tuning_candidates (census loads in flight, early
-mosquito loads, XCD-contiguous community order) and the extremes of the
+mosquito loads, XCD-contiguous community order, 8 waves per SIMD, several
+waves per community) and the extremes of the
 long-distance split give the oracle's census and series bit for bit, and an
 autotuned run, graph replay included, equals an untuned one."""
 import numpy as np
@@ -24,7 +25,7 @@ def _reference(oracle_lib, specs, T):
     return out
 
 
-@pytest.mark.parametrize("case", ["ragged", "batch_pair", "climate"])
+@pytest.mark.parametrize("case", ["ragged", "batch_pair", "climate", "burst"])
 def test_every_tuning_is_bit_identical(case, hip_lib, oracle_lib):
     specs, T = CASES[case]()
     ref = _reference(oracle_lib, specs, T)

@@ -289,18 +289,19 @@ def test_tuning_calls_on_the_oracle(oracle_lib):
     a = Engine(oracle_lib, threads=2)
     a.init(specs, 40)
     cands = a.tuning_candidates()
-    assert len(cands) == 10 and cands[0] == {"census_unroll": 4, "early_mosq": 4, "xcd_order": 0, "ld_split": 8,
-                                             "waves_per_simd": 0}
+    assert len(cands) == 12 and cands[0] == {"census_unroll": 4, "early_mosq": 4, "xcd_order": 0, "ld_split": 8,
+                                             "waves_per_simd": 0, "waves_per_comm": 1}
     assert cands[8]["waves_per_simd"] == 8
+    assert [c["waves_per_comm"] for c in cands[10:]] == [2, 4]
     a.set_tuning(xcd_order=1, ld_split=3)
     assert a.tuning()["ld_split"] == 3
     with pytest.raises(RuntimeError):
         a.set_tuning(census_unroll=3)
     a.step(5)
     chosen, ms = a.autotune(days_each=2)
-    assert a.day == 5 + 20 and ms > 0
+    assert a.day == 5 + 24 and ms > 0
     with pytest.raises(RuntimeError):
-        a.autotune(days_each=3)   # 30 more days exceed the 40-day capacity
+        a.autotune(days_each=3)   # 36 more days exceed the 40-day capacity
     a.step(40 - a.day)
     b = Engine(oracle_lib, threads=2)
     b.init(specs, 40)
