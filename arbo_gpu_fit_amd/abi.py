@@ -10,8 +10,8 @@ import os
 
 ABI_VERSION = 7
 MAX_SEROTYPES = 4
-MAX_COMMUNITY_HUMANS = 8192
-MAX_COMMUNITY_SLOTS = 8192
+MAX_COMMUNITY_HUMANS = 65535
+MAX_COMMUNITY_SLOTS = 65535
 
 OK = 0
 E_INVALID = -1

@@ -152,6 +152,7 @@ struct EngineView {
     uint32_t max_days;
     uint32_t death_m_lds;          // entries of the largest mosquito death table (LDS staging)
     uint32_t reduce_split;         // workgroups per simulation of k_day_reduce
+    uint32_t max_words;            // alive-bitmap words of the batch's largest community
 };
 
 // Launch knobs of the day kernels.  None of them changes a result: every

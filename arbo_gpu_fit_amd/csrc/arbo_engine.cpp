@@ -482,6 +482,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.n_comm = e->n_comm;
     v.max_days = max_days;
     v.death_m_lds = death_m_lds;
+    v.max_words = (max_cap + 31) >> 5;   // above 256 the day kernels take their eight-wave forms
     {   // about one community per thread of k_day_reduce, at most kMaxReduceSplit workgroups per
         // simulation and about 2,048 in all (8 per CU)
         uint32_t max_c = 1;

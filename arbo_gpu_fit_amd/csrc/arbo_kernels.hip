@@ -43,8 +43,10 @@
 
 namespace arbo {
 
-constexpr uint32_t kMaxWords = ARBO_MAX_COMMUNITY_SLOTS / 32;  // alive-bitmap words per community
-constexpr uint32_t kWordsPerLane = kMaxWords / kWave;          // 4
+constexpr uint32_t kWordsPerLane = 4;                          // alive-bitmap words per thread
+constexpr uint32_t kWaveWords = kWave * kWordsPerLane;         // bitmap words of one wave: 8,192 slots
+constexpr uint32_t kMaxWords = (ARBO_MAX_COMMUNITY_SLOTS + 31) / 32;   // bitmap words of the largest community
+constexpr uint32_t kMaxWaves = (kMaxWords + kWaveWords - 1) / kWaveWords;   // waves that cover it: 8
 // The demography kernel's census loads in flight per lane (UNROLL) and, of
 // the first batch, the mosquito vectors issued before the human scan
 // (EARLY_M) are template parameters chosen per engine (Tuning).
@@ -52,12 +54,12 @@ constexpr uint32_t kChunk = 256;                               // events per cla
 constexpr uint32_t kPerLane = kChunk / kWave;                  // 4
 constexpr uint32_t kHash = 512;                                // claim-table entries (2 x chunk)
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr uint32_t kKeyBits = 14;                              // member index < 2^14
+constexpr uint32_t kKeyBits = 17;                              // member index < 2^17
 constexpr uint32_t kKeyMask = (1u << kKeyBits) - 1u;
 constexpr uint32_t kHumanTable = 65536;
 constexpr uint32_t kMaxRowLen = 9;                             // Moore-8 + self
 constexpr uint32_t kGuide = 256;                               // death-age guide entries (life_bucket, padded)
-static_assert(kWordsPerLane == 4, "four bitmap words per lane");
+static_assert(kMaxWaves == 8, "eight waves cover the largest community's bitmap");
 static_assert(kLifeBuckets <= kGuide, "guide table holds every life bucket");
 static_assert(ARBO_MAX_COMMUNITY_SLOTS <= (1u << kKeyBits) && ARBO_MAX_COMMUNITY_HUMANS <= (1u << kKeyBits),
               "member index must fit the claim key");
@@ -348,9 +350,11 @@ __device__ __forceinline__ HumanDefer human_vec(const EngineView& v, const CommD
 }
 
 // Queue of the wave's deferred human changes (in LDS): entry k holds the
-// member index (bits 0-13), 1 << 14 for an onset, and the onset's E -> I day
-// (bits 16-31) in qi[k], the member's word in qw[k].
+// member index (bits 0-15) and, for an onset, its E -> I day (bits 16-31) in
+// qi[k], the member's word in qw[k].  A death is told from an onset by the
+// word: the member dies today iff its death day (bits 8-31) is at most t.
 constexpr uint32_t kHumanQueue = 256;
+static_assert(ARBO_MAX_COMMUNITY_HUMANS <= 65536u, "queue entries keep the member index in 16 bits");
 
 // Appends every lane's deferred changes of one vector at qi/qw[base...]:
 // member e of each lane goes to base + (changes of members < e in the wave)
@@ -365,7 +369,7 @@ __device__ __forceinline__ void human_enqueue(const HumanDefer& d, uint32_t q, u
         const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[e] >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bal[e], 0u));
         const uint32_t pos = base + before[e] + below;
-        qi[pos] = (4u * q + e) | (((d.onset >> e) & 1u) << 14) | (d.nxd[e] << 16);
+        qi[pos] = (4u * q + e) | (((d.onset >> e) & 1u) ? d.nxd[e] << 16 : 0u);
         qw[pos] = ws[e];
     }
 }
@@ -379,9 +383,9 @@ __device__ __forceinline__ void human_deferred(const EngineView& v, const CommDe
                                                uint32_t t, const uint32_t* qi, const uint32_t* qw, uint32_t n,
                                                uint32_t lane, HumanCount& hc) {
     for (uint32_t k = lane; k < n; k += kWave) {
-        const uint32_t e = qi[k], i = e & kKeyMask;
+        const uint32_t e = qi[k], i = e & 0xFFFFu, w = qw[k];
         const uint64_t gi = cd.h_off + i;
-        if (!(e & (1u << 14))) {   // death and instant rebirth
+        if (w < ((t + 1u) << 8)) {   // death and instant rebirth
             const U4 r = draw(key, cd.h_base + i, t, P_H_BIRTH, 0);
             const uint32_t da = death_age(v.tables + sd.death_h_off, kHumanTable - 1, (r.x >> 8) + 1);
             v.h_word[gi] = (t + da) << 8;
@@ -392,7 +396,7 @@ __device__ __forceinline__ void human_deferred(const EngineView& v, const CommDe
             const U4 r = draw(key, cd.h_base + i, t, P_H_REC, 0);
             const uint32_t nn = (e >> 16) + stoch_round(sd.recovery, r.x);
             v.h_next[gi] = (uint16_t)nn;
-            v.h_word[gi] = (qw[k] & ~3u) | 2u;
+            v.h_word[gi] = (w & ~3u) | 2u;
             const uint32_t ev = day_of(t, nn) + 1;
             hc.minev = ev < hc.minev ? ev : hc.minev;
         }
@@ -407,7 +411,7 @@ __device__ __forceinline__ void human_deferred(const EngineView& v, const CommDe
 // stride over the workgroup, each wave queues its own deferred human changes,
 // and the counts, the dead-slot prefix and the high-water mark are combined
 // across the waves through LDS; a community may hold up to NW times
-// kMaxWords bitmap words of slots (four per thread).
+// kWaveWords bitmap words of slots (four per thread).
 template <uint32_t UNROLL, uint32_t EARLY_M, bool XCD, bool HIST, uint32_t WPS, uint32_t NW>
 __global__ __launch_bounds__(kWave * NW) __attribute__((amdgpu_waves_per_eu(WPS ? WPS : 1)))
 void k_day_demography(EngineView v) {
@@ -448,9 +452,9 @@ void k_day_demography(EngineView v) {
     const uint32_t S = sd.S;
 
     extern __shared__ uint32_t s_dtab[];                            // mosquito lifetime thresholds
-    __shared__ uint32_t s_dead[kMaxWords * NW];                     // dead-slot bitmap
-    __shared__ uint32_t s_died[kMaxWords * NW];                     // died today
-    __shared__ uint32_t s_dpre[kMaxWords * NW];                     // dead slots before each word
+    __shared__ uint32_t s_dead[kWaveWords * NW];                    // dead-slot bitmap
+    __shared__ uint32_t s_died[kWaveWords * NW];                    // died today
+    __shared__ uint32_t s_dpre[kWaveWords * NW];                    // dead slots before each word
     __shared__ uint32_t s_red[NW][8];                               // per-wave counts (NW > 1)
     __shared__ uint32_t s_wtot[NW];                                 // per-wave dead slots (NW > 1)
     __shared__ uint32_t s_wtop[NW];                                 // per-wave high-water marks (NW > 1)
@@ -527,7 +531,7 @@ void k_day_demography(EngineView v) {
             __builtin_amdgcn_wave_barrier();
         }
     };
-    static_assert(kMaxWords >= kHumanQueue, "the human queue fits the bitmap arrays");
+    static_assert(kWaveWords >= kHumanQueue, "each wave's human queue fits its share of the bitmap arrays");
     auto scan_vector = [&](uint32_t q, uint4 w4) {   // every lane calls it; lanes past nvec add nothing
         HumanDefer d{0u, 0u, {0u, 0u, 0u, 0u}};
         if (q < nvec) d = human_vec(v, cd, t, q, w4, hc);
@@ -656,7 +660,7 @@ void k_day_demography(EngineView v) {
     im23 = wave_sum(im23);
     uint32_t minev = scan_h ? wave_min(hc.minev) : hs.x;
     uint32_t alive_c = wave_sum(alive);
-    if (NW > 1) {   // the waves' counts through LDS (16-bit pair halves cannot carry: each is at most 8,192 NW)
+    if (NW > 1) {   // the waves' counts through LDS (16-bit pair halves cannot carry: each is below 2^16)
         if (lane == 0) {
             s_red[wv][0] = ih01; s_red[wv][1] = ih23; s_red[wv][2] = inc01; s_red[wv][3] = inc23;
             s_red[wv][4] = im01; s_red[wv][5] = im23; s_red[wv][6] = minev; s_red[wv][7] = alive_c;
@@ -939,7 +943,10 @@ constexpr uint32_t kHumanKey = ARBO_MAX_COMMUNITY_SLOTS;   // claim keys of huma
 static_assert(ARBO_MAX_COMMUNITY_SLOTS + ARBO_MAX_COMMUNITY_HUMANS <= (1u << kKeyBits),
               "mosquito and human claim keys fit the key field");
 
-template <bool XCD, bool HIST>
+// MW: bitmap words of a community in units of kWaveWords (1, or kMaxWaves
+// for batches with communities above 8,192 slots): the alive-rank index is
+// built in MW passes of four words per lane.
+template <bool XCD, bool HIST, uint32_t MW>
 __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     const uint32_t g = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, lane = threadIdx.x;
     const uint32_t t = *v.day;
@@ -960,9 +967,9 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     __shared__ uint32_t s_rot[2];
     __shared__ uint32_t s_seg[2][4 * 10 + 1];
     __shared__ uint32_t s_qseg[2][4 * 10 + 1];
-    __shared__ uint32_t s_word[kMaxWords];
-    __shared__ uint16_t s_wpre[kMaxWords];
-    __shared__ uint16_t s_sel[kMaxWords];   // word holding alive rank 32k, k = 0..(Md-1)/32
+    __shared__ uint32_t s_word[kWaveWords * MW];
+    __shared__ uint16_t s_wpre[kWaveWords * MW];   // alive slots before each word (below 2^16)
+    __shared__ uint16_t s_sel[kWaveWords * MW];    // word holding alive rank 32k, k = 0..(Md-1)/32
     __shared__ uint32_t s_tab[kHash];
 
     if (lane < in_len) {
@@ -1010,18 +1017,20 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     const uint32_t tq_m = s_qseg[DIR_HM][nseg], tq_h = s_qseg[DIR_MH][nseg];
     const uint32_t nwords = (cd.cap + 31) >> 5;
     uint32_t Md = 0;
-    if (tq_m > 0) {
+    for (uint32_t c0 = 0; tq_m > 0 && c0 < MW && kWaveWords * c0 < nwords; ++c0) {   // wave-uniform
+        const uint32_t i4 = kWave * c0 + lane;   // the lane's 16-byte group of words
         uint32_t wv[kWordsPerLane] = {0u, 0u, 0u, 0u}, c = 0;
-        if (kWordsPerLane * lane < nwords) {   // one 16-byte load (words past the capacity are zero)
-            const uint4 a4 = reinterpret_cast<const uint4*>(v.alive + (cd.m_off >> 5))[lane];
+        if (kWordsPerLane * i4 < nwords) {   // one 16-byte load (words past the capacity are zero)
+            const uint4 a4 = reinterpret_cast<const uint4*>(v.alive + (cd.m_off >> 5))[i4];
             wv[0] = a4.x; wv[1] = a4.y; wv[2] = a4.z; wv[3] = a4.w;
         }
 #pragma unroll
         for (uint32_t k = 0; k < kWordsPerLane; ++k) c += __popc(wv[k]);
-        uint32_t pre = wave_exclusive_scan(c, Md);
+        uint32_t chunk;
+        uint32_t pre = Md + wave_exclusive_scan(c, chunk);
 #pragma unroll
         for (uint32_t k = 0; k < kWordsPerLane; ++k) {
-            const uint32_t w = kWordsPerLane * lane + k;
+            const uint32_t w = kWordsPerLane * i4 + k;
             const uint32_t cw = __popc(wv[k]);
             if (w < nwords) {
                 s_word[w] = wv[k];
@@ -1031,6 +1040,7 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
             }
             pre += cw;
         }
+        Md += chunk;
     }
     const uint32_t mq = Md > 0 ? tq_m : 0u;   // with no alive mosquito every event onto one fails
     // ---- onto humans: uniform target, success iff S and never infected by s ----
@@ -1403,8 +1413,13 @@ hipError_t launch_init(const EngineView& v, hipStream_t st) {
 // budget, and the kWideShapes with several waves per community.
 template <bool XCD, bool HIST>
 static hipError_t launch_demography_xh(const EngineView& v, const Tuning& tu, hipStream_t st) {
-    const dim3 grid(v.n_comm), block(kWave * tu.nw);
     const size_t lds = (size_t)v.death_m_lds * 4;
+    if (v.max_words > kWaveWords * tu.nw) {   // a community's bitmap needs more waves than the tuning's
+        hipLaunchKernelGGL((k_day_demography<2, 2, false, HIST, 0, kMaxWaves>), dim3(v.n_comm),
+                           dim3(kWave * kMaxWaves), lds, st, v);
+        return hipGetLastError();
+    }
+    const dim3 grid(v.n_comm), block(kWave * tu.nw);
     if (tu.nw == 2) {
         if (tu.unroll == 4 && tu.early_m == 2 && tu.wps == 0)
             hipLaunchKernelGGL((k_day_demography<4, 2, XCD, HIST, 0, 2>), grid, block, lds, st, v);
@@ -1464,12 +1479,15 @@ hipError_t launch_demography(const EngineView& v, const Tuning& tu, hipStream_t 
 
 hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st) {
     const dim3 grid(v.n_comm), block(kWave);
-    if (tu.xcd & 2u) {
-        if (v.hist) hipLaunchKernelGGL((k_day_events<true, true>), grid, block, 0, st, v);
-        else hipLaunchKernelGGL((k_day_events<true, false>), grid, block, 0, st, v);
+    if (v.max_words > kWaveWords) {   // communities above 8,192 slots
+        if (v.hist) hipLaunchKernelGGL((k_day_events<false, true, kMaxWaves>), grid, block, 0, st, v);
+        else hipLaunchKernelGGL((k_day_events<false, false, kMaxWaves>), grid, block, 0, st, v);
+    } else if (tu.xcd & 2u) {
+        if (v.hist) hipLaunchKernelGGL((k_day_events<true, true, 1>), grid, block, 0, st, v);
+        else hipLaunchKernelGGL((k_day_events<true, false, 1>), grid, block, 0, st, v);
     } else {
-        if (v.hist) hipLaunchKernelGGL((k_day_events<false, true>), grid, block, 0, st, v);
-        else hipLaunchKernelGGL((k_day_events<false, false>), grid, block, 0, st, v);
+        if (v.hist) hipLaunchKernelGGL((k_day_events<false, true, 1>), grid, block, 0, st, v);
+        else hipLaunchKernelGGL((k_day_events<false, false, 1>), grid, block, 0, st, v);
     }
     if (v.aoi_mark) hipLaunchKernelGGL(k_day_aoi, dim3(v.n_sims), dim3(256), 0, st, v);
     return hipGetLastError();

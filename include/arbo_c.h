@@ -40,8 +40,8 @@ extern "C" {
 
 #define ARBO_ABI_VERSION 7u
 #define ARBO_MAX_SEROTYPES 4u
-#define ARBO_MAX_COMMUNITY_HUMANS 8192u
-#define ARBO_MAX_COMMUNITY_SLOTS 8192u
+#define ARBO_MAX_COMMUNITY_HUMANS 65535u
+#define ARBO_MAX_COMMUNITY_SLOTS 65535u
 
 enum {
     ARBO_OK = 0,

@@ -4,8 +4,9 @@ Each case is a list of simulations and a number of days.  Together they cover
 the 10x10-lattice workload (1e5 humans, 365 days, in full), the reference's own parameter set
 (von Neumann lattice, unit weights, one serotype, bi-Weibull humans,
 Weibull(4, 23 d) mosquitoes, external introductions), isolated and ragged
-communities (1 human; the maximum of 8,192 humans and slots; a community with
-no mosquito slots), large Poisson means (the PTRS branch), Beta(2,5) immunity,
+communities (1 human; 8,192 humans and slots, one wave's bitmap; a community
+with no mosquito slots), communities up to the maximum of 65,535 humans and
+slots, large Poisson means (the PTRS branch), Beta(2,5) immunity,
 both event-count laws with and without a staying fraction, two
 simulations of the 256-simulation batch, climate-driven days (per-day
 transmission, EIP and mosquito lifetimes, the logistic emergence target),
@@ -109,6 +110,18 @@ def case_burst():
     return [SimSpec(2, 2, humans, p, 31, mosq_slots=slots)], 40
 
 
+def case_large():
+    """Communities above one wave's 8,192 bitmap slots: up to the maximum of
+    65,535 humans and 65,535 mosquito slots per community, which the day
+    kernels handle with eight waves per community (demography) and eight
+    bitmap passes (events)."""
+    humans = np.array([65535, 20000, 9000, 300], np.uint32)
+    slots = np.array([40000, 65535, 8193, 600], np.uint32)
+    p = configs.base_params(n_serotypes=2, beta_hm=0.5, beta_mh=0.5, seed_frac_h=0.005, seed_frac_m=0.005,
+                            long_distance=0.05)
+    return [SimSpec(2, 2, humans, p, 47, mosq_slots=slots)], 30
+
+
 def case_batch_pair():
     sims = [configs.batch_sim(i) for i in range(24)]
     sims.sort(key=lambda s: int(s.humans.sum()))
@@ -128,6 +141,7 @@ CASES = {
     "climate": case_climate,
     "hetero": case_hetero,
     "burst": case_burst,
+    "large": case_large,
 }
 
 # days at which the full census is compared (plus the last day)

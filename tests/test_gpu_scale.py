@@ -101,7 +101,7 @@ def test_device_error_paths(hip_lib):
         eng.init([SimSpec(2, 2, np.full(4, 10, np.uint32), bad, 1)], 5)
     assert ei.value.code == abi.E_INVALID and "n_serotypes" in str(ei.value)
     with pytest.raises(ArboError):
-        eng.init([SimSpec(2, 2, np.full(4, 10000, np.uint32), configs.base_params(), 1)], 5)
+        eng.init([SimSpec(2, 2, np.full(4, abi.MAX_COMMUNITY_HUMANS + 1, np.uint32), configs.base_params(), 1)], 5)
     eng.init([SimSpec(2, 2, np.full(4, 10, np.uint32), configs.base_params(), 1)], 5)
     eng.step(5)
     with pytest.raises(ArboError) as ei:

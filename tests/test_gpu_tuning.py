@@ -92,3 +92,32 @@ def test_every_tuning_keeps_the_outputs(hip_lib, oracle_lib):
         assert np.array_equal(gpu.aoi(0), ref[2]), cand
         _compare_state(gpu.state(0), ref[3], f"{cand}")
     gpu.close()
+
+
+@pytest.mark.parametrize("flags", [0, abi.F_SPATIAL | abi.F_AOI, abi.F_GRAPH])
+def test_large_communities_under_every_tuning(hip_lib, oracle_lib, flags):
+    """Communities above 8,192 slots (up to 65,535 humans and slots) take the
+    eight-wave demography kernel and the eight-pass event kernel whatever
+    the tuning asks for; census, series and kept outputs equal the
+    oracle's."""
+    specs, T = CASES["large"]()
+    orc = Engine(oracle_lib, flags=flags & ~abi.F_GRAPH, threads=8)
+    orc.init(specs, T)
+    orc.step(T)
+    ref = (orc.state(0), orc.series(0))
+    kept = (orc.spatial(0), orc.history(0), orc.aoi(0)) if flags & abi.F_AOI else None
+    orc.close()
+    gpu = Engine(hip_lib, flags=flags)
+    cands = gpu.tuning_candidates()
+    for cand in (cands[0], cands[-1]):   # one wave per community, and four
+        print(f"[tuning] large communities, flags={flags}, tuning={cand}", flush=True)
+        gpu.init(specs, T)
+        gpu.set_tuning(**cand)
+        gpu.step(T)
+        _compare_state(gpu.state(0), ref[0], f"large {cand}")
+        _compare_series(gpu.series(0), ref[1], f"large {cand}")
+        if kept:
+            assert np.array_equal(gpu.spatial(0), kept[0])
+            assert np.array_equal(gpu.history(0), kept[1])
+            assert np.array_equal(gpu.aoi(0), kept[2])
+    gpu.close()

@@ -365,8 +365,8 @@ def test_unsimulable_proposals_are_rejected_not_fatal(lib, oracle_lib):
     orc = Engine(oracle_lib, threads=2)
     cfg = mcmc.McmcConfig(n_chains=3, n_weeks=W, priors=configs.climate_priors(), adapt_after=50, seed=5,
                           param_set=mcmc.FIT_CLIMATE)
-    # chain 1 starts at a ratio whose capacity needs more than 8,192 slots per community
-    th0 = np.array([[2.5, 3.0, 1.0, 1.0, 3.0, 0.1, 0.2], [2.5, 3.0, 1.0, 1.0, 6.0, 0.1, 0.2],
+    # chain 1 starts at a ratio whose capacity needs more than 65,535 slots per community
+    th0 = np.array([[2.5, 3.0, 1.0, 1.0, 3.0, 0.1, 0.2], [2.5, 3.0, 1.0, 1.0, 80.0, 0.1, 0.2],
                     [2.5, 3.0, 1.0, 1.0, 2.0, 0.1, 0.2]])
     m = mcmc.Mcmc(cfg, 2, 2, humans, configs.base_params(seed_frac_h=0.02, seed_frac_m=0.02),
                   np.full(W, 20, np.uint32), theta0=th0, lib=lib, climate=cl)
@@ -375,7 +375,7 @@ def test_unsimulable_proposals_are_rejected_not_fatal(lib, oracle_lib):
     cnt = m.counters()
     assert cnt["unsimulable"] >= 1
     assert np.isfinite(ll[0, 0]) and np.isfinite(ll[0, 2])
-    assert ll[0, 1] == -np.inf or th[0, 1, 4] != 6.0   # stuck at -inf unless it moved to a simulable vector
+    assert ll[0, 1] == -np.inf or th[0, 1, 4] != 80.0   # stuck at -inf unless it moved to a simulable vector
     m.close()
     orc.close()
 

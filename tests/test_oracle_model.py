@@ -200,12 +200,12 @@ def test_invalid_shapes_and_order(oracle_lib):
     with pytest.raises(ArboError) as ei:
         eng.step(1)
     assert ei.value.code == abi.E_STATE
-    for humans in (np.array([0] * 4, np.uint32), np.array([9000] * 4, np.uint32)):
+    for humans in (np.array([0] * 4, np.uint32), np.array([abi.MAX_COMMUNITY_HUMANS + 1] * 4, np.uint32)):
         with pytest.raises(ArboError):
             eng.init([SimSpec(2, 2, humans, configs.base_params(), 1)], 10)
     with pytest.raises(ArboError):
         eng.init([SimSpec(2, 2, np.full(4, 5, np.uint32), configs.base_params(), 1,
-                          mosq_slots=np.full(4, 9000, np.uint32))], 10)
+                          mosq_slots=np.full(4, abi.MAX_COMMUNITY_SLOTS + 1, np.uint32))], 10)
     with pytest.raises(ArboError):
         eng.init([small_spec()], 0)
     eng.init([small_spec()], 5)
