@@ -21,7 +21,11 @@
 //    so gradients hold to float32-like accuracy while every multiply runs
 //    on the bfloat16 matrix cores.
 //  * The weight-gradient product appends a column of ones to its B operand,
-//    so the bias gradient (the batch sum of dZ) comes out of the same launch.
+//    so the bias gradient (the batch sum of dZ) comes out of the same launch,
+//    and the weight gradients of all layers, independent once the input
+//    gradients are known, share one launch (k_gemm_group): a training step
+//    is then 5 forward products, 4 input-gradient products and one grouped
+//    launch, where each launch costs about 5 us whatever its size.
 // Parameters stay float32 and are read by the products directly, so an
 // Adam step needs no repacking.
 #include <hip/hip_runtime.h>
@@ -180,10 +184,9 @@ __device__ inline void store_panel(const float* v, bool kcontig, uint32_t tid, P
 // MFMA operand layout of v_mfma_f32_32x32x16_bf16: lane l holds A(l % 32,
 // 8 (l / 32) + v) and B(8 (l / 32) + v, l % 32) for v = 0..7; accumulator r
 // of lane l is D(8 (r / 4) + 4 (l / 32) + r % 4, l % 32).
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-    __shared__ Panel sa, sb;
+__device__ __forceinline__ void gemm_tile(const GemmArgs& g, uint32_t bx, uint32_t by, Panel& sa, Panel& sb) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-    const uint32_t i0 = blockIdx.y * kBM, j0 = blockIdx.x * kBN;
+    const uint32_t i0 = by * kBM, j0 = bx * kBN;
     const uint32_t wi = (wave >> 1) * 32u, wj = (wave & 1u) * 32u;
     const uint32_t rowsB = g.ones ? g.N - 1 : g.N;
     const bool ka = g.sAk == 1, kb = g.sBk == 1;
@@ -244,6 +247,26 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+    __shared__ Panel sa, sb;
+    gemm_tile(g, blockIdx.x, blockIdx.y, sa, sb);
+}
+
+// Independent products in one launch: product z = blockIdx.z of the group,
+// each over the tiles it has (a workgroup past them leaves at once).  The
+// weight gradients of every layer are independent once the input gradients
+// are known, so a backward pass runs them together.
+struct GemmGroup {
+    GemmArgs g[ARBO_SUR_MAX_LAYERS];
+};
+
+__global__ __launch_bounds__(256) void k_gemm_group(GemmGroup gg) {
+    __shared__ Panel sa, sb;
+    const GemmArgs& g = gg.g[blockIdx.z];
+    if (blockIdx.x * kBN >= g.N || blockIdx.y * kBM >= g.M) return;   // workgroup-uniform
+    gemm_tile(g, blockIdx.x, blockIdx.y, sa, sb);
+}
+
 // rows idx[r] of x [*, w] to out [r, w]
 __global__ void k_gather(const float* __restrict__ x, const uint32_t* __restrict__ idx, float* __restrict__ out,
                          uint32_t rows, uint32_t w) {
@@ -256,16 +279,20 @@ __global__ void k_gather(const float* __restrict__ x, const uint32_t* __restrict
 // Poisson loss terms of the output layer: dF = (exp(f) - y) * scale; the
 // loss's partial sums per workgroup into part[blockIdx.x] and, for a running
 // epoch sum, into *acc
+// With `rows` the targets are rows rows[r] of y (width w): the minibatch's
+// rows gathered on the fly.
 __global__ __launch_bounds__(256) void k_poisson(const float* __restrict__ f, const float* __restrict__ y,
-                                                 uint64_t n, float scale, float* __restrict__ df,
-                                                 double* __restrict__ part, double* __restrict__ acc) {
+                                                 const uint32_t* __restrict__ rows, uint32_t w, uint64_t n,
+                                                 float scale, float* __restrict__ df, double* __restrict__ part,
+                                                 double* __restrict__ acc) {
     __shared__ double s[256];
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double l = 0.0;
     if (i < n) {
+        const float yi = rows ? y[(uint64_t)rows[i / w] * w + i % w] : y[i];
         const float e = expf(f[i]);
-        df[i] = (e - y[i]) * scale;
-        l = (double)e - (double)y[i] * (double)f[i];
+        df[i] = (e - yi) * scale;
+        l = (double)e - (double)yi * (double)f[i];
     }
     s[threadIdx.x] = l;
     __syncthreads();
@@ -348,7 +375,7 @@ struct ArboSurrogate {
     Buf<float> yin, dxin;
     Buf<double> part;               // the loss's per-workgroup sums (max_batch x outputs / 256)
     Buf<double> epoch_sum;          // the loss summed over one training epoch
-    Buf<float> data_x, data_y, by;
+    Buf<float> data_x, data_y;
     Buf<uint32_t> idx;
     float* scratch = nullptr;       // this device's no-allocation activation buffers
     uint64_t adam_t = 0;
@@ -395,12 +422,28 @@ uint32_t blocks(uint64_t n, uint32_t t) { return (uint32_t)((n + t - 1) / t); }
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
-// C[M, N] = sum_k A(i, k) B(j, k) with the epilogue of g.epi.
-hipError_t gemm(hipStream_t st, GemmArgs g) {
+void set_vec(GemmArgs& g) {
     g.vecA = aligned16(g.A) && ((g.sAk == 1 ? g.sAi : g.sAk) % 4 == 0);
     g.vecB = aligned16(g.B) && ((g.sBk == 1 ? g.sBj : g.sBk) % 4 == 0);
+}
+
+// C[M, N] = sum_k A(i, k) B(j, k) with the epilogue of g.epi.
+hipError_t gemm(hipStream_t st, GemmArgs g) {
+    set_vec(g);
     dim3 grid(blocks(g.N, kBN), blocks(g.M, kBM));
     hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, st, g);
+    return hipGetLastError();
+}
+
+// n independent products in one launch.
+hipError_t gemm_group(hipStream_t st, GemmGroup gg, uint32_t n) {
+    uint32_t bx = 1, by = 1;
+    for (uint32_t z = 0; z < n; ++z) {
+        set_vec(gg.g[z]);
+        bx = std::max(bx, blocks(gg.g[z].N, kBN));
+        by = std::max(by, blocks(gg.g[z].M, kBM));
+    }
+    hipLaunchKernelGGL(k_gemm_group, dim3(bx, by, n), dim3(256), 0, st, gg);
     return hipGetLastError();
 }
 
@@ -438,27 +481,14 @@ int forward(ArboSurrogate* s, uint32_t B) {
     return ARBO_OK;
 }
 
-// Backward pass from dZ[L-1]: weight and bias gradients into G when want_w
-// (one launch per layer: the bias gradient is the ones column), input
-// gradients into dxin when want_x.
+// Backward pass from dZ[L-1]: the input-gradient chain dZ[l-1] = (dZ[l] W_l)
+// * gelu'(Z[l-1]) down to layer 1 (and to the inputs, into dxin, when
+// want_x); then, when want_w, the weight and bias gradients of every layer
+// in one launch ([dW | db] = dZ^T [X | 1]: the bias gradient is the ones
+// column), since they are independent once the dZ are known.
 int backward(ArboSurrogate* s, uint32_t B, bool want_w, bool want_x) {
-    for (int l = (int)s->L - 1; l >= 0; --l) {
+    for (int l = (int)s->L - 1; l >= (want_x ? 0 : 1); --l) {
         const uint32_t K = s->w[l], N = s->w[l + 1];
-        if (want_w) {   // [dW | db] = dZ^T [X | 1]
-            GemmArgs g = args(N, K + 1, B, EPI_STORE);
-            g.A = s->dZ[l].p;
-            g.sAi = 1;
-            g.sAk = N;
-            g.B = s->H[l].p;
-            g.sBj = 1;
-            g.sBk = K;
-            g.ones = 1;
-            g.outf = s->G.p + s->woff[l];
-            g.ldf = K;
-            g.outb = s->G.p + s->boff[l];
-            SUR_TRY(s, gemm(s->stream, g));
-        }
-        if (l == 0 && !want_x) break;
         GemmArgs g = args(B, K, N, l > 0 ? EPI_GELU_GRAD : EPI_STORE);   // dX = dZ W
         g.A = s->dZ[l].p;
         g.sAi = N;
@@ -477,6 +507,25 @@ int backward(ArboSurrogate* s, uint32_t B, bool want_w, bool want_x) {
         }
         SUR_TRY(s, gemm(s->stream, g));
     }
+    if (want_w) {
+        GemmGroup gg{};
+        for (uint32_t l = 0; l < s->L; ++l) {
+            const uint32_t K = s->w[l], N = s->w[l + 1];
+            GemmArgs& g = gg.g[l];
+            g = args(N, K + 1, B, EPI_STORE);
+            g.A = s->dZ[l].p;
+            g.sAi = 1;
+            g.sAk = N;
+            g.B = s->H[l].p;
+            g.sBj = 1;
+            g.sBk = K;
+            g.ones = 1;
+            g.outf = s->G.p + s->woff[l];
+            g.ldf = K;
+            g.outb = s->G.p + s->boff[l];
+        }
+        SUR_TRY(s, gemm_group(s->stream, gg, s->L));
+    }
     return ARBO_OK;
 }
 
@@ -487,11 +536,12 @@ int upload_x(ArboSurrogate* s, uint32_t n, const float* x) {
 
 // Poisson loss and output gradient for B rows with targets in `y` (device);
 // the loss sum is returned through *loss (synchronises).
-int poisson(ArboSurrogate* s, uint32_t B, const float* y, float scale, double* loss, double* acc = nullptr) {
+int poisson(ArboSurrogate* s, uint32_t B, const float* y, float scale, double* loss, double* acc = nullptr,
+            const uint32_t* rows = nullptr) {
     const uint32_t out = s->w[s->L];
     const uint64_t n = (uint64_t)B * out;
     const uint32_t nb = blocks(n, 256);
-    hipLaunchKernelGGL(k_poisson, dim3(nb), dim3(256), 0, s->stream, s->Z[s->L - 1].p, y, n, scale,
+    hipLaunchKernelGGL(k_poisson, dim3(nb), dim3(256), 0, s->stream, s->Z[s->L - 1].p, y, rows, out, n, scale,
                        s->dZ[s->L - 1].p, s->part.p, acc);
     SUR_TRY(s, hipGetLastError());
     if (loss) {
@@ -710,7 +760,6 @@ ARBO_API int arbo_sur_train(ArboSurrogate* s, uint32_t n, const float* x, const 
     const uint32_t in = s->w[0], out = s->w[s->L];
     int rc = alloc(s, s->data_x, (size_t)n * in);
     if (!rc) rc = alloc(s, s->data_y, (size_t)n * out);
-    if (!rc) rc = alloc(s, s->by, (size_t)batch * out);
     if (!rc) rc = alloc(s, s->idx, n);
     if (rc) return rc;
     SUR_TRY(s, hipMemcpyAsync(s->data_x.p, x, (size_t)n * in * 4, hipMemcpyHostToDevice, s->stream));
@@ -737,12 +786,12 @@ ARBO_API int arbo_sur_train(ArboSurrogate* s, uint32_t n, const float* x, const 
             const uint32_t B = std::min(batch, n - b0);
             hipLaunchKernelGGL(k_gather, dim3(blocks((uint64_t)B * in, 256)), dim3(256), 0, s->stream, s->data_x.p,
                                s->idx.p + b0, s->H[0].p, B, in);
-            hipLaunchKernelGGL(k_gather, dim3(blocks((uint64_t)B * out, 256)), dim3(256), 0, s->stream, s->data_y.p,
-                               s->idx.p + b0, s->by.p, B, out);
             SUR_TRY(s, hipGetLastError());
             if ((rc = forward(s, B))) return rc;
-            // each batch's loss before its update, summed on the device
-            if ((rc = poisson(s, B, s->by.p, 1.0f / ((float)B * out), nullptr, epoch_loss ? s->epoch_sum.p : nullptr)))
+            // each batch's loss before its update, summed on the device; the
+            // targets are read through the batch's row indices
+            if ((rc = poisson(s, B, s->data_y.p, 1.0f / ((float)B * out), nullptr,
+                              epoch_loss ? s->epoch_sum.p : nullptr, s->idx.p + b0)))
                 return rc;
             if ((rc = backward(s, B, true, false))) return rc;
             if ((rc = adam_step(s, lr))) return rc;
@@ -775,7 +824,6 @@ ARBO_API void arbo_sur_destroy(ArboSurrogate* s) {
     release(s->epoch_sum);
     release(s->data_x);
     release(s->data_y);
-    release(s->by);
     release(s->idx);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;

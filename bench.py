@@ -3,6 +3,8 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--sims 256] [--no-cpu] [--no-plain]
     python bench.py --workload plain     # only the single 100x100-lattice simulation
     python bench.py --mcmc               # MCMC ESS/hour, plain vs surrogate delayed acceptance
+    python bench.py --mcmc-noise         # the fit's likelihood noise under common seeds
+    python bench.py --surrogate          # surrogate training throughput
 
 Headline workload: the 256-simulation batch (communities log-uniform
 400..10,000 squared to a lattice, 200..2,000 humans per community,
@@ -380,6 +382,77 @@ def mcmc_bench(args) -> dict:
     return out
 
 
+def mcmc_noise_bench(args) -> dict:
+    """Noise of the fit's likelihood estimate, the quantity that decides
+    whether pseudo-marginal chains can move: at the true parameters and at
+    1, 3 and 10 fixed-proposal steps along each of the five epidemic
+    parameters and along 3 random directions (3 steps), `noise_seeds`
+    simulations each under the same seeds (common random numbers), binned by
+    week and scored with arbo_nb_loglik.  Reports the sd over seeds of the
+    log-likelihood at the truth and, per step, of ll(theta + step) and of
+    ll(theta + step) - ll(theta)."""
+    import numpy as np
+
+    from arbo_gpu_fit_amd import abi, configs, mcmc
+    from arbo_gpu_fit_amd.engine import Engine, SimSpec
+
+    lib = mcmc.bind(abi.load_hip_library())
+    eng = Engine(lib, flags=abi.F_GRAPH)
+    data, _ = configs.mcmc_data(eng)
+    W = configs.MCMC_WEEKS
+    T = 7 * W - 1
+    g, humans = configs.mcmc_shape()
+    truth = configs.MCMC_TRUTH.copy()
+    D = len(truth)
+    # the fixed proposal's sd per parameter: 0.1 prior sd / sqrt(D)
+    psd = []
+    for kind, a, b in configs.mcmc_priors():
+        if kind in (mcmc.PRIOR_UNIFORM, mcmc.PRIOR_CIRCULAR):
+            psd.append((b - a) / np.sqrt(12.0))
+        elif kind == mcmc.PRIOR_BETA:
+            psd.append(np.sqrt(a * b / ((a + b) ** 2 * (a + b + 1))))
+        else:
+            psd.append(0.0)
+    sd0 = 0.1 * np.array(psd) / np.sqrt(D)
+    steps = []
+    for i in range(5):
+        for m in (1.0, 3.0, 10.0):
+            d = np.zeros(D)
+            d[i] = m * sd0[i]
+            steps.append((f"{mcmc.PARAM_NAMES[i]} x{m:g}", d))
+    rng = np.random.default_rng(1)
+    for r in range(3):
+        d = np.zeros(D)
+        d[:5] = rng.normal(size=5) * sd0[:5] * 3
+        steps.append((f"random{r} x3", d))
+    thetas = [("truth", np.zeros(D))] + steps
+    seeds = [(s << 32) | 77 for s in range(args.noise_seeds)]
+    specs = [SimSpec(g, g, humans, mcmc.sim_params(lib, truth + d, configs.base_params()), sd)
+             for _, d in thetas for sd in seeds]
+    t0 = time.perf_counter()
+    ll = np.zeros((len(thetas), len(seeds)))
+    for b0 in range(0, len(specs), 64):
+        chunk = specs[b0:b0 + 64]
+        eng.init(chunk, T)
+        eng.step(T)
+        for k in range(len(chunk)):
+            i, j = divmod(b0 + k, len(seeds))
+            wk = mcmc.weekly_incidence(lib, eng.series(k)["n_incidence"], W)
+            ll[i, j] = mcmc.nb_loglik(lib, wk, data, 1, truth[5], 1.0 / truth[6])
+        log(f"[noise] {b0 + len(chunk)}/{len(specs)} simulations, {time.perf_counter() - t0:.1f} s")
+    eng.close()
+    rows = []
+    for i, (name, _) in enumerate(thetas[1:], 1):
+        diff = ll[i] - ll[0]
+        rows.append({"step": name, "mean_diff": float(diff.mean()), "sd_diff": float(diff.std(ddof=1)),
+                     "sd_ll": float(ll[i].std(ddof=1))})
+    return {"metric": "log-likelihood noise of one simulation per proposal", "unit": "nats",
+            "value": float(ll[0].std(ddof=1)), "seeds": len(seeds), "proposal_sd": sd0.tolist(),
+            "ll_truth": ll[0].tolist(), "rows": rows,
+            "config": {"workload": f"the fit's model (1e6 humans, 32x32 communities, {T} days, {W} weeks), "
+                                   f"{len(thetas)} parameter vectors x {len(seeds)} common seeds"}}
+
+
 def surrogate_bench(args) -> dict:
     """Training throughput of the surrogate MLP 6 -> 256^4 -> 260 (GELU) on
     the MFMA kernels: `train_rows` rows of synthetic data (inputs uniform on
@@ -460,6 +533,9 @@ def main():
     ap.add_argument("--save", default="", help="save the surrogate's training set (npz)")
     ap.add_argument("--load", default="", help="load the surrogate's training set (npz)")
     ap.add_argument("--surrogate", action="store_true", help="surrogate training throughput instead")
+    ap.add_argument("--mcmc-noise", action="store_true",
+                    help="log-likelihood noise of the fit's one-simulation estimate under common seeds instead")
+    ap.add_argument("--noise-seeds", type=int, default=8, help="common seeds per parameter vector (--mcmc-noise)")
     ap.add_argument("--train-rows", type=int, default=50000)
     ap.add_argument("--sur-epochs", type=int, default=20)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
@@ -488,6 +564,9 @@ def main():
         return
     if args.surrogate:
         print(json.dumps(surrogate_bench(args)), flush=True)
+        return
+    if args.mcmc_noise:
+        print(json.dumps(mcmc_noise_bench(args)), flush=True)
         return
     if args.cpu_sims == 0:
         args.cpu_sims = 64 if args.steps <= 60 else 8
