@@ -74,7 +74,7 @@ struct ArboEngine {
     // timing
     std::vector<hipEvent_t> ev_pool, ev_used;
     double kernel_ms[2] = {0.0, 0.0};
-    Tuning tuning{4, 4, 0, 8, 0, 1};
+    Tuning tuning{4, 2, 0, 8, 0, 1};   // the autotuner's usual pick on large batches
     uint64_t kernel_launches[2] = {0, 0};
 };
 

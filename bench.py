@@ -44,7 +44,9 @@ day, against HBM bandwidth.
     serotype) for comparison.
 cpu_baseline: the CPU oracle built -O3 -march=native on this host, OpenMP
 (and serial beside it), resumed from the GPU's census at the first timed
-day and run over the timed days (at most 60) for a sample of simulations.
+day and run over the timed days (at most 60 for the batch, 600 for the
+plain simulation) for a sample of simulations (64 of the batch, 128 when
+at most 60 days are timed): about 10-15 s of CPU work.
 """
 from __future__ import annotations
 
@@ -281,7 +283,7 @@ def sim_bench(workload: str, args, lib) -> dict:
         out["roofline"]["pass"] = f"a second run of the same days ({warmup}..{days - 1}) with per-kernel HIP events"
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        cdays = min(steps, 60)
+        cdays = min(steps, 600 if plain else 60)   # about 10-15 s of CPU work either way
         log(f"[bench] {workload}: CPU oracle baseline on {threads} threads")
         what = ("the 100x100-lattice simulation" if plain else
                 f"simulations 0..{len(cpu_sims) - 1} of the batch")
@@ -514,7 +516,7 @@ def main():
     ap.add_argument("--sims", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sims", type=int, default=0,
-                    help="simulations of the batch in the CPU sample (0: 64 for windows up to 60 days, else 8)")
+                    help="simulations of the batch in the CPU sample (0: 128 for windows up to 60 days, else 64)")
     ap.add_argument("--graph", action="store_true", help="batch: replay the day loop as hipGraphs")
     ap.add_argument("--no-graph", action="store_true",
                     help="plain: launch every kernel on its own (for profilers; the same kernels and days)")
@@ -569,7 +571,7 @@ def main():
         print(json.dumps(mcmc_noise_bench(args)), flush=True)
         return
     if args.cpu_sims == 0:
-        args.cpu_sims = 64 if args.steps <= 60 else 8
+        args.cpu_sims = 128 if args.steps <= 60 else 64   # about 10-15 s of CPU work
 
     from arbo_gpu_fit_amd import abi
 

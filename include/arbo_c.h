@@ -296,7 +296,7 @@ typedef struct ArboTuning {
 
 /* The tunings arbo_autotune tries (out may be NULL to count them). */
 ARBO_API int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n);
-/* Set or read the engine's tuning (default {4, 4, 0, 8, 0, 1}); ARBO_E_INVALID for a
+/* Set or read the engine's tuning (default {4, 2, 0, 8, 0, 1}); ARBO_E_INVALID for a
  * combination the library was not built with. */
 ARBO_API int arbo_set_tuning(ArboEngine* e, const ArboTuning* t);
 ARBO_API int arbo_get_tuning(const ArboEngine* e, ArboTuning* out);

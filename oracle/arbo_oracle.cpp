@@ -110,7 +110,7 @@ struct ArboEngine {
     int threads = 0;
     bool ready = false;
     std::string err;
-    ArboTuning tuning{4, 4, 0, 8, 0, 1};   // launch knobs: accepted and reported, no effect on the CPU
+    ArboTuning tuning{4, 2, 0, 8, 0, 1};   // launch knobs: accepted and reported, no effect on the CPU
     uint32_t flags = 0;
     uint64_t scan_h = 0, scan_m = 0;    // census vectors of the days stepped (every member, every day)
     uint64_t full_h = 0, full_m = 0;    // one full scan: 4-human and 8-slot vectors of every community
