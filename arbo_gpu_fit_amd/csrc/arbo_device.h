@@ -131,6 +131,7 @@ struct EngineView {
                                    // 2^-24 units (Poisson counts) or the count (stochastic rounding)
     uint32_t* series;
     uint32_t* part;                // [G][kPartWords] per-community partials of the day (see kPartWords)
+    uint32_t* ld_done;             // [n_sims] reduce workgroups done today (fused long-distance draws)
     unsigned long long* scanned;   // [n_sims][2] census vectors read by the demography kernel (humans,
                                    // mosquitoes), summed over days by k_day_reduce
     uint32_t* spatial;             // per-community infectious humans [(T+1)][S][C] per simulation, or null
@@ -147,6 +148,8 @@ struct EngineView {
     const uint32_t* tables;        // death / initial-age / immunity pools
     const float* dens;
     const uint32_t* day;           // device day counter (graph replay)
+    uint32_t day_offset;           // the launch's day is *day + day_offset (a captured block of days
+                                   // reads one counter, advanced once per block)
     uint32_t n_sims;
     uint32_t n_comm;
     uint32_t max_days;
@@ -162,7 +165,8 @@ struct Tuning {
     uint32_t early_m;    // demography: of the first batch, mosquito loads issued before the humans
     uint32_t xcd;        // bit 0 demography, bit 1 events: the workgroups one XCD receives take a
                          // contiguous range of communities
-    uint32_t ld_split;   // long-distance kernel: workgroups per simulation
+    uint32_t ld_split;   // long-distance kernel: workgroups per simulation (0: no separate kernel, the
+                         // reduce kernel's last workgroup per simulation draws them)
     uint32_t wps;        // demography: 0 the compiler's register budget, 8 eight waves per SIMD
     uint32_t nw;         // demography: waves per community (1, 2, 4)
 };
@@ -184,7 +188,7 @@ hipError_t launch_init(const EngineView& v, hipStream_t st);
 hipError_t launch_demography(const EngineView& v, const Tuning& tu, hipStream_t st);
 hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st);
 hipError_t launch_longdist(const EngineView& v, const Tuning& tu, hipStream_t st);
-hipError_t launch_advance(uint32_t* day, hipStream_t st);
+hipError_t launch_advance(uint32_t* day, uint32_t n, hipStream_t st);
 hipError_t launch_fill(uint32_t* p, uint64_t n, uint32_t value, hipStream_t st);
 bool tuning_supported(const Tuning& tu);
 

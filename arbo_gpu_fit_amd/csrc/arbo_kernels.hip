@@ -420,7 +420,7 @@ void k_day_demography(EngineView v) {
     constexpr uint32_t NT = kWave * NW;   // threads per community
     const uint32_t g = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, tid = threadIdx.x;
     const uint32_t lane = NW == 1 ? tid : tid & (kWave - 1), wv = NW == 1 ? 0u : tid / kWave;
-    const uint32_t t = *v.day;
+    const uint32_t t = *v.day + v.day_offset;
     const CommDesc cd = v.comm[g];
     // Humans are scanned only on days when one of them may die or change
     // state: the summary holds the earliest such day (0 forces a scan) and the
@@ -949,7 +949,7 @@ static_assert(ARBO_MAX_COMMUNITY_SLOTS + ARBO_MAX_COMMUNITY_HUMANS <= (1u << kKe
 template <bool XCD, bool HIST, uint32_t MW>
 __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
     const uint32_t g = XCD ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, lane = threadIdx.x;
-    const uint32_t t = *v.day;
+    const uint32_t t = *v.day + v.day_offset;
     const CommDesc cd = v.comm[g];
     const uint32_t act = v.active[g];
     if (!act && !(cd.flags & kCommIntro)) return;   // no events arrive and no introductions
@@ -1169,10 +1169,10 @@ __global__ __launch_bounds__(kWave) void k_day_events(EngineView v) {
 // of all sources); event l lands on community mulhi(Philox(l), C).  Only the
 // number landing on each destination matters (their targets are drawn at the
 // destination), so the destinations are counted with integer atomics.
-__global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t split) {
-    // split workgroups per simulation
-    const uint32_t sim = blockIdx.x / split, part = blockIdx.x % split, tid = threadIdx.x;
-    const uint32_t t = *v.day;
+// Part `part` of `split` of simulation `sim`'s long-distance events of day t
+// (all 256 threads of the workgroup call it).
+__device__ __forceinline__ void longdist_part(const EngineView& v, uint32_t sim, uint32_t t, uint32_t part,
+                                              uint32_t split, uint32_t tid) {
     const SimDesc sd = v.sims[sim];
     const Key key{sd.k0, sd.k1};
     __shared__ uint32_t s_L[8];
@@ -1180,7 +1180,10 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t spl
         const uint32_t dir = tid >> 2, s = tid & 3u;
         uint32_t L = 0;
         if (s < sd.S) {
-            const unsigned long long fx = v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + tid];
+            // the sums are complete: added by the reduce kernel, or by the
+            // simulation's other reduce workgroups before it counted them done
+            const unsigned long long fx = __hip_atomic_load(
+                &v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (sd.count_law == ARBO_COUNT_STOCH_ROUND) {
                 L = (uint32_t)fx;   // the sources' own long-distance counts
             } else {
@@ -1208,6 +1211,12 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t spl
     if (part == 0 && tid < 8) v.lam[((size_t)((t + 1) & 1u) * v.n_sims + sim) * 8 + tid] = 0ull;
 }
 
+__global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t split) {
+    // split workgroups per simulation
+    const uint32_t sim = blockIdx.x / split, part = blockIdx.x % split;
+    longdist_part(v, sim, *v.day + v.day_offset, part, split, threadIdx.x);
+}
+
 // Sums the day's per-community partials of a simulation: `split`
 // workgroups of 256 threads (about one community per thread, at most 64
 // per simulation and about 2,048 in all) each take a share of its
@@ -1216,10 +1225,14 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t spl
 // then) and to the long-distance totals the long-distance kernel reads
 // (zeroed by the previous day's long-distance kernel), `split` integer
 // atomics per word, so the order does not matter.
+// FUSED: the simulation's last reduce workgroup to finish (a per-simulation
+// counter, v.ld_done) then draws the day's long-distance events itself, one
+// workgroup per simulation, in place of a separate k_day_longdist launch.
+template <bool FUSED>
 __global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split) {
     const uint32_t sim = blockIdx.x / split, part = blockIdx.x % split;
     const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
-    const uint32_t t = *v.day;
+    const uint32_t t = *v.day + v.day_offset;
     const SimDesc& sd = v.sims[sim];
     __shared__ unsigned long long s_fx[8];
     __shared__ uint32_t s_cnt[4][10];   // per wave: humans, mosquitoes, infectious[4], onsets[4]
@@ -1280,6 +1293,20 @@ __global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split
     if (tid >= 32 && tid < 40 && s_fx[tid - 32])
         atomicAdd(&v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + (tid - 32)], s_fx[tid - 32]);
     if (tid >= 64 && tid < 66 && s_scan[tid - 64]) atomicAdd(&v.scanned[2 * (size_t)sim + (tid - 64)], s_scan[tid - 64]);
+    if (FUSED) {
+        __shared__ uint32_t s_last;
+        __threadfence();   // this workgroup's additions to the sums are visible device-wide
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t done = atomicAdd(&v.ld_done[sim], 1u);
+            s_last = done == split - 1 ? 1u : 0u;
+            if (s_last) v.ld_done[sim] = 0u;   // every workgroup of the simulation has counted: reset for tomorrow
+        }
+        __syncthreads();
+        if (!s_last) return;   // workgroup-uniform
+        __threadfence();
+        longdist_part(v, sim, t, 0, 1, tid);
+    }
 }
 
 // Age-of-infection rings (nepidemic.cu:229-243), one workgroup per
@@ -1291,7 +1318,7 @@ __global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split
 // the writing pass stores only the day's last kAoiRecord records of each
 // ring, so no slot is written twice in one launch.
 __global__ __launch_bounds__(256) void k_day_aoi(EngineView v) {
-    const uint32_t t = *v.day;
+    const uint32_t t = *v.day + v.day_offset;
     if (t < v.aoi_from) return;
     const uint32_t sim = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
     constexpr uint32_t kWaves = 256 / kWave, kChunk = 4 * 256;
@@ -1392,8 +1419,8 @@ __global__ __launch_bounds__(256) void k_day_aoi(EngineView v) {
     if (tid < 4) v.aoi_pos[4 * sim + tid] = (s_pos[tid] + s_tot[tid]) % kAoiRecord;
 }
 
-__global__ void k_advance_day(uint32_t* day) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *day = *day + 1u;
+__global__ void k_advance_day(uint32_t* day, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *day = *day + n;
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t value) {
@@ -1470,7 +1497,7 @@ bool tuning_supported(const Tuning& tu) {
     } else if (tu.wps == 0) {
         for (const auto& d : kDemographyShapes) shape = shape || (d[0] == tu.unroll && d[1] == tu.early_m);
     }
-    return shape && tu.xcd <= 3 && tu.ld_split >= 1 && tu.ld_split <= kMaxLdSplit;
+    return shape && tu.xcd <= 3 && tu.ld_split <= kMaxLdSplit;   // ld_split 0: fused into the reduce kernel
 }
 
 hipError_t launch_demography(const EngineView& v, const Tuning& tu, hipStream_t st) {
@@ -1494,13 +1521,17 @@ hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st) 
 }
 
 hipError_t launch_longdist(const EngineView& v, const Tuning& tu, hipStream_t st) {
-    hipLaunchKernelGGL(k_day_reduce, dim3(v.n_sims * v.reduce_split), dim3(256), 0, st, v, v.reduce_split);
+    if (tu.ld_split == 0) {   // long-distance draws in the reduce kernel's last workgroup per simulation
+        hipLaunchKernelGGL(k_day_reduce<true>, dim3(v.n_sims * v.reduce_split), dim3(256), 0, st, v, v.reduce_split);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_day_reduce<false>, dim3(v.n_sims * v.reduce_split), dim3(256), 0, st, v, v.reduce_split);
     hipLaunchKernelGGL(k_day_longdist, dim3(v.n_sims * tu.ld_split), dim3(256), 0, st, v, tu.ld_split);
     return hipGetLastError();
 }
 
-hipError_t launch_advance(uint32_t* day, hipStream_t st) {
-    hipLaunchKernelGGL(k_advance_day, dim3(1), dim3(64), 0, st, day);
+hipError_t launch_advance(uint32_t* day, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance_day, dim3(1), dim3(64), 0, st, day, n);
     return hipGetLastError();
 }
 

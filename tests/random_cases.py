@@ -52,6 +52,12 @@ def random_sim(rng: np.random.Generator, seed: int, days: int) -> SimSpec:
         w = rng.uniform(0.2, 1.0, C)
         share = (w / w.sum()).astype(np.float32)
         share = (share / share.sum(dtype=np.float64)).astype(np.float32)
+    # now and then one community of 9,000-11,000 humans (without a climate, whose
+    # mosquito capacity is unbounded here): above one wave's 8,192 slots when
+    # its mosquito ratio and seasonality ask for it
+    big_u, big_c, big_n = rng.random(), int(rng.integers(C)), int(rng.integers(9000, 11001))
+    if climate is None and share is None and big_u < 0.2:
+        humans[big_c] = big_n
     return SimSpec(W, H, humans, p, seed, climate=climate, mosq_share=share)
 
 

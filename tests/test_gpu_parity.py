@@ -2,6 +2,8 @@
 
 Runs through the C ABI of libarbo_hip.so (GPU) and liboracle (CPU) on the same
 seeded inputs."""
+import os
+
 import numpy as np
 import pytest
 
@@ -27,10 +29,12 @@ def _compare_series(a: dict, b: dict, where: str):
             raise AssertionError(f"{where}: series {k} differs from row {rows[0]}")
 
 
-def _run_pair(specs, T, hip_lib, oracle_lib, flags=0):
+def _run_pair(specs, T, hip_lib, oracle_lib, flags=0, tuning=None):
     gpu = Engine(hip_lib, flags=flags)
     orc = Engine(oracle_lib, threads=8)
     gpu.init(specs, T)
+    if tuning:
+        gpu.set_tuning(**tuning)
     orc.init(specs, T)
     days = sorted(set([d for d in CHECK_DAYS if d <= T] + [T]))
     done = 0
@@ -194,11 +198,19 @@ def test_aoi_rings_match_oracle(name, hip_lib, oracle_lib):
     orc.close()
 
 
-@pytest.mark.parametrize("k", range(N_RANDOM))
+# ARBO_RANDOM_CASES=N widens the sweep to cases 0..N-1 (the default 12 run
+# with every suite; a wider sweep is a one-off run).
+@pytest.mark.parametrize("k", range(int(os.environ.get("ARBO_RANDOM_CASES", N_RANDOM))))
 def test_random_parity(k, hip_lib, oracle_lib):
-    """Seeded random combinations of every model switch (tests/random_cases.py)."""
+    """Seeded random combinations of every model switch (tests/random_cases.py),
+    case k under launch tuning k (mod the candidates): one, two or four
+    waves per community, every census-load shape, either XCD order."""
     specs, T = random_case(k)
-    print(f"[parity] random case {k}: {len(specs)} sims, {T} days", flush=True)
-    gpu, orc = _run_pair(specs, T, hip_lib, oracle_lib)
+    probe = Engine(hip_lib)
+    cands = probe.tuning_candidates()
+    probe.close()
+    tuning = cands[k % len(cands)]
+    print(f"[parity] random case {k}: {len(specs)} sims, {T} days, tuning {tuning}", flush=True)
+    gpu, orc = _run_pair(specs, T, hip_lib, oracle_lib, tuning=tuning)
     gpu.close()
     orc.close()
