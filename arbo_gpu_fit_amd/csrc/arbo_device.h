@@ -131,7 +131,6 @@ struct EngineView {
                                    // 2^-24 units (Poisson counts) or the count (stochastic rounding)
     uint32_t* series;
     uint32_t* part;                // [G][kPartWords] per-community partials of the day (see kPartWords)
-    uint32_t* ld_done;             // [n_sims] reduce workgroups done today (fused long-distance draws)
     unsigned long long* scanned;   // [n_sims][2] census vectors read by the demography kernel (humans,
                                    // mosquitoes), summed over days by k_day_reduce
     uint32_t* spatial;             // per-community infectious humans [(T+1)][S][C] per simulation, or null
@@ -165,8 +164,7 @@ struct Tuning {
     uint32_t early_m;    // demography: of the first batch, mosquito loads issued before the humans
     uint32_t xcd;        // bit 0 demography, bit 1 events: the workgroups one XCD receives take a
                          // contiguous range of communities
-    uint32_t ld_split;   // long-distance kernel: workgroups per simulation (0: no separate kernel, the
-                         // reduce kernel's last workgroup per simulation draws them)
+    uint32_t ld_split;   // long-distance kernel: workgroups per simulation
     uint32_t wps;        // demography: 0 the compiler's register budget, 8 eight waves per SIMD
     uint32_t nw;         // demography: waves per community (1, 2, 4)
 };

@@ -65,7 +65,6 @@ struct ArboEngine {
     DevBuf<uint32_t> aoi, aoi_pos;
     DevBuf<unsigned long long> lam, scanned;
     DevBuf<uint32_t> part;   // per-community partials of the day
-    DevBuf<uint32_t> ld_done;   // [n_sims] reduce workgroups done today (fused long-distance draws)
     DevBuf<float> dens;
     uint64_t series_total = 0;
     EngineView view{};
@@ -130,7 +129,7 @@ void free_batch(ArboEngine* e) {
     dev_free(e->h_word); dev_free(e->alive); dev_free(e->cnt); dev_free(e->nld); dev_free(e->active); dev_free(e->hsum); dev_free(e->series); dev_free(e->spatial); dev_free(e->csr_thr); dev_free(e->csr_inbox);
     dev_free(e->in_edge); dev_free(e->in_slot); dev_free(e->tables); dev_free(e->day_ctr);
     dev_free(e->h_next); dev_free(e->m_word); dev_free(e->m_next); dev_free(e->h_birth); dev_free(e->hist); dev_free(e->aoi_mark); dev_free(e->aoi); dev_free(e->aoi_pos);
-    dev_free(e->lam); dev_free(e->scanned); dev_free(e->dens); dev_free(e->part); dev_free(e->ld_done);
+    dev_free(e->lam); dev_free(e->scanned); dev_free(e->dens); dev_free(e->part);
     e->ready = false;
 }
 
@@ -426,7 +425,7 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
         (rc = dev_alloc(e, e->aoi_pos, (e->flags & ARBO_F_AOI) ? (size_t)n_sims * 4 : 0)) || (rc = dev_alloc(e, e->m_word, m_tot)) ||
         (rc = dev_alloc(e, e->m_next, m_tot)) || (rc = dev_alloc(e, e->alive, m_tot / 32)) || (rc = dev_alloc(e, e->cnt, comm.size() * kInboxEdges * 8)) || (rc = dev_alloc(e, e->nld, comm.size() * 8)) ||
         (rc = dev_alloc(e, e->hsum, comm.size() * 4)) || (rc = dev_alloc(e, e->active, comm.size())) ||
-        (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->scanned, (size_t)2 * n_sims)) || (rc = dev_alloc(e, e->ld_done, n_sims)) || (rc = dev_alloc(e, e->part, comm.size() * kPartWords)) || (rc = dev_alloc(e, e->series, ser_tot)) || (rc = dev_alloc(e, e->spatial, spa_tot)) ||
+        (rc = dev_alloc(e, e->lam, (size_t)2 * n_sims * 8)) || (rc = dev_alloc(e, e->scanned, (size_t)2 * n_sims)) || (rc = dev_alloc(e, e->part, comm.size() * kPartWords)) || (rc = dev_alloc(e, e->series, ser_tot)) || (rc = dev_alloc(e, e->spatial, spa_tot)) ||
         (rc = dev_alloc(e, e->csr_thr, csr_thr.size())) || (rc = dev_alloc(e, e->csr_inbox, csr_inbox.size())) || (rc = dev_alloc(e, e->in_edge, in_edge.size())) ||
         (rc = dev_alloc(e, e->in_slot, in_slot.size())) || (rc = dev_alloc(e, e->tables, table_pool.size())) ||
         (rc = dev_alloc(e, e->dens, dens_pool.size())) || (rc = dev_alloc(e, e->day_ctr, 1))) {
@@ -446,7 +445,6 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     HIP_TRY(e, launch_fill(reinterpret_cast<uint32_t*>(e->lam.p), (uint64_t)2 * n_sims * 8 * 2, 0u, e->stream));
     HIP_TRY(e, launch_fill(reinterpret_cast<uint32_t*>(e->scanned.p), (uint64_t)2 * n_sims * 2, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->day_ctr.p, 1, 0u, e->stream));
-    HIP_TRY(e, launch_fill(e->ld_done.p, n_sims, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->nld.p, comm.size() * 8, 0u, e->stream));
     HIP_TRY(e, launch_fill(e->hsum.p, comm.size() * 4, 0u, e->stream));   // every community scans its humans on day 0
     HIP_TRY(e, launch_fill(e->active.p, comm.size(), 0u, e->stream));
@@ -478,7 +476,6 @@ int arbo_batch_init(ArboEngine* e, uint32_t n_sims, const ArboShape* shapes, con
     v.lam = e->lam.p;
     v.part = e->part.p;
     v.scanned = e->scanned.p;
-    v.ld_done = e->ld_done.p;
     v.series = e->series.p;
     v.spatial = (e->flags & ARBO_F_SPATIAL) ? e->spatial.p : nullptr;
     v.hist = (e->flags & (ARBO_F_HISTORY | ARBO_F_AOI)) ? e->hist.p : nullptr;
@@ -568,12 +565,8 @@ int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n) {
             if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8, 0, 1};
             ++k;
         }
-    {   // eight waves per SIMD, for batches of few communities (the (2, 2) shape is settable too)
-        if (out && k < cap) out[k] = ArboTuning{kOccShapes[0][0], kOccShapes[0][1], 0, 8, 8, 1};
-        ++k;
-    }
-    {   // the long-distance draws in the reduce kernel: one launch less a day
-        if (out && k < cap) out[k] = ArboTuning{4, 2, 0, 0, 0, 1};
+    for (const auto& d : kOccShapes) {   // eight waves per SIMD: for batches of few communities
+        if (out && k < cap) out[k] = ArboTuning{d[0], d[1], 0, 8, 8, 1};
         ++k;
     }
     for (const auto& d : kWideShapes) {   // several waves per community: for batches of few communities
@@ -653,9 +646,8 @@ int arbo_autotune(ArboEngine* e, uint32_t days_each, ArboTuning* chosen, double*
         (void)rc2;
         return rc;
     }
-    Tuning keep = from_abi(cand[ka]);   // the demography knobs of the fastest demography,
-    keep.xcd = (keep.xcd & 1u) | (cand[kb].xcd_order & 2u);   // the rest-of-day knobs of the fastest rest of day
-    keep.ld_split = cand[kb].ld_split;
+    Tuning keep = from_abi(cand[ka]);
+    keep.xcd = (keep.xcd & 1u) | (cand[kb].xcd_order & 2u);
     if ((rc = apply_tuning(e, keep))) return rc;
     if (chosen) *chosen = to_abi(keep);
     if (ms_per_day) *ms_per_day = (ta[ka] + tb[kb]) / days_each;

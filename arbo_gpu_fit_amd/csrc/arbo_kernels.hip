@@ -1180,10 +1180,7 @@ __device__ __forceinline__ void longdist_part(const EngineView& v, uint32_t sim,
         const uint32_t dir = tid >> 2, s = tid & 3u;
         uint32_t L = 0;
         if (s < sd.S) {
-            // the sums are complete: added by the reduce kernel, or by the
-            // simulation's other reduce workgroups before it counted them done
-            const unsigned long long fx = __hip_atomic_load(
-                &v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long fx = v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + tid];   // summed by k_day_reduce
             if (sd.count_law == ARBO_COUNT_STOCH_ROUND) {
                 L = (uint32_t)fx;   // the sources' own long-distance counts
             } else {
@@ -1225,10 +1222,6 @@ __global__ __launch_bounds__(256) void k_day_longdist(EngineView v, uint32_t spl
 // then) and to the long-distance totals the long-distance kernel reads
 // (zeroed by the previous day's long-distance kernel), `split` integer
 // atomics per word, so the order does not matter.
-// FUSED: the simulation's last reduce workgroup to finish (a per-simulation
-// counter, v.ld_done) then draws the day's long-distance events itself, one
-// workgroup per simulation, in place of a separate k_day_longdist launch.
-template <bool FUSED>
 __global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split) {
     const uint32_t sim = blockIdx.x / split, part = blockIdx.x % split;
     const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
@@ -1293,20 +1286,6 @@ __global__ __launch_bounds__(256) void k_day_reduce(EngineView v, uint32_t split
     if (tid >= 32 && tid < 40 && s_fx[tid - 32])
         atomicAdd(&v.lam[((size_t)(t & 1u) * v.n_sims + sim) * 8 + (tid - 32)], s_fx[tid - 32]);
     if (tid >= 64 && tid < 66 && s_scan[tid - 64]) atomicAdd(&v.scanned[2 * (size_t)sim + (tid - 64)], s_scan[tid - 64]);
-    if (FUSED) {
-        __shared__ uint32_t s_last;
-        __threadfence();   // this workgroup's additions to the sums are visible device-wide
-        __syncthreads();
-        if (tid == 0) {
-            const uint32_t done = atomicAdd(&v.ld_done[sim], 1u);
-            s_last = done == split - 1 ? 1u : 0u;
-            if (s_last) v.ld_done[sim] = 0u;   // every workgroup of the simulation has counted: reset for tomorrow
-        }
-        __syncthreads();
-        if (!s_last) return;   // workgroup-uniform
-        __threadfence();
-        longdist_part(v, sim, t, 0, 1, tid);
-    }
 }
 
 // Age-of-infection rings (nepidemic.cu:229-243), one workgroup per
@@ -1497,7 +1476,7 @@ bool tuning_supported(const Tuning& tu) {
     } else if (tu.wps == 0) {
         for (const auto& d : kDemographyShapes) shape = shape || (d[0] == tu.unroll && d[1] == tu.early_m);
     }
-    return shape && tu.xcd <= 3 && tu.ld_split <= kMaxLdSplit;   // ld_split 0: fused into the reduce kernel
+    return shape && tu.xcd <= 3 && tu.ld_split >= 1 && tu.ld_split <= kMaxLdSplit;
 }
 
 hipError_t launch_demography(const EngineView& v, const Tuning& tu, hipStream_t st) {
@@ -1521,11 +1500,7 @@ hipError_t launch_events(const EngineView& v, const Tuning& tu, hipStream_t st) 
 }
 
 hipError_t launch_longdist(const EngineView& v, const Tuning& tu, hipStream_t st) {
-    if (tu.ld_split == 0) {   // long-distance draws in the reduce kernel's last workgroup per simulation
-        hipLaunchKernelGGL(k_day_reduce<true>, dim3(v.n_sims * v.reduce_split), dim3(256), 0, st, v, v.reduce_split);
-        return hipGetLastError();
-    }
-    hipLaunchKernelGGL(k_day_reduce<false>, dim3(v.n_sims * v.reduce_split), dim3(256), 0, st, v, v.reduce_split);
+    hipLaunchKernelGGL(k_day_reduce, dim3(v.n_sims * v.reduce_split), dim3(256), 0, st, v, v.reduce_split);
     hipLaunchKernelGGL(k_day_longdist, dim3(v.n_sims * tu.ld_split), dim3(256), 0, st, v, tu.ld_split);
     return hipGetLastError();
 }

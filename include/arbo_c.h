@@ -286,8 +286,7 @@ typedef struct ArboTuning {
     uint32_t early_mosq;      /* of the first batch, mosquito loads issued before the human scan */
     uint32_t xcd_order;       /* bit 0 demography, bit 1 event kernel: the workgroups one XCD receives
                                  take a contiguous range of communities */
-    uint32_t ld_split;        /* long-distance kernel: workgroups per simulation, 1..64; 0: no separate
-                                 launch, the reduce kernel's last workgroup per simulation draws them */
+    uint32_t ld_split;        /* long-distance kernel: workgroups per simulation, 1..64 */
     uint32_t waves_per_simd;  /* demography kernel: 0 the compiler's register budget, 8 one that
                                  keeps 8 waves per SIMD (with census_unroll/early_mosq 4/2 or
                                  2/2): more resident waves for batches of few communities */

@@ -1104,12 +1104,8 @@ int arbo_tuning_candidates(ArboTuning* out, uint32_t cap, uint32_t* n) {
             if (out && k < cap) out[k] = ArboTuning{d[0], d[1], x, 8, 0, 1};
             ++k;
         }
-    {
-        if (out && k < cap) out[k] = ArboTuning{kOcc8Shapes[0][0], kOcc8Shapes[0][1], 0, 8, 8, 1};
-        ++k;
-    }
-    {
-        if (out && k < cap) out[k] = ArboTuning{4, 2, 0, 0, 0, 1};
+    for (const auto& d : kOcc8Shapes) {
+        if (out && k < cap) out[k] = ArboTuning{d[0], d[1], 0, 8, 8, 1};
         ++k;
     }
     for (const auto& d : kWideShapes) {
@@ -1134,7 +1130,7 @@ int arbo_set_tuning(ArboEngine* e, const ArboTuning* t) {
     } else if (t->waves_per_simd == 0) {
         for (const auto& d : kShapes) shape = shape || (d[0] == t->census_unroll && d[1] == t->early_mosq);
     }
-    if (!shape || t->xcd_order > 3 || t->ld_split > 64) return fail(e, ARBO_E_INVALID, "unsupported tuning");
+    if (!shape || t->xcd_order > 3 || t->ld_split < 1 || t->ld_split > 64) return fail(e, ARBO_E_INVALID, "unsupported tuning");
     e->tuning = *t;
     return ARBO_OK;
 }
