@@ -816,15 +816,16 @@ void k_day_demography(EngineView v) {
     // word * row length gives the column (high half) and the alias test
     // uniform (low half).
     const uint32_t rl = cd.row_len;
-    uint32_t qpre[9];
-    qpre[0] = 0;
+    uint32_t qtot = 0;   // quads of the 8 segments (the segment of a quad is found by walking s_n)
 #pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) qpre[k + 1] = qpre[k] + ((s_n[k] + 3) >> 2);
-    for (uint32_t gq = tid; gq < qpre[8]; gq += NT) {
-        uint32_t k = 0;
-#pragma unroll
-        for (uint32_t q = 1; q < 8; ++q) k += (gq >= qpre[q]) ? 1u : 0u;
-        const uint32_t j = gq - qpre[k];
+    for (uint32_t k = 0; k < 8; ++k) qtot += (s_n[k] + 3) >> 2;
+    for (uint32_t gq = tid; gq < qtot; gq += NT) {
+        uint32_t k = 0, base = 0, nq = (s_n[0] + 3) >> 2;
+        while (gq >= base + nq) {   // ends: gq < qtot
+            base += nq;
+            nq = (s_n[++k] + 3) >> 2;
+        }
+        const uint32_t j = gq - base;
         const uint32_t n = s_n[k];
         const U4 r = draw(key, cd.c, t, ctr_tag(P_DEST, k >> 2, k & 3u, 0), j);
         const uint32_t w4[4] = {r.x, r.y, r.z, r.w};
