@@ -5,7 +5,7 @@
 // model switch the parity cases use (both count laws, a staying fraction,
 // von Neumann and Moore lattices with and without self entries, 1-4
 // serotypes, Beta(2,5) immunity, introductions, a climate, mosquito shares,
-// ragged communities up to the 8,192 maximum and one-human ones), the kept
+// ragged communities up to the 65,535 maximum and one-human ones), the kept
 // outputs (history, age-of-infection rings, per-community series), reads of
 // every output, a checkpoint written back at a later day, and a batch of
 // several simulations for the OpenMP paths.  Any sanitizer report ends the
@@ -172,13 +172,13 @@ int main(int argc, char** argv) {
         std::vector<Sim> v{make_sim(4, 4, 200, reference_params(), 12)};
         run("reference parameters", v, 80, ARBO_F_HISTORY, threads);
     }
-    {   // ragged communities: 1 human, the maximum 8,192, no mosquito slots
+    {   // ragged communities: 1 human, the maxima of 65,535 humans and slots, no mosquito slots
         ArboParams p = base_params();
         p.n_serotypes = 3;
         p.long_distance = 0.2f;
         Sim s = make_sim(5, 3, 1, p, 13);
-        const uint32_t hum[15] = {1, 7, 8192, 13, 64, 65, 255, 256, 257, 1000, 2, 3, 4, 5, 999};
-        const uint32_t sl[15] = {0, 16, 8192, 30, 100, 200, 511, 512, 513, 3000, 1, 8, 9, 7, 2500};
+        const uint32_t hum[15] = {1, 7, 8192, 13, 64, 65, 255, 256, 257, 1000, 2, 3, 65535, 5, 999};
+        const uint32_t sl[15] = {0, 16, 8192, 30, 100, 200, 511, 512, 513, 3000, 1, 8, 9, 65535, 2500};
         s.humans.assign(hum, hum + 15);
         s.slots.assign(sl, sl + 15);
         std::vector<Sim> v{s};
