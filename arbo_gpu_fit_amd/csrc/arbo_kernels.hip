@@ -90,6 +90,14 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t x) {
     return wave_reduce(x, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
 }
 
+__device__ __forceinline__ uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+
+// Two 16-bit differences a - b (mod 2^16) in one word (v_pk_sub_u16).
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t sub_u16x2(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
+}
+
 // Exclusive prefix sum over the wavefront; `total` gets the wave sum.
 __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, uint32_t& total) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -270,61 +278,39 @@ __global__ __launch_bounds__(kWave) void k_init(EngineView v) {
 // ---------------------------------------------------------------------------
 // Day kernel A: demography, counters, event counts, series row t+1.
 
-// Per-lane human tallies; minev is the earliest day, after today's updates,
-// on which one of the lane's humans dies or changes state.
+// Per-lane tallies of the exposed and infectious humans: infectious (ih)
+// and E -> I onsets today (inc) as pairs of 16-bit serotype counts, and the
+// earliest next state-change day of those members (minev).
 struct HumanCount { uint32_t ih01, ih23, inc01, inc23, minev; };
 
 __device__ __forceinline__ uint32_t day_of(uint32_t t, uint32_t nx) {   // absolute day of a 16-bit day near t
     return (uint32_t)((int32_t)t + (int32_t)(int16_t)(uint16_t)(nx - t));
 }
 
-// Four humans of ndemographic.cu:57-103 (one 16-byte vector of packed words,
-// index 4q..4q+3).  The next-event days of the vector are read with one
-// 8-byte load, only when one of its members is infected and survives.  The
-// changes that need a random draw (a death with its rebirth's lifetime, an
-// E -> I onset with its infectious period) are not made here: they come back
-// as `defer` (bit e: member e) with `onset` (bit e: an onset, else a death)
-// and the onset's E -> I day in nxd[e], for the whole wave to process
-// together (human_deferred).  Everything else (counts, recoveries) is done
-// here; an onset is counted here too.
-struct HumanDefer { uint32_t defer, onset, nxd[4]; };
-
-__device__ __forceinline__ HumanDefer human_vec(const EngineView& v, const CommDesc& cd, uint32_t t, uint32_t q,
-                                                uint4 w4, HumanCount& hc) {
-    HumanDefer d{0u, 0u, {0u, 0u, 0u, 0u}};
-    const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
-    // death day <= t  <=>  word < (t + 1) 2^8 (t < 2^23, so no overflow)
-    const uint32_t tt = (t + 1u) << 8;
-    const uint32_t left = cd.n - 4 * q;                  // >= 1
-    const uint32_t valid = left >= 4 ? 0xFu : (1u << left) - 1u;
-    uint32_t dying = 0, need = 0;
+// The members of one vector of humans (ndemographic.cu:57-103; four packed
+// words, index 4q..4q+3) that are exposed or infectious and survive today
+// (word >= tt).  An I -> R recovery is made here.  An E -> I onset is counted
+// here and returned (bit e: member e): its infectious period is drawn later,
+// with the wave's other deferred changes (human_deferred); its E -> I day
+// goes to onx[e].  The counts and every other member's next state-change day
+// go to the lane's tallies.  The next-event days of the vector are read with
+// one 8-byte load.
+__device__ __forceinline__ uint32_t human_states(const EngineView& v, const CommDesc& cd, uint32_t t, uint32_t tt,
+                                                 uint32_t q, const uint32_t (&ws)[4], HumanCount& hc,
+                                                 uint16_t* onx) {
+    // The tallies are updated in place only on this (rare) path.  The empty
+    // asm hides their values from the compiler, which would otherwise carry
+    // each increment out of the branch and add it after the join: a register
+    // cleared for every count on every vector scanned, infected or not.
+    asm volatile("" : "+v"(hc.ih01), "+v"(hc.ih23), "+v"(hc.inc01), "+v"(hc.inc23), "+v"(hc.minev));
+    const uint2 nx2 = *reinterpret_cast<const uint2*>(v.h_next + cd.h_off + 4 * q);
+    const uint32_t nxs[4] = {nx2.x & 0xFFFFu, nx2.x >> 16, nx2.y & 0xFFFFu, nx2.y >> 16};
+    uint32_t ons = 0;
 #pragma unroll
     for (uint32_t e = 0; e < 4; ++e) {
-        dying |= (ws[e] < tt ? 1u : 0u) << e;
-        need |= ((ws[e] & 3u) != 0u ? 1u : 0u) << e;
-    }
-    dying &= valid;
-    need &= valid & ~dying;
-    {   // earliest death day of the members that survive today (words order by death day)
-        uint32_t mw = 0xFFFFFFFFu;
-#pragma unroll
-        for (uint32_t e = 0; e < 4; ++e)
-            if (((valid & ~dying) >> e) & 1u) mw = ws[e] < mw ? ws[e] : mw;
-        const uint32_t md = mw >> 8;
-        hc.minev = md < hc.minev ? md : hc.minev;
-    }
-    d.defer = dying;   // death and instant rebirth
-    if (!need) return d;
-    uint32_t nxs[4];
-    {
-        const uint2 nx2 = *reinterpret_cast<const uint2*>(v.h_next + cd.h_off + 4 * q);
-        nxs[0] = nx2.x & 0xFFFFu; nxs[1] = nx2.x >> 16; nxs[2] = nx2.y & 0xFFFFu; nxs[3] = nx2.y >> 16;
-    }
-#pragma unroll
-    for (uint32_t e = 0; e < 4; ++e) {
-        if (!((need >> e) & 1u)) continue;
         const uint32_t w = ws[e];
         const uint32_t st = w & 3u, strain = (w >> 2) & 3u;
+        if (st == 0u || w < tt) continue;
         const uint32_t nx = nxs[e];
         const bool due = (int16_t)(uint16_t)(t - nx) > 0;
         uint32_t ev = 0xFFFFFFFFu;                // next state change: the day after `next`
@@ -338,15 +324,14 @@ __device__ __forceinline__ HumanDefer human_vec(const EngineView& v, const CommD
         } else if (due) {                        // E -> I: counted now, its infectious period drawn later
             add_pair(hc.inc01, hc.inc23, strain);
             add_pair(hc.ih01, hc.ih23, strain);
-            d.defer |= 1u << e;
-            d.onset |= 1u << e;
-            d.nxd[e] = nx;
+            ons |= 1u << e;
+            onx[e] = (uint16_t)nx;
         } else {
             ev = day_of(t, nx) + 1;
         }
         hc.minev = ev < hc.minev ? ev : hc.minev;
     }
-    return d;
+    return ons;
 }
 
 // Queue of the wave's deferred human changes (in LDS): entry k holds the
@@ -356,24 +341,6 @@ __device__ __forceinline__ HumanDefer human_vec(const EngineView& v, const CommD
 constexpr uint32_t kHumanQueue = 256;
 static_assert(ARBO_MAX_COMMUNITY_HUMANS <= 65536u, "queue entries keep the member index in 16 bits");
 
-// Appends every lane's deferred changes of one vector at qi/qw[base...]:
-// member e of each lane goes to base + (changes of members < e in the wave)
-// + (lanes below with a change of member e).  All lanes call it; `bal` are
-// the wave's ballots of defer bits 0..3 and `before` their running sums.
-__device__ __forceinline__ void human_enqueue(const HumanDefer& d, uint32_t q, uint4 w4, const uint64_t (&bal)[4],
-                                              const uint32_t (&before)[4], uint32_t* qi, uint32_t* qw, uint32_t base) {
-    const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-    for (uint32_t e = 0; e < 4; ++e) {
-        if (!((d.defer >> e) & 1u)) continue;
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[e] >> 32),
-                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bal[e], 0u));
-        const uint32_t pos = base + before[e] + below;
-        qi[pos] = (4u * q + e) | (((d.onset >> e) & 1u) ? d.nxd[e] << 16 : 0u);
-        qw[pos] = ws[e];
-    }
-}
-
 // The deferred changes, one per lane: a death draws the newborn's lifetime
 // (P_H_BIRTH) and rebirths the member; an onset draws the infectious period
 // (P_H_REC) and makes the member infectious.  The same draws and writes as
@@ -381,7 +348,7 @@ __device__ __forceinline__ void human_enqueue(const HumanDefer& d, uint32_t q, u
 template <bool HIST>
 __device__ __forceinline__ void human_deferred(const EngineView& v, const CommDesc& cd, const SimDesc& sd, Key key,
                                                uint32_t t, const uint32_t* qi, const uint32_t* qw, uint32_t n,
-                                               uint32_t lane, HumanCount& hc) {
+                                               uint32_t lane, uint32_t& minev) {
     for (uint32_t k = lane; k < n; k += kWave) {
         const uint32_t e = qi[k], i = e & 0xFFFFu, w = qw[k];
         const uint64_t gi = cd.h_off + i;
@@ -391,14 +358,14 @@ __device__ __forceinline__ void human_deferred(const EngineView& v, const CommDe
             v.h_word[gi] = (t + da) << 8;
             v.h_birth[gi] = (int32_t)t;
             if (HIST && v.hist) reinterpret_cast<int4*>(v.hist)[gi] = make_int4(-1, -1, -1, -1);   // naive newborn
-            hc.minev = t + da < hc.minev ? t + da : hc.minev;
+            minev = t + da < minev ? t + da : minev;
         } else {                   // E -> I
             const U4 r = draw(key, cd.h_base + i, t, P_H_REC, 0);
             const uint32_t nn = (e >> 16) + stoch_round(sd.recovery, r.x);
             v.h_next[gi] = (uint16_t)nn;
             v.h_word[gi] = (w & ~3u) | 2u;
             const uint32_t ev = day_of(t, nn) + 1;
-            hc.minev = ev < hc.minev ? ev : hc.minev;
+            minev = ev < minev ? ev : minev;
         }
     }
 }
@@ -436,11 +403,13 @@ void k_day_demography(EngineView v) {
     const uint32_t mlim = (hs.w == 0u || hs.w - 1u > mvec) ? mvec : hs.w - 1u;
     const uint4* w4p = reinterpret_cast<const uint4*>(v.h_word + cd.h_off);
     const uint4* m4p = reinterpret_cast<const uint4*>(v.m_word + cd.m_off);
+    // lanes past the last vector of humans read that vector again (its words
+    // are not counted), so every lane tests defined words
     uint4 hb[UNROLL], mb[EARLY_M];
 #pragma unroll
     for (uint32_t u = 0; u < UNROLL; ++u) {
         const uint32_t q = u * NT + tid;
-        if (scan_h && q < nvec) hb[u] = w4p[q];
+        if (scan_h) hb[u] = w4p[umin(q, nvec - 1)];
     }
 #pragma unroll
     for (uint32_t u = 0; u < EARLY_M; ++u) {
@@ -463,6 +432,10 @@ void k_day_demography(EngineView v) {
     __shared__ uint32_t s_n[8];
     __shared__ uint32_t s_hist[kCntPerComm];
     uint8_t* s_dead8 = reinterpret_cast<uint8_t*>(s_dead);
+    // E -> I days of each lane's onsets in the vector being scanned: the
+    // dead-slot prefix array, not used before the mosquito scan
+    uint16_t* s_onx = reinterpret_cast<uint16_t*>(s_dpre);
+    static_assert(4 * kWave * sizeof(uint16_t) <= kWaveWords * sizeof(uint32_t), "onset days fit the prefix array");
     uint8_t* s_died8 = reinterpret_cast<uint8_t*>(s_died);
 
     // Every small operand of the day goes out now, behind the first census
@@ -508,10 +481,6 @@ void k_day_demography(EngineView v) {
 
     // ---- humans ----
     HumanCount hc{0, 0, 0, 0, 0xFFFFFFFFu};
-    if (!scan_h && tid == 0) {
-        hc.ih01 = hs.y;
-        hc.ih23 = hs.z;
-    }
     // Changes that need a random draw queue up in LDS (the dead-slot bitmap
     // arrays, not used before the mosquito scan; one queue per wave) and are
     // processed a lane each when the queue could overflow and after the scan.
@@ -532,25 +501,66 @@ void k_day_demography(EngineView v) {
         }
     };
     static_assert(kWaveWords >= kHumanQueue, "each wave's human queue fits its share of the bitmap arrays");
-    auto scan_vector = [&](uint32_t q, uint4 w4) {   // every lane calls it; lanes past nvec add nothing
-        HumanDefer d{0u, 0u, {0u, 0u, 0u, 0u}};
-        if (q < nvec) d = human_vec(v, cd, t, q, w4, hc);
+    // Humans of ndemographic.cu:57-103, four to a lane (one 16-byte vector of
+    // packed words, index 4q..4q+3).  A member dies today iff its word is
+    // below tt (death day <= t; t < 2^23, so no overflow).  The words past
+    // the community's last member in its last vector read as kPadWord: a
+    // member that neither dies nor changes state.  Only a vector with an
+    // exposed or infectious member reads its next-event days
+    // (human_states).  The changes that need a random draw (a death with its
+    // rebirth's lifetime, an E -> I onset with its infectious period) are
+    // queued: member e of each lane goes to hq_n + (changes of members < e in
+    // the wave) + (lanes below with a change of member e).
+    constexpr uint32_t kPadWord = 0xFFFFFF00u;
+    const uint32_t tt = (t + 1u) << 8;
+    const uint32_t nlast = (cd.n & 3u) ? cd.n >> 2 : 0xFFFFFFFFu;   // the vector with fewer than four members
+    uint32_t minw = 0xFFFFFFFFu;   // smallest word of the lane's members that survive today
+    auto scan_vector = [&](uint32_t q, uint4 w4) {   // every lane calls it; lanes past nvec count nothing
+        uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+        const bool in = q < nvec;
+        if (q == nlast) {
+#pragma unroll
+            for (uint32_t e = 1; e < 4; ++e)
+                if (4 * q + e >= cd.n) ws[e] = kPadWord;
+        }
+        // deaths, straight from the compares; the smallest surviving word
         uint64_t bal[4];
+        const uint64_t bin = __builtin_amdgcn_ballot_w64(in);
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) bal[e] = __builtin_amdgcn_ballot_w64(ws[e] < tt) & bin;
+        const uint32_t m01 = umin(ws[0] < tt ? 0xFFFFFFFFu : ws[0], ws[1] < tt ? 0xFFFFFFFFu : ws[1]);
+        const uint32_t m23 = umin(ws[2] < tt ? 0xFFFFFFFFu : ws[2], ws[3] < tt ? 0xFFFFFFFFu : ws[3]);
+        minw = in ? umin(minw, umin(m01, m23)) : minw;
+        // exposed and infectious members (rare): their onsets join the deaths
+        uint32_t ons = 0;
+        if (in && ((ws[0] | ws[1] | ws[2] | ws[3]) & 3u) != 0u)
+            ons = human_states(v, cd, t, tt, q, ws, hc, s_onx + 4 * tid);
+        if (__builtin_amdgcn_ballot_w64(ons != 0u)) {
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) bal[e] |= __builtin_amdgcn_ballot_w64((ons >> e) & 1u);
+        }
         uint32_t before[4], many = 0;
 #pragma unroll
         for (uint32_t e = 0; e < 4; ++e) {
-            bal[e] = __ballot((d.defer >> e) & 1u);
             before[e] = many;
             many += (uint32_t)__popcll(bal[e]);
         }
         if (many == 0) return;
         if (hq_n + many > kHumanQueue) {
             queue_sync();
-            human_deferred<HIST>(v, cd, sd, key, t, hq_i, hq_w, hq_n, lane, hc);
+            human_deferred<HIST>(v, cd, sd, key, t, hq_i, hq_w, hq_n, lane, hc.minev);
             queue_sync();
             hq_n = 0;
         }
-        human_enqueue(d, q, w4, bal, before, hq_i, hq_w, hq_n);
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+            if (!((in && ws[e] < tt) || ((ons >> e) & 1u))) continue;
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal[e] >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)bal[e], 0u));
+            const uint32_t pos = hq_n + before[e] + below;
+            hq_i[pos] = (4u * q + e) | (((ons >> e) & 1u) ? (uint32_t)s_onx[4 * tid + e] << 16 : 0u);
+            hq_w[pos] = ws[e];
+        }
         hq_n += many;
     };
     if (scan_h) {
@@ -559,16 +569,13 @@ void k_day_demography(EngineView v) {
         for (uint32_t q0 = UNROLL * NT; q0 < nvec; q0 += UNROLL * NT) {
             uint4 w4[UNROLL];
 #pragma unroll
-            for (uint32_t u = 0; u < UNROLL; ++u) {
-                const uint32_t q = q0 + u * NT + tid;
-                if (q < nvec) w4[u] = w4p[q];
-            }
+            for (uint32_t u = 0; u < UNROLL; ++u) w4[u] = w4p[umin(q0 + u * NT + tid, nvec - 1)];
 #pragma unroll
             for (uint32_t u = 0; u < UNROLL; ++u) scan_vector(q0 + u * NT + tid, w4[u]);
         }
         if (hq_n) {
             queue_sync();
-            human_deferred<HIST>(v, cd, sd, key, t, hq_i, hq_w, hq_n, lane, hc);
+            human_deferred<HIST>(v, cd, sd, key, t, hq_i, hq_w, hq_n, lane, hc.minev);
             queue_sync();
         }
     }
@@ -576,14 +583,21 @@ void k_day_demography(EngineView v) {
 
     // ---- mosquitoes: death and E->I (mdemographic.cu:68-98), eight slots per u32 op ----
     // A 16-byte vector holds slots 0..7 as four u32 pairs.  The status and
-    // strain fields of all eight are gathered into one word each, slot e at
-    // bit position 16*(e&1) + 2*(e>>1), and tested together.
+    // strain fields and the death test of all eight are gathered into one
+    // word each, slot e at bit position 16*(e&1) + 2*(e>>1), and tested
+    // together.
     uint32_t im01 = 0, im23 = 0, alive = 0;
     const uint32_t nwords = (cd.cap + 31) >> 5;
     {
-        // per 16-bit half: t mod 4096 plus a 4096 guard, so (t - death day) mod 4096
-        // is the half's low 12 bits with no borrow between halves
-        const uint32_t T2 = ((t & 0xFFFu) * 0x00010001u) | 0x10001000u;
+        // per 16-bit half x = death day << 4 | strain << 2 | status:
+        // ((t mod 4096) << 4 | 15) - x = ((t - death day) mod 4096) << 4 | (15 -
+        // the low nibble) mod 2^16, whose bit 15 is clear iff (t - death day)
+        // mod 4096 < 2048, i.e. the slot's death day is today or past
+        const uint32_t Th = ((((t & 0xFFFu) << 4) | 0xFu) * 0x00010001u);
+        // the valid slots of a vector: all eight below mfull, else those below cap
+        const uint32_t mfull = cd.cap >> 3, pb = (1u << (cd.cap & 7u)) - 1u;
+        const uint32_t mpart = (pb & 0x55u) | ((pb & 0xAAu) << 15);
+        uint32_t ic[4] = {0u, 0u, 0u, 0u};   // infectious slots per serotype
         for (uint32_t q0 = 0; q0 < mlim; q0 += UNROLL * NT) {
             uint4 raw[UNROLL];
             if (q0 == 0) {
@@ -605,20 +619,17 @@ void k_day_demography(EngineView v) {
                 const uint32_t q = q0 + u * NT + tid;
                 if (q >= mlim) continue;
                 const uint32_t rw[4] = {raw[u].x, raw[u].y, raw[u].z, raw[u].w};
-                uint32_t st = 0, sn = 0, dy = 0;
+                uint32_t st = 0, ndy = 0;
 #pragma unroll
                 for (uint32_t p = 0; p < 4; ++p) {
                     const uint32_t x = rw[p];
                     st |= (x & 0x00030003u) << (2 * p);
-                    sn |= ((x >> 2) & 0x00030003u) << (2 * p);
-                    dy |= ((~(T2 - ((x >> 4) & 0x0FFF0FFFu)) >> 11) & 0x00010001u) << (2 * p);
+                    ndy |= (sub_u16x2(Th, x) >> (15 - 2 * p)) & (0x00010001u << (2 * p));
                 }
-                const uint32_t left = cd.cap - 8 * q;
-                const uint32_t vbyte = left >= 8 ? 0xFFu : ((1u << left) - 1u);
-                const uint32_t valid = (vbyte & 0x55u) | ((vbyte & 0xAAu) << 15);
+                const uint32_t valid = q < mfull ? 0x00550055u : mpart;
                 const uint32_t b0 = st & 0x00550055u, b1 = (st >> 1) & 0x00550055u;
                 const uint32_t empty = b0 & b1;                       // status 3
-                const uint32_t died = dy & ~empty & valid;
+                const uint32_t died = ~ndy & ~empty & valid;
                 const uint32_t dead = (empty | died) & valid;
                 const uint32_t live = valid & ~dead;
                 uint32_t em = b0 & ~b1 & live;                        // status 1
@@ -638,14 +649,23 @@ void k_day_demography(EngineView v) {
                         }
                     }
                 }
-                const uint32_t sa = sn & 0x00550055u, sb = (sn >> 1) & 0x00550055u;
                 alive += __popc(live);
-                im01 += __popc(im & ~sa & ~sb) + (__popc(im & sa & ~sb) << 16);
-                im23 += __popc(im & ~sa & sb) + (__popc(im & sa & sb) << 16);
+                if (im) {
+                    uint32_t sn = 0;
+#pragma unroll
+                    for (uint32_t p = 0; p < 4; ++p) sn |= ((rw[p] >> 2) & 0x00030003u) << (2 * p);
+                    const uint32_t sa = sn & 0x00550055u, sb = (sn >> 1) & 0x00550055u;
+                    ic[0] += __popc(im & ~sa & ~sb);
+                    ic[1] += __popc(im & sa & ~sb);
+                    ic[2] += __popc(im & ~sa & sb);
+                    ic[3] += __popc(im & sa & sb);
+                }
                 s_dead8[q] = (uint8_t)((dead & 0x55u) | ((dead >> 15) & 0xAAu));
                 s_died8[q] = (uint8_t)((died & 0x55u) | ((died >> 15) & 0xAAu));
             }
         }
+        im01 = ic[0] | (ic[1] << 16);   // a lane's counts are below 2^16
+        im23 = ic[2] | (ic[3] << 16);
         // the empty slots not read (all dead, none died today) and the bytes
         // past the capacity in the last bitmap word
         for (uint32_t q = mlim + tid; q < 4 * nwords; q += NT) {
@@ -654,11 +674,21 @@ void k_day_demography(EngineView v) {
             s_died8[q] = 0;
         }
     }
-    uint32_t ih01 = wave_sum(hc.ih01), ih23 = wave_sum(hc.ih23);
-    uint32_t inc01 = wave_sum(hc.inc01), inc23 = wave_sum(hc.inc23);
+    // the wave's human counts (pairs of 16-bit serotype counts; a wave's
+    // counts are below 2^16) and earliest event day
+    uint32_t ih01, ih23, inc01, inc23, minev;
+    if (scan_h) {
+        ih01 = wave_sum(hc.ih01); ih23 = wave_sum(hc.ih23);
+        inc01 = wave_sum(hc.inc01); inc23 = wave_sum(hc.inc23);
+        minev = wave_min(umin(hc.minev, minw >> 8));
+    } else {
+        ih01 = wv == 0 ? hs.y : 0u;
+        ih23 = wv == 0 ? hs.z : 0u;
+        inc01 = inc23 = 0;
+        minev = hs.x;
+    }
     im01 = wave_sum(im01);
     im23 = wave_sum(im23);
-    uint32_t minev = scan_h ? wave_min(hc.minev) : hs.x;
     uint32_t alive_c = wave_sum(alive);
     if (NW > 1) {   // the waves' counts through LDS (16-bit pair halves cannot carry: each is below 2^16)
         if (lane == 0) {
