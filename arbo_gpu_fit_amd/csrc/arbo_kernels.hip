@@ -423,7 +423,7 @@ void k_day_demography(EngineView v) {
     extern __shared__ uint32_t s_dtab[];                            // mosquito lifetime thresholds
     __shared__ uint32_t s_dead[kWaveWords * NW];                    // dead-slot bitmap
     __shared__ uint32_t s_died[kWaveWords * NW];                    // died today
-    __shared__ uint32_t s_dpre[kWaveWords * NW];                    // dead slots before each word
+    __shared__ __align__(16) uint32_t s_dpre[kWaveWords * NW];      // dead slots before each word
     __shared__ uint32_t s_red[NW][8];                               // per-wave counts (NW > 1)
     __shared__ uint32_t s_wtot[NW];                                 // per-wave dead slots (NW > 1)
     __shared__ uint32_t s_wtop[NW];                                 // per-wave high-water marks (NW > 1)
@@ -594,6 +594,7 @@ void k_day_demography(EngineView v) {
         // the low nibble) mod 2^16, whose bit 15 is clear iff (t - death day)
         // mod 4096 < 2048, i.e. the slot's death day is today or past
         const uint32_t Th = ((((t & 0xFFFu) << 4) | 0xFu) * 0x00010001u);
+        const uint32_t T16 = (t & 0xFFFFu) * 0x00010001u;   // t mod 2^16 in both halves
         // the valid slots of a vector: all eight below mfull, else those below cap
         const uint32_t mfull = cd.cap >> 3, pb = (1u << (cd.cap & 7u)) - 1u;
         const uint32_t mpart = (pb & 0x55u) | ((pb & 0xAAu) << 15);
@@ -637,16 +638,22 @@ void k_day_demography(EngineView v) {
                 if (em) {  // E -> I when the extrinsic incubation is over; one 16-byte read of the next days
                     const uint4 nx4 = *reinterpret_cast<const uint4*>(v.m_next + cd.m_off + 8 * q);
                     const uint32_t nw[4] = {nx4.x, nx4.y, nx4.z, nx4.w};
-                    while (em) {
-                        const uint32_t b = __ffs(em) - 1;
-                        em &= em - 1;
+                    // due: (int16)(t - next) > 0, i.e. bit 15 clear in both d = t - next and
+                    // d - 1 (per 16-bit half, mod 2^16), for all eight slots at once
+                    uint32_t due = 0;
+#pragma unroll
+                    for (uint32_t p = 0; p < 4; ++p) {
+                        const uint32_t d = sub_u16x2(T16, nw[p]), d1 = sub_u16x2(d, 0x00010001u);
+                        due |= (~(d | d1) >> (15 - 2 * p)) & (0x00010001u << (2 * p));
+                    }
+                    uint32_t go = em & due;
+                    im |= go;
+                    while (go) {
+                        const uint32_t b = __ffs(go) - 1;
+                        go &= go - 1;
                         const uint32_t e = b < 16 ? b : b - 15;           // slot within the vector
-                        const uint32_t nx = (nw[e >> 1] >> ((e & 1u) * 16u)) & 0xFFFFu;
-                        if ((int16_t)(uint16_t)(t - nx) > 0) {
-                            const uint32_t w = (rw[e >> 1] >> ((e & 1u) * 16u)) & 0xFFFFu;
-                            v.m_word[cd.m_off + 8 * q + e] = (uint16_t)((w & ~3u) | 2u);
-                            im |= 1u << b;
-                        }
+                        const uint32_t w = (rw[e >> 1] >> ((e & 1u) * 16u)) & 0xFFFFu;
+                        v.m_word[cd.m_off + 8 * q + e] = (uint16_t)((w & ~3u) | 2u);
                     }
                 }
                 alive += __popc(live);
@@ -667,7 +674,8 @@ void k_day_demography(EngineView v) {
         im01 = ic[0] | (ic[1] << 16);   // a lane's counts are below 2^16
         im23 = ic[2] | (ic[3] << 16);
         // the empty slots not read (all dead, none died today) and the bytes
-        // past the capacity in the last bitmap word
+        // past the capacity in the last bitmap word (a few iterations at most)
+#pragma unroll 1
         for (uint32_t q = mlim + tid; q < 4 * nwords; q += NT) {
             const uint32_t left = q < mvec ? cd.cap - 8 * q : 0u;
             s_dead8[q] = (uint8_t)(left >= 8 ? 0xFFu : ((1u << left) - 1u));
@@ -734,12 +742,24 @@ void k_day_demography(EngineView v) {
 #pragma unroll
             for (uint32_t w = 0; w < NW; ++w) pre += w < wv ? s_wtot[w] : 0u;
         }
+        // every thread's four prefix entries are written (words past nwords
+        // hold the total, above every rank), so a thread's four can be read
+        // as one vector
 #pragma unroll
         for (uint32_t k = 0; k < kWordsPerLane; ++k) {
-            const uint32_t w = kWordsPerLane * tid + k;
             wpre[k] = pre;
-            if (w < nwords) s_dpre[w] = pre;
+            s_dpre[kWordsPerLane * tid + k] = pre;
             pre += __popc(dw[k]);
+        }
+        if (NW == 1) {
+            // Owners of the first 64 birth quads: quad g starts at rank 4g, in
+            // the words of the lane whose ranks [wpre[0], pre) hold it.  Lane L
+            // marks s_own[g] = L + 1 at the first quad that starts in its words
+            // (if any); quad g's owner is the last mark at or before g, since
+            // the owners ascend with g.  s_died is free: kw[] holds its words.
+            s_died[lane] = 0u;
+            const uint32_t qa = (wpre[0] + 3) >> 2;
+            if (4 * qa < pre && qa < kWave) s_died[qa] = lane + 1;
         }
     }
     __syncthreads();
@@ -748,7 +768,20 @@ void k_day_demography(EngineView v) {
         const U4 r = draw(key, cd.c, t, P_M_BIRTH, g4);
         const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
         uint32_t w, pos;
-        select_bit(4 * g4, s_dead, s_dpre, nwords, w, pos);
+        if (NW == 1 && g4 < kWave) {   // the first 64 quads (one per lane): from the owner marks
+            const uint64_t marked = __builtin_amdgcn_ballot_w64(s_died[g4] != 0u);
+            const uint64_t upto = marked & (~0ull >> (63 - g4));   // marks at quads <= g4 (quad 0 is marked)
+            const uint32_t j = 63u - (uint32_t)__clzll(upto);
+            const uint32_t o = s_died[j] - 1;                        // the owner lane
+            const uint4 p = reinterpret_cast<const uint4*>(s_dpre)[o];
+            const uint32_t rk = 4 * g4;
+            const uint32_t k = (p.y <= rk ? 1u : 0u) + (p.z <= rk ? 1u : 0u) + (p.w <= rk ? 1u : 0u);
+            const uint32_t pk = k == 0 ? p.x : k == 1 ? p.y : k == 2 ? p.z : p.w;
+            w = kWordsPerLane * o + k;
+            pos = select_in_word(s_dead[w], rk - pk);
+        } else {
+            select_bit(4 * g4, s_dead, s_dpre, nwords, w, pos);
+        }
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
             if (4 * g4 + j >= births) break;
