@@ -11,7 +11,9 @@ must agree bit for bit:
 * the 100x100 lattice (1e7 humans, 3,650 days configured) at days 1,000 and
   3,000, in the endemic regime;
 * the three simulations of the 256-simulation batch with the most
-  communities (10,000 each), inside the full batch, at days 400 and 1,500;
+  communities (10,000 each), inside the full batch, at day 120 (the
+  epidemic peak of the batch, where most communities hold exposed and
+  infectious humans and mosquitoes every day) and at days 400 and 1,500;
 * runs across the wraps of the packed day fields: mosquito death days are
   kept mod 4,096 and the human and mosquito next-event days mod 65,536.
 """
@@ -68,14 +70,16 @@ def test_batch_largest_simulations_late_windows_bit_exact(hip_lib, oracle_lib):
     gpu = Engine(hip_lib, flags=abi.F_GRAPH)
     gpu.init(specs, T)
     done = 0
-    for day, days in ((400, 20), (1500, 20)):
+    for day, days in ((120, 10), (400, 20), (1500, 20)):
         print(f"[late] batch of {len(specs)} to day {day}; simulations {big} "
               f"({[specs[i].communities for i in big]} communities) {days} days against the oracle", flush=True)
         gpu.step(day - done)
         _resume_and_compare(gpu, specs, big, day, days, T, oracle_lib, "batch")
         done = day + days
     for i in big:
-        assert gpu.series(i)["n_incidence"][:400].sum() > 0
+        ser = gpu.series(i)
+        assert ser["n_incidence"][121:131].sum() > 0   # the day-120 window is inside the epidemic
+        assert ser["n_incidence"][:400].sum() > 0
     gpu.close()
 
 
