@@ -98,17 +98,21 @@ __device__ __forceinline__ uint32_t sub_u16x2(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
 }
 
-// Exclusive prefix sum over the wavefront; `total` gets the wave sum.
+// Exclusive prefix sum over the wavefront, every lane active; `total` gets
+// the wave sum.  An inclusive scan by DPP: row_shr 1, 2, 4, 8 within each
+// row of 16 lanes (lanes shifted in from outside the row add 0), then lane
+// 15 of rows 0 and 2 added to rows 1 and 3 (row_bcast:15) and lane 31 to
+// rows 2 and 3 (row_bcast:31).
 __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, uint32_t& total) {
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    uint32_t incl = x;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(incl, o, 64);
-        if (lane >= (uint32_t)o) incl += y;
-    }
-    total = __shfl(incl, 63, 64);
-    return incl - x;
+    uint32_t v = x;
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);   // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);   // row_bcast:31
+    total = (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+    return v - x;
 }
 
 // min{a in [1, hi] : K[a] >= k}; hi when K[hi] < k (hi = table end).
