@@ -7,10 +7,13 @@ are compared bit for bit as well.  Draws use numpy's PCG64 with the case
 seed; the same seed gives the same simulations everywhere."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from arbo_gpu_fit_amd import abi, configs
-from arbo_gpu_fit_amd.engine import ClimateSpec, SimSpec
+from arbo_gpu_fit_amd.engine import ClimateSpec, SimSpec, climate_day
+from tests.oracle_lib import load_oracle
 
 N_RANDOM = 12
 
@@ -47,6 +50,14 @@ def random_sim(rng: np.random.Generator, seed: int, days: int) -> SimSpec:
                                            biting_rate=float(rng.uniform(0.3, 1.2)),
                                            mos_net_growth=float(rng.uniform(0.0, 0.6)),
                                            min_mos_to_human=float(rng.uniform(0.5, 2.0)))
+        if p.m_shape == 1.0:
+            # exponential lifetimes under a cold climate (scale above about 120
+            # days) outlast the 2,047-day lifetime table, which the engine
+            # rejects: such a draw gets shape 4 (no extra random draw, so the
+            # other cases are unchanged)
+            smax = max(climate_day(climate, r, load_oracle())[3] for r in range(1, days + 1))
+            if -math.expm1(-2046.0 / smax) < 1.0 - 2.0 ** -24.5:
+                p.m_shape = 4.0
     share = None
     if C > 1 and rng.random() < 0.3:
         w = rng.uniform(0.2, 1.0, C)
